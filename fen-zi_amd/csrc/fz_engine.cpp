@@ -1,0 +1,876 @@
+// fz_engine.cpp -- host runtime of libfenzi.
+//
+// Replaces the reference's InitHostMem / InitMem / InitDeviceConstants /
+// GenExcl_minimal_new / build_constraints (src/prmread.cu:86,1303,2928;
+// src/mem.cu:49; src/cell_based_kernels.cu:1831) and the step orchestration of
+// SingleStep / ComputeAccelGPU (src/dynamics.cu:623, src/mdcuda.cu:557).
+// Capacities that the reference fixes at compile time (CELL_ATOMS, MAXNB,
+// MAXLATTICE; src/configure:89-137) are sized from the input at run time and
+// grown on overflow, after which the interrupted steps are replayed from a
+// device-side snapshot.
+#include "fz_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+namespace fz {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+const char* last_error() { return g_err.c_str(); }
+
+static const double kPi = 3.14159265358979323846;
+static const double kCC = 332.0716, kTU = 20.4548, kRC = 0.001987;
+static const double kTwo64 = 18446744073709551616.0;
+
+void launch_zero_forces(const Launch& L);
+
+// ---------------------------------------------------------------------------
+// topology processing
+
+static bool smooth235(int n) {
+  if (n < 1) return false;
+  for (int p : {2, 3, 5})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+static int uf_find(std::vector<int>& p, int x) {
+  while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
+  return x;
+}
+
+static inline uint64_t pkey(int a, int b, int n) {
+  int lo = std::min(a, b), hi = std::max(a, b);
+  return (uint64_t)lo * (uint64_t)n + (uint64_t)hi;
+}
+
+static int process_topology(const fz_topology* t, HostTopo& h) {
+  const int n = t->n_atoms;
+  if (n <= 0) return set_error(FZ_ERR_ARG, "n_atoms must be positive");
+  if (t->n_types <= 0 || t->n_types > 256) return set_error(FZ_ERR_ARG, "n_types must be in 1..256");
+  h.n = n;
+  h.ntypes = t->n_types;
+  h.q.assign(t->charge, t->charge + n);
+  h.mass.assign(t->mass, t->mass + n);
+  h.type.assign(t->type, t->type + n);
+  h.minv.resize(n);
+  h.q2sum = 0.0;
+  for (int i = 0; i < n; i++) {
+    if (!(h.mass[i] > 0.0)) return set_error(FZ_ERR_ARG, "atom %d has non-positive mass", i);
+    if (h.type[i] < 0 || h.type[i] >= t->n_types) return set_error(FZ_ERR_ARG, "atom %d has type out of range", i);
+    h.minv[i] = 1.0 / h.mass[i];
+    h.q2sum += h.q[i] * h.q[i];
+  }
+  const int T2 = t->n_types * t->n_types;
+  h.lj_eps.assign(t->lj_eps, t->lj_eps + T2);
+  h.lj_sig2.assign(t->lj_sig2, t->lj_sig2 + T2);
+  h.lj14_eps.assign(t->lj14_eps, t->lj14_eps + T2);
+  h.lj14_sig2.assign(t->lj14_sig2, t->lj14_sig2 + T2);
+  auto chk = [&](int a) { return a >= 0 && a < n; };
+  for (int i = 0; i < t->n_bonds; i++) {
+    int a = t->bond_atoms[2 * i], b = t->bond_atoms[2 * i + 1];
+    if (!chk(a) || !chk(b)) return set_error(FZ_ERR_ARG, "bond %d atom out of range", i);
+    h.bond_ij.push_back(make_int2(a, b));
+    h.bond_p.push_back(make_double2(t->bond_k[i], t->bond_r0[i]));
+  }
+  for (int i = 0; i < t->n_angles; i++) {
+    int a = t->angle_atoms[3 * i], b = t->angle_atoms[3 * i + 1], c = t->angle_atoms[3 * i + 2];
+    if (!chk(a) || !chk(b) || !chk(c)) return set_error(FZ_ERR_ARG, "angle %d atom out of range", i);
+    h.angle_ijk.push_back(make_int4(a, b, c, 0));
+    h.angle_p.push_back(make_double4(t->angle_k[i], t->angle_theta0[i], t->ub_k ? t->ub_k[i] : 0.0,
+                                     t->ub_s0 ? t->ub_s0[i] : 0.0));
+  }
+  for (int i = 0; i < t->n_dihedrals; i++) {
+    const int32_t* a = t->dihedral_atoms + 4 * i;
+    for (int k = 0; k < 4; k++)
+      if (!chk(a[k])) return set_error(FZ_ERR_ARG, "dihedral %d atom out of range", i);
+    h.dihe_ijkl.push_back(make_int4(a[0], a[1], a[2], a[3]));
+    h.dihe_p.push_back(make_double2(t->dihedral_k[i], t->dihedral_delta[i]));
+    h.dihe_n.push_back(t->dihedral_n[i]);
+  }
+  for (int i = 0; i < t->n_impropers; i++) {
+    const int32_t* a = t->improper_atoms + 4 * i;
+    for (int k = 0; k < 4; k++)
+      if (!chk(a[k])) return set_error(FZ_ERR_ARG, "improper %d atom out of range", i);
+    h.impr_ijkl.push_back(make_int4(a[0], a[1], a[2], a[3]));
+    h.impr_k.push_back(t->improper_k[i]);
+  }
+  // exclusions (src/prmread.cu:2928-3062): 1-2 bonds, 1-3 angle ends -> Ewald
+  // correction set; 1-4 dihedral ends -> unique 1-4 pairs not already 1-2/1-3
+  std::vector<uint64_t> ew, p14;
+  for (auto& b : h.bond_ij)
+    if (b.x != b.y) ew.push_back(pkey(b.x, b.y, n));
+  for (auto& a : h.angle_ijk)
+    if (a.x != a.z) ew.push_back(pkey(a.x, a.z, n));
+  std::sort(ew.begin(), ew.end());
+  ew.erase(std::unique(ew.begin(), ew.end()), ew.end());
+  for (auto& d : h.dihe_ijkl) {
+    if (d.x == d.w) continue;
+    uint64_t k = pkey(d.x, d.w, n);
+    if (!std::binary_search(ew.begin(), ew.end(), k)) p14.push_back(k);
+  }
+  std::sort(p14.begin(), p14.end());
+  p14.erase(std::unique(p14.begin(), p14.end()), p14.end());
+  for (uint64_t k : ew) h.ew_ij.push_back(make_int2((int)(k / n), (int)(k % n)));
+  for (uint64_t k : p14) h.p14_ij.push_back(make_int2((int)(k / n), (int)(k % n)));
+  h.excl_bits.assign(n, 0ull);
+  std::vector<std::vector<int>> ex(n);
+  bool wide = false;
+  auto add_ex = [&](int a, int b) {
+    ex[a].push_back(b);
+    ex[b].push_back(a);
+    int off = b - a;
+    if (off >= -32 && off <= 31) h.excl_bits[a] |= 1ull << (off + 32); else wide = true;
+    off = a - b;
+    if (off >= -32 && off <= 31) h.excl_bits[b] |= 1ull << (off + 32); else wide = true;
+  };
+  for (auto& p : h.ew_ij) add_ex(p.x, p.y);
+  for (auto& p : h.p14_ij) add_ex(p.x, p.y);
+  h.use_excl_list = wide ? 1 : 0;
+  h.excl_off.assign(n + 1, 0);
+  for (int i = 0; i < n; i++) {
+    std::sort(ex[i].begin(), ex[i].end());
+    h.excl_off[i + 1] = h.excl_off[i] + (int)ex[i].size();
+  }
+  h.excl_list.clear();
+  h.excl_list.reserve(h.excl_off[n] + 1);
+  for (int i = 0; i < n; i++) h.excl_list.insert(h.excl_list.end(), ex[i].begin(), ex[i].end());
+  if (h.excl_list.empty()) h.excl_list.push_back(-1);
+  // constraint clusters (src/prmread.cu:1192-1303)
+  h.n_constraints = t->n_constraints;
+  if (t->n_constraints > 0) {
+    std::vector<int> par(n);
+    std::iota(par.begin(), par.end(), 0);
+    for (int c = 0; c < t->n_constraints; c++) {
+      int a = t->constraint_atoms[2 * c], b = t->constraint_atoms[2 * c + 1];
+      if (!chk(a) || !chk(b) || a == b) return set_error(FZ_ERR_ARG, "constraint %d atoms invalid", c);
+      int ra = uf_find(par, a), rb = uf_find(par, b);
+      if (ra != rb) par[ra] = rb;
+    }
+    std::vector<int> cid(n, -1);
+    struct Cl { int at[4]; int na; int c1[3], c2[3]; double d2[3]; int nc; };
+    std::vector<Cl> cls;
+    for (int c = 0; c < t->n_constraints; c++) {
+      int a = t->constraint_atoms[2 * c], b = t->constraint_atoms[2 * c + 1];
+      int r = uf_find(par, a);
+      if (cid[r] < 0) {
+        cid[r] = (int)cls.size();
+        Cl z;
+        memset(&z, 0, sizeof z);
+        for (int k = 0; k < 4; k++) z.at[k] = -1;
+        cls.push_back(z);
+      }
+      Cl& cl = cls[cid[r]];
+      int la = -1, lb = -1;
+      for (int k = 0; k < cl.na; k++) { if (cl.at[k] == a) la = k; if (cl.at[k] == b) lb = k; }
+      if (la < 0) { if (cl.na >= 4) return set_error(FZ_ERR_ARG, "constraint cluster with more than 4 atoms"); la = cl.na; cl.at[cl.na++] = a; }
+      if (lb < 0) { if (cl.na >= 4) return set_error(FZ_ERR_ARG, "constraint cluster with more than 4 atoms"); lb = cl.na; cl.at[cl.na++] = b; }
+      if (cl.nc >= 3) return set_error(FZ_ERR_ARG, "constraint cluster with more than 3 constraints");
+      cl.c1[cl.nc] = la; cl.c2[cl.nc] = lb;
+      cl.d2[cl.nc] = t->constraint_length[c] * t->constraint_length[c];
+      cl.nc++;
+    }
+    for (auto& cl : cls) {
+      h.ccl_atoms.push_back(make_int4(cl.at[0], cl.at[1], cl.at[2], cl.at[3]));
+      int p[3] = {0, 0, 0};
+      for (int k = 0; k < cl.nc; k++) p[k] = cl.c1[k] | (cl.c2[k] << 8);
+      h.ccl_cons.push_back(make_int4(p[0], p[1], p[2], cl.nc));
+      h.ccl_d2.push_back(make_double4(cl.d2[0], cl.nc > 1 ? cl.d2[1] : 1.0, cl.nc > 2 ? cl.d2[2] : 1.0, 0.0));
+    }
+  }
+  return FZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// System
+
+System::~System() { free_device(); }
+
+void System::free_device() {
+  if (!allocs.empty()) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    for (void* p : allocs) (void)hipFree(p);
+    (void)hipSetDevice(cur);
+  }
+  allocs.clear();
+  bytes = 0;
+}
+
+template <class T>
+int System::dalloc(T** p, size_t count, bool zero) {
+  size_t b = std::max<size_t>(1, count) * sizeof(T);
+  b = (b + 255) & ~(size_t)255;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, b);
+  if (e != hipSuccess) return set_error(FZ_ERR_NOMEM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+  allocs.push_back(q);
+  bytes += b;
+  if (zero) FZ_HIP(hipMemset(q, 0, b));
+  *p = (T*)q;
+  return FZ_OK;
+}
+
+template <class T>
+static int up(T* dst, const std::vector<T>& v) {
+  if (v.empty()) return FZ_OK;
+  FZ_HIP(hipMemcpy(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return FZ_OK;
+}
+
+int System::init(const fz_topology* t, const fz_params* p, int precision, int dev) {
+  device = dev;
+  mixed = precision == FZ_MIXED ? 1 : 0;
+  if (precision != FZ_FP64 && precision != FZ_MIXED) return set_error(FZ_ERR_ARG, "unknown precision %d", precision);
+  par = *p;
+  FZ_TRY(process_topology(t, topo));
+  for (int d = 0; d < 3; d++) {
+    if (!(par.box[d] > 0)) return set_error(FZ_ERR_ARG, "box must be positive");
+    if (!(par.cutoff < 0.5 * par.box[d])) return set_error(FZ_ERR_ARG, "cutoff %.3f must be below half the box (%.3f)", par.cutoff, par.box[d]);
+    if (!smooth235(par.pme_grid[d]) || par.pme_grid[d] > MAX_FFT || par.pme_grid[d] < 4)
+      return set_error(FZ_ERR_ARG, "PME grid %d must be 2,3,5-smooth and in 4..%d", par.pme_grid[d], MAX_FFT);
+  }
+  if (par.pme_order != 0 && par.pme_order != 4) return set_error(FZ_ERR_ARG, "only PME order 4 is supported");
+  if (!(par.cuton >= 0 && par.cuton < par.cutoff)) return set_error(FZ_ERR_ARG, "cuton must be in [0, cutoff)");
+  if (!(par.pairlist >= par.cutoff)) return set_error(FZ_ERR_ARG, "pairlistdist must be >= cutoff");
+  if (!(par.beta > 0)) return set_error(FZ_ERR_ARG, "beta (KAPPa) must be positive");
+  if (!(par.dt_ps > 0)) par.dt_ps = 0.001;
+  if (!(par.shake_tol > 0)) par.shake_tol = 1e-8;
+  self_energy = -kCC * par.beta / std::sqrt(kPi) * topo.q2sum;
+  return alloc_device();
+}
+
+int System::alloc_device() {
+  const HostTopo& h = topo;
+  SysDev& d = dv;
+  memset(&d, 0, sizeof d);
+  const int n = h.n;
+  d.n_atoms = n;
+  d.n_types = h.ntypes;
+  const double V = par.box[0] * par.box[1] * par.box[2];
+  for (int k = 0; k < 3; k++) {
+    d.L[k] = par.box[k];
+    d.sc[k] = par.box[k] / kTwo64;
+    d.isc[k] = kTwo64 / par.box[k];
+    d.sc32[k] = (float)(par.box[k] / 4294967296.0);
+    d.K[k] = par.pme_grid[k];
+  }
+  d.Kxh = d.K[0] / 2 + 1;
+  d.rc2 = par.cutoff * par.cutoff;
+  d.ron2 = par.cuton * par.cuton;
+  d.rl2 = par.pairlist * par.pairlist;
+  double half_skin = 0.5 * (par.pairlist - par.cutoff);
+  d.skin2 = half_skin * half_skin;
+  d.beta = par.beta;
+  d.sw_inv = 1.0 / std::pow(d.rc2 - d.ron2, 3);
+  d.dt = kTU * par.dt_ps;
+  d.shake_tol = par.shake_tol;
+  d.pme_pre = kCC / (kPi * V);
+  d.pi2_beta2 = kPi * kPi / (par.beta * par.beta);
+  // column grid: columns about cbrt(8 / rho) wide so that a cluster of 8 is
+  // roughly cubic
+  const double rho = n / V;
+  const double w = std::cbrt(CLUSTER / rho);
+  int ncx = par.column_grid[0] > 0 ? par.column_grid[0] : std::max(1, (int)std::floor(par.box[0] / w));
+  int ncy = par.column_grid[1] > 0 ? par.column_grid[1] : std::max(1, (int)std::floor(par.box[1] / w));
+  while ((double)n / (ncx * ncy) > 0.5 * MAX_COLUMN_ATOMS) { ncx++; ncy++; }
+  d.ncx = ncx; d.ncy = ncy; d.ncol = ncx * ncy;
+  d.kx_range = (int)std::ceil(par.pairlist / (par.box[0] / ncx)) + 1;
+  d.ky_range = (int)std::ceil(par.pairlist / (par.box[1] / ncy)) + 1;
+  d.colw_x = (unsigned long long)(kTwo64 / ncx);
+  d.colw_y = (unsigned long long)(kTwo64 / ncy);
+  d.use_excl_list = h.use_excl_list;
+  d.n_ccl = (int)h.ccl_atoms.size();
+  d.n_bonds = (int)h.bond_ij.size();
+  d.n_angles = (int)h.angle_ijk.size();
+  d.n_diheds = (int)h.dihe_ijkl.size();
+  d.n_imprs = (int)h.impr_ijkl.size();
+  d.n_ew = (int)h.ew_ij.size();
+  d.n_p14 = (int)h.p14_ij.size();
+  // per-atom state
+  FZ_TRY(dalloc(&d.ux, n)); FZ_TRY(dalloc(&d.uy, n)); FZ_TRY(dalloc(&d.uz, n));
+  FZ_TRY(dalloc(&d.rx, n)); FZ_TRY(dalloc(&d.ry, n)); FZ_TRY(dalloc(&d.rz, n));
+  FZ_TRY(dalloc(&d.ox, n)); FZ_TRY(dalloc(&d.oy, n)); FZ_TRY(dalloc(&d.oz, n));
+  FZ_TRY(dalloc(&d.vx, n)); FZ_TRY(dalloc(&d.vy, n)); FZ_TRY(dalloc(&d.vz, n));
+  FZ_TRY(dalloc(&d.q, n)); FZ_TRY(dalloc(&d.minv, n)); FZ_TRY(dalloc(&d.type, n));
+  FZ_TRY(up(d.q, h.q)); FZ_TRY(up(d.minv, h.minv)); FZ_TRY(up(d.type, h.type));
+  FZ_TRY(dalloc(&d.atom_slot, n)); FZ_TRY(dalloc(&d.atom_col, n));
+  FZ_TRY(dalloc(&d.col_count, d.ncol)); FZ_TRY(dalloc(&d.col_fill, d.ncol)); FZ_TRY(dalloc(&d.col_clu0, d.ncol + 1));
+  FZ_TRY(dalloc(&d.n_clusters, 1)); FZ_TRY(dalloc(&d.n_pairs, 1));
+  FZ_TRY(dalloc(&d.excl_bits, n)); FZ_TRY(up(d.excl_bits, h.excl_bits));
+  FZ_TRY(dalloc(&d.excl_off, n + 1)); FZ_TRY(up(d.excl_off, h.excl_off));
+  FZ_TRY(dalloc(&d.excl_list, h.excl_list.size())); FZ_TRY(up(d.excl_list, h.excl_list));
+  FZ_TRY(dalloc(&d.bond_ij, h.bond_ij.size())); FZ_TRY(up(d.bond_ij, h.bond_ij));
+  FZ_TRY(dalloc(&d.bond_p, h.bond_p.size())); FZ_TRY(up(d.bond_p, h.bond_p));
+  FZ_TRY(dalloc(&d.angle_ijk, h.angle_ijk.size())); FZ_TRY(up(d.angle_ijk, h.angle_ijk));
+  FZ_TRY(dalloc(&d.angle_p, h.angle_p.size())); FZ_TRY(up(d.angle_p, h.angle_p));
+  FZ_TRY(dalloc(&d.dihe_ijkl, h.dihe_ijkl.size())); FZ_TRY(up(d.dihe_ijkl, h.dihe_ijkl));
+  FZ_TRY(dalloc(&d.dihe_p, h.dihe_p.size())); FZ_TRY(up(d.dihe_p, h.dihe_p));
+  FZ_TRY(dalloc(&d.dihe_n, h.dihe_n.size())); FZ_TRY(up(d.dihe_n, h.dihe_n));
+  FZ_TRY(dalloc(&d.impr_ijkl, h.impr_ijkl.size())); FZ_TRY(up(d.impr_ijkl, h.impr_ijkl));
+  FZ_TRY(dalloc(&d.impr_k, h.impr_k.size())); FZ_TRY(up(d.impr_k, h.impr_k));
+  FZ_TRY(dalloc(&d.ew_ij, h.ew_ij.size())); FZ_TRY(up(d.ew_ij, h.ew_ij));
+  FZ_TRY(dalloc(&d.p14_ij, h.p14_ij.size())); FZ_TRY(up(d.p14_ij, h.p14_ij));
+  {
+    const int T2 = h.ntypes * h.ntypes;
+    std::vector<float2> l32(T2);
+    std::vector<double2> l64(T2), l14(T2);
+    for (int k = 0; k < T2; k++) {
+      l32[k] = make_float2((float)h.lj_eps[k], (float)h.lj_sig2[k]);
+      l64[k] = make_double2(h.lj_eps[k], h.lj_sig2[k]);
+      l14[k] = make_double2(h.lj14_eps[k], h.lj14_sig2[k]);
+    }
+    FZ_TRY(dalloc(&d.lj32, T2)); FZ_TRY(up(d.lj32, l32));
+    FZ_TRY(dalloc(&d.lj64, T2)); FZ_TRY(up(d.lj64, l64));
+    FZ_TRY(dalloc(&d.lj14, T2)); FZ_TRY(up(d.lj14, l14));
+  }
+  // constraints
+  FZ_TRY(dalloc(&d.ccl_atoms, h.ccl_atoms.size())); FZ_TRY(up(d.ccl_atoms, h.ccl_atoms));
+  FZ_TRY(dalloc(&d.ccl_cons, h.ccl_cons.size())); FZ_TRY(up(d.ccl_cons, h.ccl_cons));
+  FZ_TRY(dalloc(&d.ccl_d2, h.ccl_d2.size())); FZ_TRY(up(d.ccl_d2, h.ccl_d2));
+  // PME
+  {
+    const size_t K3 = (size_t)d.K[0] * d.K[1] * d.K[2];
+    const size_t KH = (size_t)d.Kxh * d.K[1] * d.K[2];
+    const size_t csz = mixed ? 8 : 16;
+    void* p;
+    FZ_TRY(dalloc((unsigned long long**)&p, K3)); d.Qacc = p;
+    FZ_TRY(dalloc((char**)&p, KH * csz)); d.Qc = p;
+    FZ_TRY(dalloc((char**)&p, K3 * (mixed ? 4 : 8))); d.phi = p;
+    const int KS = d.K[0] + d.K[1] + d.K[2];
+    std::vector<double> bsp(KS), ex(KS), mv(KS);
+    std::vector<double2> tw64(KS);
+    std::vector<float2> tw32(KS);
+    int off = 0;
+    for (int ax = 0; ax < 3; ax++) {
+      const int K = d.K[ax];
+      for (int j = 0; j < K; j++) {
+        const int jj = j <= K / 2 ? j : j - K;
+        const double m = jj / par.box[ax];
+        const double c = std::cos(2.0 * kPi * j / K);
+        bsp[off + j] = 9.0 / ((2.0 + c) * (2.0 + c));
+        ex[off + j] = std::exp(-kPi * kPi * m * m / (par.beta * par.beta));
+        mv[off + j] = m;
+        const double a = -2.0 * kPi * j / K;
+        tw64[off + j] = make_double2(std::cos(a), std::sin(a));
+        tw32[off + j] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+      off += K;
+    }
+    FZ_TRY(dalloc(&d.pme_bsp, KS)); FZ_TRY(up(d.pme_bsp, bsp));
+    FZ_TRY(dalloc(&d.pme_ex, KS)); FZ_TRY(up(d.pme_ex, ex));
+    FZ_TRY(dalloc(&d.pme_m, KS)); FZ_TRY(up(d.pme_m, mv));
+    if (mixed) { float2* t; FZ_TRY(dalloc(&t, KS)); FZ_TRY(up(t, tw32)); d.tw = t; }
+    else { double2* t; FZ_TRY(dalloc(&t, KS)); FZ_TRY(up(t, tw64)); d.tw = t; }
+  }
+  FZ_TRY(dalloc(&d.acc, A_NSLOT));
+  FZ_TRY(dalloc(&d.flags, F_NSLOT));
+  FZ_TRY(dalloc(&d.mstep, 1));
+  FZ_TRY(dalloc(&snap, (size_t)6 * n));
+  // neighbour structures
+  const int clusters_cap = n / CLUSTER + d.ncol + 64;
+  const double hw = 0.5 * w;
+  const double est = 4.0 / 3.0 * kPi * std::pow(par.pairlist + 2.0 * hw + 1.0, 3) * rho / CLUSTER * 0.5 + 2.0;
+  long long per = std::min<long long>(clusters_cap, (long long)std::ceil(1.6 * est));
+  return alloc_neighbor((long long)clusters_cap * per + 4096, clusters_cap);
+}
+
+int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
+  SysDev& d = dv;
+  d.n_clusters_cap = clusters_cap;
+  d.n_slots_cap = clusters_cap * CLUSTER;
+  d.n_pairs_cap = pairs_cap;
+  const int ns = d.n_slots_cap;
+  FZ_TRY(dalloc(&d.slot_atom, ns));
+  FZ_TRY(dalloc((char**)&d.xs, (size_t)ns * (mixed ? 16 : 32)));
+  FZ_TRY(dalloc(&d.ts, ns));
+  FZ_TRY(dalloc((unsigned long long**)&d.Fs, (size_t)3 * ns));
+  FZ_TRY(dalloc(&d.bin_tmp, ns));
+  FZ_TRY(dalloc(&d.clu_col, clusters_cap));
+  FZ_TRY(dalloc(&d.bb, (size_t)6 * clusters_cap));
+  FZ_TRY(dalloc(&d.search_cnt, clusters_cap + 1));
+  FZ_TRY(dalloc(&d.search_tmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));
+  FZ_TRY(dalloc(&d.pl_off, clusters_cap + 1));
+  FZ_TRY(dalloc(&d.pl_i, pairs_cap, false));
+  FZ_TRY(dalloc(&d.pl_j, pairs_cap, false));
+  FZ_TRY(dalloc(&d.pl_mask, pairs_cap, false));
+  return FZ_OK;
+}
+
+int System::upload_fixed(const uint64_t* u) {
+  const int n = topo.n;
+  std::vector<unsigned long long> c(n);
+  unsigned long long* dst[3] = {dv.ux, dv.uy, dv.uz};
+  for (int k = 0; k < 3; k++) {
+    for (int i = 0; i < n; i++) c[i] = u[3 * i + k];
+    FZ_HIP(hipMemcpy(dst[k], c.data(), n * sizeof(unsigned long long), hipMemcpyHostToDevice));
+  }
+  need_rebuild = true;
+  return FZ_OK;
+}
+
+int System::upload_positions(const double* xyz) {
+  const int n = topo.n;
+  std::vector<uint64_t> u((size_t)3 * n);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) {
+      double t = xyz[3 * i + k] / par.box[k];
+      t -= std::floor(t);
+      if (!(t < 1.0)) t = 0.0;
+      u[3 * i + k] = (uint64_t)(t * kTwo64);
+    }
+  return upload_fixed(u.data());
+}
+
+int System::upload_velocities(const double* v) {
+  const int n = topo.n;
+  std::vector<double> c(n, 0.0);
+  double* dst[3] = {dv.vx, dv.vy, dv.vz};
+  for (int k = 0; k < 3; k++) {
+    if (v)
+      for (int i = 0; i < n; i++) c[i] = v[3 * i + k];
+    FZ_HIP(hipMemcpy(dst[k], c.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return FZ_OK;
+}
+
+int System::download_state(double* xyz, double* v, uint64_t* u) {
+  const int n = topo.n;
+  if (xyz || u) {
+    std::vector<unsigned long long> c(n);
+    unsigned long long* src[3] = {dv.ux, dv.uy, dv.uz};
+    for (int k = 0; k < 3; k++) {
+      FZ_HIP(hipMemcpy(c.data(), src[k], n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; i++) {
+        if (u) u[3 * i + k] = c[i];
+        if (xyz) {
+          double x = (double)c[i] * dv.sc[k];
+          if (x >= 0.5 * par.box[k]) x -= par.box[k];
+          xyz[3 * i + k] = x;
+        }
+      }
+    }
+  }
+  if (v) {
+    std::vector<double> c(n);
+    double* src[3] = {dv.vx, dv.vy, dv.vz};
+    for (int k = 0; k < 3; k++) {
+      FZ_HIP(hipMemcpy(c.data(), src[k], n * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; i++) v[3 * i + k] = c[i];
+    }
+  }
+  return FZ_OK;
+}
+
+// energy / virial accumulators are int64 fixed point at 2^28 per kcal/mol
+static inline double fx(unsigned long long v) { return (double)(long long)v / 268435456.0; }
+
+int System::energies(fz_energies* e) {
+  memset(e, 0, sizeof *e);
+  const unsigned long long* a = acc_host;
+  e->step = step;
+  e->bond = fx(a[A_BOND]);
+  e->angle = fx(a[A_ANGLE]);
+  e->urey_bradley = fx(a[A_UB]);
+  e->improper = fx(a[A_IMPR]);
+  e->dihedral = fx(a[A_DIHED]);
+  e->vdw = fx(a[A_VDW]);
+  e->elec = fx(a[A_ELEC]);
+  e->pme = fx(a[A_PME]);
+  e->ewald_excl = fx(a[A_EWEXCL]);
+  e->ewald_self = self_energy;
+  e->potential = e->bond + e->angle + e->urey_bradley + e->improper + e->dihedral + e->vdw + e->elec + e->pme +
+                 e->ewald_excl + e->ewald_self;
+  e->kinetic = fx(a[A_KIN]);
+  const double dof = 3.0 * topo.n - 6.0 - topo.n_constraints;
+  e->temperature = dof > 0 ? 2.0 * e->kinetic / (kRC * dof) : 0.0;
+  for (int k = 0; k < 9; k++) e->virial[k] = fx(a[A_VIR + k]);
+  e->pairs_within_cutoff = (int64_t)a[A_NPAIR];
+  e->list_builds = flags_host[F_NBUILD];
+  return FZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+
+Engine::Engine(int device, std::vector<System*> sys) : device_(device), sys_(std::move(sys)) {}
+
+Engine::~Engine() {
+  drop_graphs();
+  for (int* p : d_pre_) (void)hipFree(p);
+  if (d_sys_) (void)hipFree(d_sys_);
+  if (L_.chunk_pre) (void)hipFree(L_.chunk_pre);
+  if (L_.stream) (void)hipStreamDestroy(L_.stream);
+}
+
+void Engine::drop_graphs() {
+  if (g_plain_) (void)hipGraphExecDestroy(g_plain_);
+  if (g_ev_) (void)hipGraphExecDestroy(g_ev_);
+  g_plain_ = g_ev_ = nullptr;
+}
+
+int Engine::make_work(Work& w, const std::vector<int>& blocks) {
+  std::vector<int> pre(blocks.size() + 1, 0);
+  for (size_t s = 0; s < blocks.size(); s++) pre[s + 1] = pre[s] + blocks[s];
+  int* d = nullptr;
+  FZ_HIP(hipMalloc(&d, pre.size() * sizeof(int)));
+  d_pre_.push_back(d);
+  FZ_HIP(hipMemcpy(d, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+  w.pre = d;
+  w.total = pre.back();
+  return FZ_OK;
+}
+
+int Engine::upload_desc() {
+  std::vector<SysDev> h(sys_.size());
+  for (size_t s = 0; s < sys_.size(); s++) h[s] = sys_[s]->dv;
+  FZ_HIP(hipMemcpy(d_sys_, h.data(), h.size() * sizeof(SysDev), hipMemcpyHostToDevice));
+  return FZ_OK;
+}
+
+int Engine::prepare() {
+  FZ_HIP(hipSetDevice(device_));
+  drop_graphs();
+  for (int* p : d_pre_) (void)hipFree(p);
+  d_pre_.clear();
+  const int ns = (int)sys_.size();
+  if (ns == 0) return set_error(FZ_ERR_ARG, "empty system set");
+  if (!L_.stream) FZ_HIP(hipStreamCreateWithFlags(&L_.stream, hipStreamNonBlocking));
+  if (!d_sys_) FZ_HIP(hipMalloc(&d_sys_, ns * sizeof(SysDev)));
+  if (!L_.chunk_pre) FZ_HIP(hipMalloc(&L_.chunk_pre, (ns + 1) * sizeof(int)));
+  FZ_TRY(upload_desc());
+  L_.d_sys = d_sys_;
+  L_.nsys = ns;
+  L_.mixed = sys_[0]->mixed;
+  for (auto* s : sys_)
+    if (s->mixed != L_.mixed) return set_error(FZ_ERR_ARG, "a batch must use one precision");
+  std::vector<int> atoms, slots, cols, clus, terms, ccl, fx_, fy_, fz_;
+  int kmax = 4;
+  for (auto* s : sys_) kmax = std::max({kmax, s->dv.K[0], s->dv.K[1], s->dv.K[2]});
+  const int csz = L_.mixed ? 8 : 16;
+  int NL = 16;
+  while (NL > 1 && (size_t)NL * 2 * kmax * csz > 49152) NL >>= 1;
+  L_.fft_nl = NL;
+  L_.fft_lds = NL * 2 * kmax * csz;
+  for (auto* s : sys_) {
+    const SysDev& d = s->dv;
+    atoms.push_back((d.n_atoms + 255) / 256);
+    slots.push_back((d.n_slots_cap + 255) / 256);
+    cols.push_back(d.ncol);
+    clus.push_back((d.n_clusters_cap + 3) / 4);
+    const int nt = d.n_bonds + d.n_angles + d.n_diheds + d.n_imprs + d.n_ew + d.n_p14;
+    terms.push_back((nt + 255) / 256);
+    ccl.push_back((d.n_ccl + 127) / 128);
+    const int ng = (d.Kxh + NL - 1) / NL;
+    fx_.push_back((d.K[1] * d.K[2] + NL - 1) / NL);
+    fy_.push_back(d.K[2] * ng);
+    fz_.push_back(d.K[1] * ng);
+  }
+  FZ_TRY(make_work(L_.w_atoms, atoms));
+  FZ_TRY(make_work(L_.w_slots, slots));
+  FZ_TRY(make_work(L_.w_cols, cols));
+  FZ_TRY(make_work(L_.w_clus, clus));
+  FZ_TRY(make_work(L_.w_terms, terms));
+  FZ_TRY(make_work(L_.w_ccl, ccl));
+  FZ_TRY(make_work(L_.w_fftx, fx_));
+  FZ_TRY(make_work(L_.w_ffty, fy_));
+  FZ_TRY(make_work(L_.w_fftz, fz_));
+  L_.pair_grid = 2048;
+  prepared_ = true;
+  gens_.clear();
+  for (auto* s : sys_) gens_.push_back(s->gen);
+  return FZ_OK;
+}
+
+int Engine::ensure() {
+  bool stale = !prepared_ || gens_.size() != sys_.size();
+  for (size_t k = 0; !stale && k < sys_.size(); k++) stale = gens_[k] != sys_[k]->gen;
+  return stale ? prepare() : FZ_OK;
+}
+
+void Engine::record_forces(bool ev) {
+  launch_nblist(L_, true);
+  launch_gather_sorted(L_);
+  launch_pair(L_, ev);
+  launch_bonded(L_, ev);
+  launch_pme(L_, ev, 0);
+}
+
+void Engine::record_step(bool ev) {
+  launch_kick_drift(L_);
+  launch_shake(L_);
+  record_forces(ev);
+  launch_kick(L_);
+  launch_rattle(L_);
+  if (ev) launch_kinetic(L_);
+}
+
+int Engine::capture(bool ev, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  FZ_HIP(hipStreamBeginCapture(L_.stream, hipStreamCaptureModeRelaxed));
+  record_step(ev);
+  FZ_HIP(hipStreamEndCapture(L_.stream, &g));
+  hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return set_error(FZ_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  return FZ_OK;
+}
+
+int Engine::read_status() {
+  for (auto* s : sys_) {
+    FZ_HIP(hipMemcpyAsync(s->flags_host, s->dv.flags, sizeof(s->flags_host), hipMemcpyDeviceToHost, L_.stream));
+    FZ_HIP(hipMemcpyAsync(s->acc_host, s->dv.acc, sizeof(s->acc_host), hipMemcpyDeviceToHost, L_.stream));
+  }
+  FZ_HIP(hipStreamSynchronize(L_.stream));
+  return FZ_OK;
+}
+
+int Engine::sync() {
+  FZ_HIP(hipStreamSynchronize(L_.stream));
+  return FZ_OK;
+}
+
+int Engine::grow_and_reset(int si, int bits) {
+  System* s = sys_[si];
+  SysDev& d = s->dv;
+  long long pairs = d.n_pairs_cap;
+  int clus = d.n_clusters_cap;
+  if (bits & OV_PAIRS) pairs = (long long)(pairs * 1.6) + 4096;
+  if (bits & OV_SLOTS) clus = (int)(clus * 1.3) + 64;
+  if (bits & (OV_COLUMN | OV_SEARCH))
+    return set_error(FZ_ERR_CAPACITY, "neighbour search capacity exceeded (column occupancy or J list > %d); "
+                     "use a finer column grid", SEARCH_TMP_CAP);
+  // the neighbour arrays are the last allocations of the system: free them
+  const size_t keep = s->allocs.size() - 13;
+  for (size_t k = keep; k < s->allocs.size(); k++) (void)hipFree(s->allocs[k]);
+  s->allocs.resize(keep);
+  FZ_TRY(s->alloc_neighbor(pairs, clus));
+  s->gen++;
+  FZ_HIP(hipMemset(d.flags, 0, F_NSLOT * sizeof(int)));
+  FZ_HIP(hipMemset(d.col_count, 0, d.ncol * sizeof(int)));
+  FZ_HIP(hipMemset(d.col_fill, 0, d.ncol * sizeof(int)));
+  s->need_rebuild = true;
+  return FZ_OK;
+}
+
+int Engine::forces(bool ev) {
+  FZ_TRY(ensure());
+  for (int attempt = 0; attempt < 4; attempt++) {
+    for (auto* s : sys_) {
+      if (s->need_rebuild) {
+        int one = 1;
+        FZ_HIP(hipMemcpyAsync(s->dv.flags + F_REBUILD, &one, sizeof(int), hipMemcpyHostToDevice, L_.stream));
+        FZ_HIP(hipStreamSynchronize(L_.stream));
+      }
+    }
+    launch_zero_acc(L_);
+    launch_zero_forces(L_);
+    record_forces(ev);
+    launch_kinetic(L_);
+    FZ_HIP(hipGetLastError());
+    FZ_TRY(read_status());
+    bool again = false;
+    for (size_t k = 0; k < sys_.size(); k++) {
+      System* s = sys_[k];
+      if (s->flags_host[F_OVERFLOW]) {
+        FZ_TRY(grow_and_reset((int)k, s->flags_host[F_OVERFLOW]));
+        again = true;
+      } else {
+        s->need_rebuild = false;
+      }
+    }
+    if (!again) return FZ_OK;
+    FZ_TRY(prepare());
+  }
+  return set_error(FZ_ERR_CAPACITY, "neighbour capacity could not be grown");
+}
+
+int Engine::steps(int n) {
+  if (n <= 0) return FZ_OK;
+  FZ_TRY(ensure());
+  for (auto* s : sys_)
+    if (s->need_rebuild) FZ_TRY(forces(true));
+  for (int attempt = 0; attempt < 4; attempt++) {
+    // snapshot u, v for replay after a capacity overflow
+    for (auto* s : sys_) {
+      const size_t nb = (size_t)s->topo.n * 8;
+      void* src[6] = {s->dv.ux, s->dv.uy, s->dv.uz, s->dv.vx, s->dv.vy, s->dv.vz};
+      for (int k = 0; k < 6; k++)
+        FZ_HIP(hipMemcpyAsync((char*)s->snap + k * nb, src[k], nb, hipMemcpyDeviceToDevice, L_.stream));
+    }
+    if (!g_plain_) FZ_TRY(capture(false, &g_plain_));
+    if (!g_ev_) FZ_TRY(capture(true, &g_ev_));
+    for (int k = 0; k < n; k++) FZ_HIP(hipGraphLaunch(k == n - 1 ? g_ev_ : g_plain_, L_.stream));
+    FZ_TRY(read_status());
+    bool again = false;
+    for (size_t k = 0; k < sys_.size(); k++) {
+      System* s = sys_[k];
+      if (s->flags_host[F_OVERFLOW]) {
+        // restore the snapshot, grow, re-evaluate forces, replay
+        const size_t nb = (size_t)s->topo.n * 8;
+        void* dst[6] = {s->dv.ux, s->dv.uy, s->dv.uz, s->dv.vx, s->dv.vy, s->dv.vz};
+        for (int c = 0; c < 6; c++)
+          FZ_HIP(hipMemcpyAsync(dst[c], (char*)s->snap + c * nb, nb, hipMemcpyDeviceToDevice, L_.stream));
+        FZ_HIP(hipStreamSynchronize(L_.stream));
+        FZ_TRY(grow_and_reset((int)k, s->flags_host[F_OVERFLOW]));
+        again = true;
+      }
+    }
+    if (!again) {
+      for (auto* s : sys_) {
+        if (s->flags_host[F_SHAKE_FAIL]) return set_error(FZ_ERR_SHAKE, "SHAKE did not converge");
+        s->step += n;
+      }
+      return FZ_OK;
+    }
+    FZ_TRY(prepare());
+    FZ_TRY(forces(true));
+  }
+  return set_error(FZ_ERR_CAPACITY, "neighbour capacity could not be grown");
+}
+
+void launch_min_move(const Launch& L);
+void launch_min_restore(const Launch& L, const int* d_reject);
+void launch_zero_vel(const Launch& L);
+
+// Steepest descent with an adaptive step (replaces MinimizeStep's SD branch,
+// src/dynamics.cu:491-620, src/mdcuda.cu:245): the atom with the largest force
+// moves by the current step; an energy decrease is accepted and grows the step
+// by 1.2, an increase is undone and halves it (the reference reverts and
+// switches method on an increase, src/dynamics.cu:597-618).
+int Engine::minimize(int nsteps, double max_step) {
+  FZ_TRY(ensure());
+  const int ns = (int)sys_.size();
+  std::vector<double> step(ns, max_step), e_old(ns);
+  std::vector<int> reject(ns, 0);
+  int* d_reject = nullptr;
+  FZ_HIP(hipMalloc(&d_reject, ns * sizeof(int)));
+  struct Free { int* p; ~Free() { if (p) (void)hipFree(p); } } guard{d_reject};
+  FZ_TRY(forces(true));
+  fz_energies e;
+  for (int s = 0; s < ns; s++) { sys_[s]->energies(&e); e_old[s] = e.potential; }
+  for (int it = 0; it < nsteps; it++) {
+    for (int s = 0; s < ns; s++)
+      FZ_HIP(hipMemcpyAsync(sys_[s]->dv.mstep, &step[s], sizeof(double), hipMemcpyHostToDevice, L_.stream));
+    launch_min_move(L_);
+    FZ_HIP(hipGetLastError());
+    FZ_TRY(forces(true));
+    bool any = false;
+    for (int s = 0; s < ns; s++) {
+      sys_[s]->energies(&e);
+      reject[s] = !(e.potential < e_old[s]);
+      if (reject[s]) { step[s] *= 0.5; any = true; }
+      else { e_old[s] = e.potential; step[s] = std::min(step[s] * 1.2, max_step); }
+    }
+    if (any) {
+      FZ_HIP(hipMemcpyAsync(d_reject, reject.data(), ns * sizeof(int), hipMemcpyHostToDevice, L_.stream));
+      launch_min_restore(L_, d_reject);
+      FZ_TRY(forces(true));
+    }
+  }
+  launch_zero_vel(L_);
+  launch_zero_acc(L_);
+  record_forces(true);
+  launch_kinetic(L_);
+  FZ_HIP(hipGetLastError());
+  return read_status();
+}
+
+int Engine::maxwell(double T, uint64_t seed, uint64_t counter) {
+  FZ_TRY(ensure());
+  launch_maxwell(L_, T, seed, counter);
+  launch_zero_acc(L_);
+  launch_kinetic(L_);
+  FZ_HIP(hipGetLastError());
+  FZ_TRY(read_status());
+  return FZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stand-alone PME convolution (test entry)
+
+int pme_convolve(int device, const int32_t K[3], const double box[3], double beta, int mixed, int mode,
+                 const double* q, double* phi, double* energy) {
+  FZ_HIP(hipSetDevice(device));
+  for (int d = 0; d < 3; d++)
+    if (!smooth235(K[d]) || K[d] > MAX_FFT || K[d] < 2) return set_error(FZ_ERR_ARG, "grid must be 2,3,5-smooth, 2..%d", MAX_FFT);
+  System s;
+  s.device = device;
+  s.mixed = mixed;
+  SysDev& d = s.dv;
+  memset(&d, 0, sizeof d);
+  for (int k = 0; k < 3; k++) { d.K[k] = K[k]; d.L[k] = box[k]; }
+  d.Kxh = K[0] / 2 + 1;
+  d.beta = beta;
+  const double V = box[0] * box[1] * box[2];
+  d.pme_pre = kCC / (kPi * V);
+  d.pi2_beta2 = kPi * kPi / (beta * beta);
+  const size_t K3 = (size_t)K[0] * K[1] * K[2], KH = (size_t)d.Kxh * K[1] * K[2];
+  const int KS = K[0] + K[1] + K[2];
+  void* p;
+  FZ_TRY(s.dalloc((unsigned long long**)&p, K3)); d.Qacc = p;
+  FZ_TRY(s.dalloc((char**)&p, KH * (mixed ? 8 : 16))); d.Qc = p;
+  FZ_TRY(s.dalloc((char**)&p, K3 * (mixed ? 4 : 8))); d.phi = p;
+  std::vector<double> bsp(KS), ex(KS), mv(KS);
+  std::vector<double2> tw64(KS);
+  std::vector<float2> tw32(KS);
+  int off = 0;
+  for (int ax = 0; ax < 3; ax++) {
+    for (int j = 0; j < K[ax]; j++) {
+      const int jj = j <= K[ax] / 2 ? j : j - K[ax];
+      const double m = jj / box[ax], c = std::cos(2.0 * kPi * j / K[ax]);
+      bsp[off + j] = 9.0 / ((2.0 + c) * (2.0 + c));
+      ex[off + j] = std::exp(-kPi * kPi * m * m / (beta * beta));
+      mv[off + j] = m;
+      const double a = -2.0 * kPi * j / K[ax];
+      tw64[off + j] = make_double2(std::cos(a), std::sin(a));
+      tw32[off + j] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    off += K[ax];
+  }
+  FZ_TRY(s.dalloc(&d.pme_bsp, KS)); FZ_TRY(up(d.pme_bsp, bsp));
+  FZ_TRY(s.dalloc(&d.pme_ex, KS)); FZ_TRY(up(d.pme_ex, ex));
+  FZ_TRY(s.dalloc(&d.pme_m, KS)); FZ_TRY(up(d.pme_m, mv));
+  if (mixed) { float2* t; FZ_TRY(s.dalloc(&t, KS)); FZ_TRY(up(t, tw32)); d.tw = t; }
+  else { double2* t; FZ_TRY(s.dalloc(&t, KS)); FZ_TRY(up(t, tw64)); d.tw = t; }
+  FZ_TRY(s.dalloc(&d.acc, A_NSLOT));
+  FZ_TRY(s.dalloc(&d.flags, F_NSLOT));
+  if (mixed) {
+    std::vector<unsigned long long> h(K3);
+    for (size_t i = 0; i < K3; i++) h[i] = (unsigned long long)llrint(q[i] * 4294967296.0);
+    FZ_HIP(hipMemcpy(d.Qacc, h.data(), K3 * 8, hipMemcpyHostToDevice));
+  } else {
+    FZ_HIP(hipMemcpy(d.Qacc, q, K3 * 8, hipMemcpyHostToDevice));
+  }
+  Engine eng(device, {&s});
+  FZ_TRY(eng.prepare());
+  launch_pme_fft_only(eng.launch(), mode);
+  FZ_HIP(hipGetLastError());
+  FZ_TRY(eng.read_status());
+  if (mixed) {
+    std::vector<float> h(K3);
+    FZ_HIP(hipMemcpy(h.data(), d.phi, K3 * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < K3; i++) phi[i] = h[i];
+  } else {
+    FZ_HIP(hipMemcpy(phi, d.phi, K3 * 8, hipMemcpyDeviceToHost));
+  }
+  if (energy) *energy = fx(s.acc_host[A_PME]);
+  return FZ_OK;
+}
+
+}  // namespace fz

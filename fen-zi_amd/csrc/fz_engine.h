@@ -1,0 +1,118 @@
+// fz_engine.h -- host runtime: systems resident in HBM and the engine that
+// advances one or many of them with one launch per phase per step.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fenzi.h"
+#include "fz_launch.h"
+#include "fz_sys.h"
+
+namespace fz {
+
+int set_error(int code, const char* fmt, ...);
+
+#define FZ_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return ::fz::set_error(FZ_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define FZ_TRY(call)             \
+  do {                           \
+    int r_ = (call);             \
+    if (r_ != FZ_OK) return r_;  \
+  } while (0)
+
+// Owned copy of the processed topology.
+struct HostTopo {
+  int n = 0, ntypes = 0;
+  std::vector<double> q, mass, minv;
+  std::vector<int> type;
+  std::vector<double> lj_eps, lj_sig2, lj14_eps, lj14_sig2;
+  std::vector<int2> bond_ij; std::vector<double2> bond_p;
+  std::vector<int4> angle_ijk; std::vector<double4> angle_p;
+  std::vector<int4> dihe_ijkl; std::vector<double2> dihe_p; std::vector<int> dihe_n;
+  std::vector<int4> impr_ijkl; std::vector<double> impr_k;
+  std::vector<int2> ew_ij, p14_ij;
+  std::vector<unsigned long long> excl_bits;
+  int use_excl_list = 0;
+  std::vector<int> excl_off, excl_list;
+  std::vector<int4> ccl_atoms, ccl_cons; std::vector<double4> ccl_d2;
+  int n_constraints = 0;
+  double q2sum = 0.0;
+};
+
+struct System {
+  int device = 0;
+  int mixed = 0;
+  fz_params par{};
+  HostTopo topo;
+  SysDev dv{};
+  std::vector<void*> allocs;
+  size_t bytes = 0;
+  int64_t step = 0;
+  int gen = 0;  // bumped whenever device buffers are reallocated
+  double self_energy = 0.0;
+  bool need_rebuild = true;
+  // snapshot buffers for capacity-overflow recovery
+  unsigned long long* snap = nullptr;  // 3n u + 3n v (as bits)
+  // host mirror of the last read accumulators
+  unsigned long long acc_host[A_NSLOT] = {0};
+  int flags_host[F_NSLOT] = {0};
+  ~System();
+  int init(const fz_topology* t, const fz_params* p, int precision, int device);
+  int alloc_device();
+  int alloc_neighbor(long long pairs_cap, int clusters_cap);
+  void free_device();
+  template <class T> int dalloc(T** p, size_t count, bool zero = true);
+  int upload_positions(const double* xyz);
+  int upload_fixed(const uint64_t* u);
+  int upload_velocities(const double* v);
+  int download_state(double* xyz, double* v, uint64_t* u);
+  int energies(fz_energies* e);
+};
+
+// Launch composition over a set of systems.
+class Engine {
+ public:
+  Engine(int device, std::vector<System*> sys);
+  ~Engine();
+  int prepare();                       // work tables, descriptors, streams
+  int forces(bool ev);                 // stand-alone evaluation at current positions
+  int steps(int n);                    // n MD steps (energies on the last)
+  int maxwell(double T, uint64_t seed, uint64_t counter);
+  int minimize(int nsteps, double max_step);  // adaptive steepest descent, velocities zeroed
+  int read_status();                   // flags + accumulators of every system -> host
+  int sync();
+  hipStream_t stream() const { return L_.stream; }
+  Launch& launch() { return L_; }
+  std::vector<System*>& systems() { return sys_; }
+
+ private:
+  int device_;
+  std::vector<System*> sys_;
+  Launch L_;
+  SysDev* d_sys_ = nullptr;
+  std::vector<int*> d_pre_;
+  hipGraphExec_t g_plain_ = nullptr, g_ev_ = nullptr;
+  bool prepared_ = false;
+  std::vector<int> gens_;
+  int ensure();  // (re)prepare when a member's buffers moved since the last prepare
+  int make_work(Work& w, const std::vector<int>& blocks);
+  int capture(bool ev, hipGraphExec_t* out);
+  void record_step(bool ev);
+  void record_forces(bool ev);
+  int grow_and_reset(int s, int bits);
+  void drop_graphs();
+  int upload_desc();
+};
+
+int pme_convolve(int device, const int32_t K[3], const double box[3], double beta, int mixed, int mode,
+                 const double* q, double* phi, double* energy);
+
+}  // namespace fz

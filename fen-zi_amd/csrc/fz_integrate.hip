@@ -1,0 +1,409 @@
+// fz_integrate.hip -- velocity Verlet, SHAKE/RATTLE, Maxwell velocities, kinetic energy.
+//
+// Replaces HalfKickGPU / UpdateCoords / CoordsUpdate (src/mdcuda.cu:27-123,
+// 354-478), solve_bond_constraints / solve_velocity_constraints
+// (src/shake.cu:20-615) and InitVelocity (src/init.cu:19-211).  Positions are
+// 64-bit fixed-point fractions, so the periodic wrap is the integer overflow of
+// the drift; SHAKE and RATTLE run in FP64 in both precision forms, one thread
+// per constraint cluster (<= 4 atoms, <= 3 constraints), converged when
+// max |r^2 - d^2| / (2 d^2) < tol.  Velocities are drawn on the device with
+// Philox4x32-10 (counter based: reproducible for any thread layout).
+#include "fz_device.h"
+#include "fz_launch.h"
+
+namespace fz {
+
+template <bool MIXED>
+__device__ __forceinline__ double take_force(const SysDev& d, int sl, int comp, bool zero) {
+  if (MIXED) {
+    unsigned long long* F = (unsigned long long*)d.Fs + (size_t)comp * d.n_slots_cap + sl;
+    double v = from_fixed_acc(*F);
+    if (zero) *F = 0ull;
+    return v;
+  } else {
+    double* F = (double*)d.Fs + (size_t)comp * d.n_slots_cap + sl;
+    double v = *F;
+    if (zero) *F = 0.0;
+    return v;
+  }
+}
+
+// first half kick + drift; clears the force accumulators, the energy
+// accumulators and the rebuild flag for the coming evaluation
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_kick_drift(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int lb = blockIdx.x - pre[s_];
+  if (lb == 0) {
+    if (threadIdx.x < A_NSLOT) d.acc[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) d.flags[F_REBUILD] = 0;
+  }
+  const int i = lb * 256 + threadIdx.x;
+  if (i >= d.n_atoms) return;
+  const int sl = d.atom_slot[i];
+  const double h = 0.5 * d.dt * d.minv[i];
+  const double vx = d.vx[i] + h * take_force<MIXED>(d, sl, 0, true);
+  const double vy = d.vy[i] + h * take_force<MIXED>(d, sl, 1, true);
+  const double vz = d.vz[i] + h * take_force<MIXED>(d, sl, 2, true);
+  d.vx[i] = vx; d.vy[i] = vy; d.vz[i] = vz;
+  const unsigned long long ux = d.ux[i], uy = d.uy[i], uz = d.uz[i];
+  d.ox[i] = ux; d.oy[i] = uy; d.oz[i] = uz;
+  d.ux[i] = ux + (unsigned long long)__double2ll_rn(d.dt * vx * d.isc[0]);
+  d.uy[i] = uy + (unsigned long long)__double2ll_rn(d.dt * vy * d.isc[1]);
+  d.uz[i] = uz + (unsigned long long)__double2ll_rn(d.dt * vz * d.isc[2]);
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_kick(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  if (i >= d.n_atoms) return;
+  const int sl = d.atom_slot[i];
+  const double h = 0.5 * d.dt * d.minv[i];
+  d.vx[i] += h * take_force<MIXED>(d, sl, 0, false);
+  d.vy[i] += h * take_force<MIXED>(d, sl, 1, false);
+  d.vz[i] += h * take_force<MIXED>(d, sl, 2, false);
+}
+
+// ---- constraint clusters ------------------------------------------------------
+
+struct CCl {
+  int na, nc;
+  int at[4];
+  int c1[3], c2[3];
+  double d2[3];
+};
+
+__device__ __forceinline__ CCl load_ccl(const SysDev& d, int c) {
+  CCl k;
+  const int4 a = d.ccl_atoms[c];
+  const int4 p = d.ccl_cons[c];
+  const double4 l = d.ccl_d2[c];
+  k.at[0] = a.x; k.at[1] = a.y; k.at[2] = a.z; k.at[3] = a.w;
+  k.na = (a.x >= 0) + (a.y >= 0) + (a.z >= 0) + (a.w >= 0);
+  k.nc = p.w;
+  const int pp[3] = {p.x, p.y, p.z};
+#pragma unroll
+  for (int j = 0; j < 3; j++) { k.c1[j] = pp[j] & 0xff; k.c2[j] = (pp[j] >> 8) & 0xff; }
+  k.d2[0] = l.x; k.d2[1] = l.y; k.d2[2] = l.z;
+  return k;
+}
+
+// in-place solve of the n x n (n <= 3) system A x = b, partial pivoting
+__device__ __forceinline__ bool solve3(double A[3][3], double* b, int n) {
+  for (int k = 0; k < 3; k++) {
+    if (k >= n) break;
+    int piv = k;
+    for (int r = k + 1; r < n; r++)
+      if (fabs(A[r][k]) > fabs(A[piv][k])) piv = r;
+    if (A[piv][k] == 0.0) return false;
+    if (piv != k) {
+      for (int c = 0; c < 3; c++) { double t = A[k][c]; A[k][c] = A[piv][c]; A[piv][c] = t; }
+      double t = b[k]; b[k] = b[piv]; b[piv] = t;
+    }
+    for (int r = k + 1; r < n; r++) {
+      const double m = A[r][k] / A[k][k];
+      for (int c = k; c < n; c++) A[r][c] -= m * A[k][c];
+      b[r] -= m * b[k];
+    }
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    double x = b[k];
+    for (int c = k + 1; c < n; c++) x -= A[k][c] * b[c];
+    b[k] = x / A[k][k];
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
+  if (c >= d.n_ccl) return;
+  const CCl k = load_ccl(d, c);
+  const int a0 = k.at[0];
+  double rn[4][3], ro[4][3], mi[4], dr[4][3];
+  for (int m = 0; m < k.na; m++) {
+    const int a = k.at[m];
+    rn[m][0] = fx_d(d.ux[a0], d.ux[a], d.sc[0]);
+    rn[m][1] = fx_d(d.uy[a0], d.uy[a], d.sc[1]);
+    rn[m][2] = fx_d(d.uz[a0], d.uz[a], d.sc[2]);
+    ro[m][0] = fx_d(d.ox[a0], d.ox[a], d.sc[0]);
+    ro[m][1] = fx_d(d.oy[a0], d.oy[a], d.sc[1]);
+    ro[m][2] = fx_d(d.oz[a0], d.oz[a], d.sc[2]);
+    mi[m] = d.minv[a];
+    dr[m][0] = dr[m][1] = dr[m][2] = 0.0;
+  }
+  double g[3][3], lam[3] = {0.0, 0.0, 0.0};
+  for (int j = 0; j < k.nc; j++)
+    for (int x = 0; x < 3; x++) g[j][x] = ro[k.c1[j]][x] - ro[k.c2[j]][x];
+  bool ok = false;
+  for (int it = 0; it < 200; it++) {
+    double rc[3][3], sig[3], worst = 0.0;
+    for (int j = 0; j < k.nc; j++) {
+      const int p = k.c1[j], q = k.c2[j];
+      for (int x = 0; x < 3; x++) rc[j][x] = (rn[p][x] + dr[p][x]) - (rn[q][x] + dr[q][x]);
+      sig[j] = rc[j][0] * rc[j][0] + rc[j][1] * rc[j][1] + rc[j][2] * rc[j][2] - k.d2[j];
+      worst = fmax(worst, fabs(sig[j]) / (2.0 * k.d2[j]));
+    }
+    if (worst < d.shake_tol) { ok = true; break; }
+    double J[3][3];
+    for (int j = 0; j < 3; j++)
+      for (int e = 0; e < 3; e++) J[j][e] = 0.0;
+    for (int j = 0; j < k.nc; j++)
+      for (int e = 0; e < k.nc; e++) {
+        const int p = k.c1[j], q = k.c2[j], ep = k.c1[e], eq = k.c2[e];
+        const double wp = ((p == ep) - (p == eq)) * mi[p], wq = ((q == ep) - (q == eq)) * mi[q];
+        J[j][e] = 2.0 * (rc[j][0] * g[e][0] + rc[j][1] * g[e][1] + rc[j][2] * g[e][2]) * (wp - wq);
+      }
+    for (int j = 0; j < k.nc; j++) sig[j] = -sig[j];
+    if (!solve3(J, sig, k.nc)) break;
+    for (int j = 0; j < k.nc; j++) lam[j] += sig[j];
+    for (int m = 0; m < k.na; m++) dr[m][0] = dr[m][1] = dr[m][2] = 0.0;
+    for (int j = 0; j < k.nc; j++) {
+      const int p = k.c1[j], q = k.c2[j];
+      for (int x = 0; x < 3; x++) {
+        dr[p][x] += lam[j] * g[j][x] * mi[p];
+        dr[q][x] -= lam[j] * g[j][x] * mi[q];
+      }
+    }
+  }
+  if (!ok) d.flags[F_SHAKE_FAIL] = 1;
+  const double idt = 1.0 / d.dt;
+  for (int m = 0; m < k.na; m++) {
+    const int a = k.at[m];
+    d.ux[a] += (unsigned long long)__double2ll_rn(dr[m][0] * d.isc[0]);
+    d.uy[a] += (unsigned long long)__double2ll_rn(dr[m][1] * d.isc[1]);
+    d.uz[a] += (unsigned long long)__double2ll_rn(dr[m][2] * d.isc[2]);
+    d.vx[a] += dr[m][0] * idt;
+    d.vy[a] += dr[m][1] * idt;
+    d.vz[a] += dr[m][2] * idt;
+  }
+}
+
+__global__ __launch_bounds__(128) void k_rattle(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
+  if (c >= d.n_ccl) return;
+  const CCl k = load_ccl(d, c);
+  const int a0 = k.at[0];
+  double r[4][3], v[4][3], mi[4];
+  for (int m = 0; m < k.na; m++) {
+    const int a = k.at[m];
+    r[m][0] = fx_d(d.ux[a0], d.ux[a], d.sc[0]);
+    r[m][1] = fx_d(d.uy[a0], d.uy[a], d.sc[1]);
+    r[m][2] = fx_d(d.uz[a0], d.uz[a], d.sc[2]);
+    v[m][0] = d.vx[a]; v[m][1] = d.vy[a]; v[m][2] = d.vz[a];
+    mi[m] = d.minv[a];
+  }
+  double rc[3][3], A[3][3], b[3];
+  for (int j = 0; j < 3; j++)
+    for (int e = 0; e < 3; e++) A[j][e] = 0.0;
+  for (int j = 0; j < k.nc; j++)
+    for (int x = 0; x < 3; x++) rc[j][x] = r[k.c1[j]][x] - r[k.c2[j]][x];
+  for (int j = 0; j < k.nc; j++) {
+    const int p = k.c1[j], q = k.c2[j];
+    b[j] = -((v[p][0] - v[q][0]) * rc[j][0] + (v[p][1] - v[q][1]) * rc[j][1] + (v[p][2] - v[q][2]) * rc[j][2]);
+    for (int e = 0; e < k.nc; e++) {
+      const int ep = k.c1[e], eq = k.c2[e];
+      const double wp = ((p == ep) - (p == eq)) * mi[p], wq = ((q == ep) - (q == eq)) * mi[q];
+      A[j][e] = (rc[j][0] * rc[e][0] + rc[j][1] * rc[e][1] + rc[j][2] * rc[e][2]) * (wp - wq);
+    }
+  }
+  if (!solve3(A, b, k.nc)) return;
+  for (int j = 0; j < k.nc; j++) {
+    const int p = k.c1[j], q = k.c2[j];
+    for (int x = 0; x < 3; x++) {
+      v[p][x] += b[j] * rc[j][x] * mi[p];
+      v[q][x] -= b[j] * rc[j][x] * mi[q];
+    }
+  }
+  for (int m = 0; m < k.na; m++) {
+    const int a = k.at[m];
+    d.vx[a] = v[m][0]; d.vy[a] = v[m][1]; d.vz[a] = v[m][2];
+  }
+}
+
+// ---- kinetic energy, velocities ---------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_kinetic(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  double ke[1] = {0.0};
+  if (i < d.n_atoms) ke[0] = 0.5 / d.minv[i] * (d.vx[i] * d.vx[i] + d.vy[i] * d.vy[i] + d.vz[i] * d.vz[i]);
+  const int slots[1] = {A_KIN};
+  block_acc_fixed<1>(ke, d.acc, slots);
+}
+
+__device__ __forceinline__ void philox4x32(unsigned c[4], unsigned k0, unsigned k1) {
+  // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const unsigned hi0 = __umulhi(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+    const unsigned hi1 = __umulhi(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+    const unsigned n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+}
+
+// v_d = sqrt(RC T / m) N(0,1) (src/init.cu:80-92); momentum accumulated for removal
+__global__ __launch_bounds__(256) void k_maxwell(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
+                                                 double T, unsigned long long seed, unsigned long long counter) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  double p[3] = {0.0, 0.0, 0.0};
+  if (i < d.n_atoms) {
+    unsigned c[4] = {(unsigned)i, 0u, (unsigned)counter, (unsigned)(counter >> 32)};
+    philox4x32(c, (unsigned)seed, (unsigned)(seed >> 32));
+    const double s = 1.0 / 4294967296.0;
+    const double u1 = ((double)c[0] + 0.5) * s, u2 = ((double)c[1] + 0.5) * s;
+    const double u3 = ((double)c[2] + 0.5) * s, u4 = ((double)c[3] + 0.5) * s;
+    const double m = 1.0 / d.minv[i], vm = sqrt(RCG * T / m);
+    const double ra = sqrt(-2.0 * log(u1)), rb = sqrt(-2.0 * log(u3));
+    const double vx = vm * ra * cos(2.0 * PI_D * u2), vy = vm * ra * sin(2.0 * PI_D * u2);
+    const double vz = vm * rb * cos(2.0 * PI_D * u4);
+    d.vx[i] = vx; d.vy[i] = vy; d.vz[i] = vz;
+    p[0] = m * vx; p[1] = m * vy; p[2] = m * vz;
+  }
+  const int slots[3] = {A_MOM, A_MOM + 1, A_MOM + 2};
+  block_acc_fixed<3>(p, d.acc, slots);
+}
+
+// v_i -= P / (N m_i) (src/init.cu:94-118)
+__global__ __launch_bounds__(256) void k_momentum(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  if (i >= d.n_atoms) return;
+  const double f = d.minv[i] / d.n_atoms;
+  d.vx[i] -= from_fixed_e(d.acc[A_MOM]) * f;
+  d.vy[i] -= from_fixed_e(d.acc[A_MOM + 1]) * f;
+  d.vz[i] -= from_fixed_e(d.acc[A_MOM + 2]) * f;
+}
+
+// ---- steepest-descent minimiser (replaces SteepestDescent, src/mdcuda.cu:245) ----
+
+// largest |F|^2 per system; non-negative doubles order like their bit patterns
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_fmax(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  double f2 = 0.0;
+  if (i < d.n_atoms) {
+    const int sl = d.atom_slot[i];
+    const double fx = take_force<MIXED>(d, sl, 0, false), fy = take_force<MIXED>(d, sl, 1, false),
+                 fz = take_force<MIXED>(d, sl, 2, false);
+    f2 = fx * fx + fy * fy + fz * fz;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) f2 = fmax(f2, __shfl_xor(f2, o, 64));
+  if ((threadIdx.x & 63) == 0 && f2 > 0.0) atomicMax(d.acc + A_FMAX, (unsigned long long)__double_as_longlong(f2));
+}
+
+// u <- u + mstep * F / max|F| (the atom with the largest force moves mstep);
+// the previous positions are kept in o for SHAKE and for a rejected move
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_min_move(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  if (i >= d.n_atoms) return;
+  const double f2max = __longlong_as_double((long long)d.acc[A_FMAX]);
+  const double a = f2max > 0.0 ? *d.mstep / sqrt(f2max) : 0.0;
+  const int sl = d.atom_slot[i];
+  const unsigned long long ux = d.ux[i], uy = d.uy[i], uz = d.uz[i];
+  d.ox[i] = ux; d.oy[i] = uy; d.oz[i] = uz;
+  d.ux[i] = ux + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, 0, false) * d.isc[0]);
+  d.uy[i] = uy + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, 1, false) * d.isc[1]);
+  d.uz[i] = uz + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, 2, false) * d.isc[2]);
+}
+
+__global__ __launch_bounds__(256) void k_min_restore(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
+                                                     const int* __restrict__ reject) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  if (i >= d.n_atoms || !reject[s_]) return;
+  d.ux[i] = d.ox[i]; d.uy[i] = d.oy[i]; d.uz[i] = d.oz[i];
+}
+
+__global__ __launch_bounds__(256) void k_zero_vel(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  if (i >= d.n_atoms) return;
+  d.vx[i] = 0.0; d.vy[i] = 0.0; d.vz[i] = 0.0;
+}
+
+void launch_min_move(const Launch& L) {
+  if (L.mixed) {
+    hipLaunchKernelGGL(k_fmax<true>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+    hipLaunchKernelGGL(k_min_move<true>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  } else {
+    hipLaunchKernelGGL(k_fmax<false>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+    hipLaunchKernelGGL(k_min_move<false>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  }
+  launch_shake(L);
+}
+void launch_min_restore(const Launch& L, const int* d_reject) {
+  hipLaunchKernelGGL(k_min_restore, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre, d_reject);
+}
+void launch_zero_vel(const Launch& L) {
+  hipLaunchKernelGGL(k_zero_vel, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+}
+
+__global__ void k_zero_acc(const SysDev* __restrict__ S, int nsys) {
+  const SysDev& d = S[blockIdx.x];
+  if (threadIdx.x < A_NSLOT) d.acc[threadIdx.x] = 0ull;
+}
+
+// clears the slot-ordered force accumulators (used before a stand-alone evaluation)
+__global__ __launch_bounds__(256) void k_zero_forces(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  if (sl >= d.n_slots_cap) return;
+  unsigned long long* F = (unsigned long long*)d.Fs;
+  F[sl] = 0ull;
+  F[d.n_slots_cap + sl] = 0ull;
+  F[2 * d.n_slots_cap + sl] = 0ull;
+}
+
+void launch_kick_drift(const Launch& L) {
+  if (L.mixed) hipLaunchKernelGGL(k_kick_drift<true>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  else hipLaunchKernelGGL(k_kick_drift<false>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+}
+void launch_kick(const Launch& L) {
+  if (L.mixed) hipLaunchKernelGGL(k_kick<true>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  else hipLaunchKernelGGL(k_kick<false>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+}
+void launch_shake(const Launch& L) {
+  if (L.w_ccl.total) hipLaunchKernelGGL(k_shake, dim3(L.w_ccl.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_ccl.pre);
+}
+void launch_rattle(const Launch& L) {
+  if (L.w_ccl.total) hipLaunchKernelGGL(k_rattle, dim3(L.w_ccl.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_ccl.pre);
+}
+void launch_kinetic(const Launch& L) {
+  hipLaunchKernelGGL(k_kinetic, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+}
+void launch_zero_acc(const Launch& L) {
+  hipLaunchKernelGGL(k_zero_acc, dim3(L.nsys), dim3(64), 0, L.stream, L.d_sys, L.nsys);
+}
+void launch_zero_forces(const Launch& L) {
+  hipLaunchKernelGGL(k_zero_forces, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+}
+void launch_maxwell(const Launch& L, double T, unsigned long long seed, unsigned long long counter) {
+  launch_zero_acc(L);
+  hipLaunchKernelGGL(k_maxwell, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre, T, seed, counter);
+  hipLaunchKernelGGL(k_momentum, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  launch_rattle(L);
+}
+
+}  // namespace fz

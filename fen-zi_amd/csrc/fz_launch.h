@@ -1,0 +1,50 @@
+// fz_launch.h -- host-side launch description shared by the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fz_sys.h"
+
+namespace fz {
+
+// A static work decomposition: pre[s] = first block of system s, total blocks.
+struct Work {
+  int* pre = nullptr;  // device, nsys + 1 entries
+  int total = 0;
+};
+
+struct Launch {
+  hipStream_t stream = nullptr;
+  const SysDev* d_sys = nullptr;  // device array of nsys descriptors
+  int nsys = 0;
+  int mixed = 0;                  // 0 = FP64 form, 1 = mixed form
+  Work w_atoms;   // 256 atoms per block
+  Work w_slots;   // 256 slots per block
+  Work w_cols;    // one block per column
+  Work w_clus;    // 4 clusters (waves) per block
+  Work w_terms;   // 256 bonded terms per block
+  Work w_ccl;     // 128 constraint clusters per block
+  Work w_fftx;    // x-line groups
+  Work w_ffty;    // y-line groups
+  Work w_fftz;    // z-line groups
+  int* chunk_pre = nullptr;  // device: pair-kernel chunks per system (prefix)
+  int pair_grid = 0;         // persistent pair-kernel blocks
+  int fft_lds = 0;           // dynamic LDS bytes for the FFT kernels
+  int fft_nl = 16;           // lines per FFT workgroup
+};
+
+void launch_nblist(const Launch& L, bool always_rebuild_check);
+void launch_gather_sorted(const Launch& L);
+void launch_pair(const Launch& L, bool ev);
+void launch_bonded(const Launch& L, bool ev);
+void launch_pme(const Launch& L, bool ev, int conv_mode);
+void launch_pme_fft_only(const Launch& L, int conv_mode);
+void launch_kick_drift(const Launch& L);
+void launch_shake(const Launch& L);
+void launch_kick(const Launch& L);
+void launch_rattle(const Launch& L);
+void launch_kinetic(const Launch& L);
+void launch_maxwell(const Launch& L, double T, unsigned long long seed, unsigned long long counter);
+void launch_zero_acc(const Launch& L);
+void launch_zero_forces(const Launch& L);
+
+}  // namespace fz

@@ -1,0 +1,385 @@
+// fz_nblist.hip -- cell (column) binning and cluster-pair neighbour list.
+//
+// Replaces the reference's CellBuild/CellUpdate/CellClean and
+// nbbuild_exclbitvec/nbbuild_excllist (src/cell_based_kernels.cu:198-399,
+// 1048-1493) and the host-side rebuild branch of SingleStep
+// (src/dynamics.cu:727-737).  Design:
+//   * positions are 64-bit fixed-point fractions of the box, so binning is an
+//     exact integer mulhi and every test below is bit-reproducible;
+//   * atoms are binned into x-y columns, sorted by (z, atom id) inside each
+//     column, and cut into clusters of 8 consecutive atoms;
+//   * a cluster pair (I <= J) enters the list iff the periodic gap between the
+//     two clusters' bounding boxes is below the pair-list distance; the gap is
+//     formed in integers and squared in FP64 without contraction;
+//   * each list entry carries a 64-bit mask, one bit per lane of the 8x8 wave64
+//     pair tile, clearing padding, the lower triangle of self pairs and
+//     excluded (1-2, 1-3, 1-4) pairs;
+//   * the rebuild decision is taken on the device (displacement since the
+//     last build > half the skin), so a step never waits on the host.
+#include "fz_device.h"
+#include "fz_launch.h"
+
+namespace fz {
+
+#define SYS_FROM(pre)                                  \
+  const int s_ = locate(pre, nsys, blockIdx.x);        \
+  const SysDev& d = S[s_];                             \
+  const int lb = blockIdx.x - pre[s_];                 \
+  (void)lb;
+
+// ---------------------------------------------------------------------------
+// rebuild trigger: any atom moved more than half the skin since the last build
+// (reference CoordsUpdate, src/mdcuda.cu:453-458)
+__global__ __launch_bounds__(256) void k_check(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  SYS_FROM(pre);
+  const int i = lb * 256 + threadIdx.x;
+  bool move = false;
+  if (i < d.n_atoms) {
+    double dx = fx_d(d.rx[i], d.ux[i], d.sc[0]);
+    double dy = fx_d(d.ry[i], d.uy[i], d.sc[1]);
+    double dz = fx_d(d.rz[i], d.uz[i], d.sc[2]);
+    move = dx * dx + dy * dy + dz * dz > d.skin2;
+  }
+  if (__any(move) && (threadIdx.x & 63) == 0) d.flags[F_REBUILD] = 1;
+}
+
+__device__ __forceinline__ bool skip_build(const SysDev& d) {
+  return d.flags[F_REBUILD] == 0 || d.flags[F_OVERFLOW] != 0;
+}
+
+__device__ __forceinline__ int column_of(const SysDev& d, int i) {
+  int cx = (int)__umul64hi(d.ux[i], (unsigned long long)d.ncx);
+  int cy = (int)__umul64hi(d.uy[i], (unsigned long long)d.ncy);
+  return cy * d.ncx + cx;
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  SYS_FROM(pre);
+  if (skip_build(d)) return;
+  const int i = lb * 256 + threadIdx.x;
+  if (i >= d.n_atoms) return;
+  int c = column_of(d, i);
+  d.atom_col[i] = c;
+  atomicAdd(d.col_count + c, 1);
+}
+
+// one block per system: clusters per column, exclusive scan, capacity checks
+__global__ __launch_bounds__(1024) void k_col_scan(const SysDev* __restrict__ S, int nsys) {
+  const SysDev& d = S[blockIdx.x];
+  if (skip_build(d)) return;
+  __shared__ int part[1024];
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const int n = d.ncol;
+  const int per = (n + 1023) / 1024;
+  const int c0 = threadIdx.x * per, c1 = min(n, c0 + per);
+  int sum = 0;
+  for (int c = c0; c < c1; c++) {
+    int cnt = d.col_count[c];
+    if (cnt > MAX_COLUMN_ATOMS) bad = 1;
+    sum += (cnt + CLUSTER - 1) / CLUSTER;
+  }
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+    int v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - sum;
+  for (int c = c0; c < c1; c++) {
+    d.col_clu0[c] = run;
+    d.col_fill[c] = 0;
+    run += (d.col_count[c] + CLUSTER - 1) / CLUSTER;
+  }
+  if (threadIdx.x == 1023) {
+    int total = part[1023];
+    d.col_clu0[n] = total;
+    *d.n_clusters = total;
+    int ov = 0;
+    if (bad) ov |= OV_COLUMN;
+    if (total > d.n_clusters_cap || total * CLUSTER > d.n_slots_cap) ov |= OV_SLOTS;
+    if (ov) atomicOr(d.flags + F_OVERFLOW, ov);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  SYS_FROM(pre);
+  if (skip_build(d)) return;
+  const int i = lb * 256 + threadIdx.x;
+  if (i >= d.n_atoms) return;
+  int c = d.atom_col[i];
+  int p = atomicAdd(d.col_fill + c, 1);
+  d.bin_tmp[d.col_clu0[c] * CLUSTER + p] = i;
+}
+
+// one block per column: sort by (u_z, id), lay out slots, snapshot positions,
+// cluster bounding boxes
+__global__ __launch_bounds__(256) void k_col_sort(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  SYS_FROM(pre);
+  if (skip_build(d)) return;
+  const int c = lb;
+  __shared__ unsigned long long key[MAX_COLUMN_ATOMS];
+  __shared__ int ids[MAX_COLUMN_ATOMS];
+  const int n = d.col_count[c];
+  const int clu0 = d.col_clu0[c];
+  const int nclu = d.col_clu0[c + 1] - clu0;
+  const int base = clu0 * CLUSTER;
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int k = threadIdx.x; k < P; k += 256) {
+    if (k < n) {
+      int a = d.bin_tmp[base + k];
+      ids[k] = a;
+      key[k] = d.uz[a];
+    } else {
+      ids[k] = 0x7fffffff;
+      key[k] = ~0ull;
+    }
+  }
+  __syncthreads();
+  // bitonic sort of (key, id) ascending
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < P / 2; t += 256) {
+        int lo = 2 * t - (t & (stride - 1));
+        int hi = lo + stride;
+        bool up = (lo & size) == 0;
+        unsigned long long ka = key[lo], kb = key[hi];
+        int ia = ids[lo], ib = ids[hi];
+        bool gt = ka > kb || (ka == kb && ia > ib);
+        if (gt == up) {
+          key[lo] = kb; key[hi] = ka;
+          ids[lo] = ib; ids[hi] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = threadIdx.x; k < nclu * CLUSTER; k += 256) {
+    int a = k < n ? ids[k] : -1;
+    d.slot_atom[base + k] = a;
+    if (a >= 0) {
+      d.atom_slot[a] = base + k;
+      d.rx[a] = d.ux[a];
+      d.ry[a] = d.uy[a];
+      d.rz[a] = d.uz[a];
+    }
+  }
+  for (int m = threadIdx.x; m < nclu; m += 256) {
+    unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0, 0, 0};
+    for (int k = m * CLUSTER; k < min(n, (m + 1) * CLUSTER); k++) {
+      int a = ids[k];
+      unsigned long long v[3] = {d.ux[a], d.uy[a], key[k]};
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        lo[e] = v[e] < lo[e] ? v[e] : lo[e];
+        hi[e] = v[e] > hi[e] ? v[e] : hi[e];
+      }
+    }
+    unsigned long long* b = d.bb + 6 * (size_t)(clu0 + m);
+#pragma unroll
+    for (int e = 0; e < 3; e++) { b[2 * e] = lo[e]; b[2 * e + 1] = hi[e]; }
+    d.clu_col[clu0 + m] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    d.col_count[c] = 0;
+    d.col_fill[c] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// periodic gap between [alo, ahi] and [blo, bhi] on the 2^64 circle
+__device__ __forceinline__ unsigned long long axis_gap(unsigned long long alo, unsigned long long ahi,
+                                                       unsigned long long blo, unsigned long long bhi) {
+  if (alo <= bhi && blo <= ahi) return 0ull;
+  unsigned long long d1 = blo - ahi, d2 = alo - bhi;
+  return d1 < d2 ? d1 : d2;
+}
+
+__device__ __forceinline__ bool bbox_within(const unsigned long long* a, const unsigned long long* b, const SysDev& d) {
+#pragma clang fp contract(off)
+  double gx = (double)axis_gap(a[0], a[1], b[0], b[1]) * d.sc[0];
+  double gy = (double)axis_gap(a[2], a[3], b[2], b[3]) * d.sc[1];
+  double gz = (double)axis_gap(a[4], a[5], b[4], b[5]) * d.sc[2];
+  double d2 = gx * gx + gy * gy;
+  d2 = d2 + gz * gz;
+  return d2 < d.rl2;
+}
+
+// conservative (never larger than exact) periodic gap in Angstrom between an
+// interval [a0, a1] and column index c of width w on a circle of length L
+__device__ __forceinline__ double col_gap(double a0, double a1, int c, double w, double L) {
+  double b0 = c * w - 1e-9, b1 = (c + 1) * w + 1e-9;
+  if (a0 <= b1 && b0 <= a1) return 0.0;
+  double d1 = b0 - a1, d2 = a0 - b1;
+  d1 -= floor(d1 / L) * L;
+  d2 -= floor(d2 / L) * L;
+  return fmax(0.0, fmin(d1, d2) - 1e-9);
+}
+
+// one wave per I cluster: J >= I with bounding-box gap < pair-list distance
+__global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  SYS_FROM(pre);
+  if (skip_build(d)) return;
+  const int lane = threadIdx.x & 63;
+  const int I = lb * 4 + (threadIdx.x >> 6);
+  if (I >= *d.n_clusters) return;
+  unsigned long long bi[6];
+#pragma unroll
+  for (int e = 0; e < 6; e++) bi[e] = d.bb[6 * (size_t)I + e];
+  const int cI = d.clu_col[I];
+  const int cx = cI % d.ncx, cy = cI / d.ncx;
+  const double ax0 = (double)bi[0] * d.sc[0], ax1 = (double)bi[1] * d.sc[0];
+  const double ay0 = (double)bi[2] * d.sc[1], ay1 = (double)bi[3] * d.sc[1];
+  const double wx = d.L[0] / d.ncx, wy = d.L[1] / d.ncy;
+  const bool allx = 2 * d.kx_range + 1 >= d.ncx, ally = 2 * d.ky_range + 1 >= d.ncy;
+  const int nyv = ally ? d.ncy : 2 * d.ky_range + 1;
+  const int nxv = allx ? d.ncx : 2 * d.kx_range + 1;
+  int* out = d.search_tmp + (size_t)I * SEARCH_TMP_CAP;
+  int count = 0;
+  bool over = false;
+  for (int iy = 0; iy < nyv; iy++) {
+    int yy = ally ? iy : (cy + iy - d.ky_range + 2 * d.ncy) % d.ncy;
+    double gy = col_gap(ay0, ay1, yy, wy, d.L[1]);
+    if (gy * gy >= d.rl2) continue;
+    for (int ix = 0; ix < nxv; ix++) {
+      int xx = allx ? ix : (cx + ix - d.kx_range + 2 * d.ncx) % d.ncx;
+      double gx = col_gap(ax0, ax1, xx, wx, d.L[0]);
+      if (gx * gx + gy * gy >= d.rl2) continue;
+      const int C = yy * d.ncx + xx;
+      int j0 = d.col_clu0[C], j1 = d.col_clu0[C + 1];
+      if (j0 < I) j0 = I;
+      for (int jb = j0; jb < j1; jb += 64) {
+        const int J = jb + lane;
+        bool hit = false;
+        if (J < j1) {
+          unsigned long long bj[6];
+#pragma unroll
+          for (int e = 0; e < 6; e++) bj[e] = d.bb[6 * (size_t)J + e];
+          hit = bbox_within(bi, bj, d);
+        }
+        unsigned long long bal = __ballot(hit);
+        int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
+        if (hit) {
+          if (pos < SEARCH_TMP_CAP) out[pos] = J;
+          else over = true;
+        }
+        count += __popcll(bal);
+      }
+    }
+  }
+  if (lane == 0) {
+    d.search_cnt[I] = count < SEARCH_TMP_CAP ? count : SEARCH_TMP_CAP;
+    if (count > SEARCH_TMP_CAP) atomicOr(d.flags + F_OVERFLOW, OV_SEARCH);
+  }
+  (void)over;
+}
+
+// one block per system: exclusive scan of per-cluster counts into list offsets
+__global__ __launch_bounds__(1024) void k_pair_scan(const SysDev* __restrict__ S, int nsys) {
+  const SysDev& d = S[blockIdx.x];
+  if (skip_build(d)) return;
+  __shared__ long long part[1024];
+  const int n = *d.n_clusters;
+  const int per = (n + 1023) / 1024;
+  const int c0 = threadIdx.x * per, c1 = min(n, c0 + per);
+  long long sum = 0;
+  for (int c = c0; c < c1; c++) sum += d.search_cnt[c];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  long long run = part[threadIdx.x] - sum;
+  for (int c = c0; c < c1; c++) {
+    d.pl_off[c] = run;
+    run += d.search_cnt[c];
+  }
+  if (threadIdx.x == 1023) {
+    long long total = part[1023];
+    d.pl_off[n] = total;
+    if (total > d.n_pairs_cap) {
+      atomicOr(d.flags + F_OVERFLOW, OV_PAIRS);
+      *d.n_pairs = 0;
+    } else {
+      *d.n_pairs = total;
+    }
+    d.flags[F_NBUILD] += 1;
+  }
+}
+
+__device__ __forceinline__ bool is_excluded(const SysDev& d, int a, int b) {
+  if (!d.use_excl_list) {
+    int off = b - a;
+    if (off < -32 || off > 31) return false;
+    return (d.excl_bits[a] >> (off + 32)) & 1ull;
+  }
+  int lo = d.excl_off[a], hi = d.excl_off[a + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    int v = d.excl_list[mid];
+    if (v < b) lo = mid + 1; else hi = mid;
+  }
+  return lo < d.excl_off[a + 1] && d.excl_list[lo] == b;
+}
+
+// one wave per I cluster: copy the J list into place and build the lane masks
+__global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  SYS_FROM(pre);
+  if (skip_build(d)) return;
+  const int lane = threadIdx.x & 63;
+  const int I = lb * 4 + (threadIdx.x >> 6);
+  if (I >= *d.n_clusters) return;
+  const int il = lane & 7, jl = lane >> 3;
+  const int a = d.slot_atom[I * CLUSTER + il];
+  const long long off = d.pl_off[I];
+  const int cnt = d.search_cnt[I];
+  const int* src = d.search_tmp + (size_t)I * SEARCH_TMP_CAP;
+  for (int k = 0; k < cnt; k++) {
+    const int J = src[k];
+    const int b = d.slot_atom[J * CLUSTER + jl];
+    bool on = a >= 0 && b >= 0 && (I != J || il < jl);
+    if (on) on = !is_excluded(d, a, b);
+    unsigned long long m = __ballot(on);
+    if (lane == 0) {
+      d.pl_i[off + k] = I;
+      d.pl_j[off + k] = J;
+      d.pl_mask[off + k] = m;
+    }
+  }
+}
+
+// single block: per-system pair-kernel work items (chunks of PAIR_CHUNK entries)
+__global__ void k_chunk_prefix(const SysDev* __restrict__ S, int nsys, int* __restrict__ chunk_pre) {
+  if (threadIdx.x != 0) return;
+  int run = 0;
+  chunk_pre[0] = 0;
+  for (int s = 0; s < nsys; s++) {
+    long long np = S[s].flags[F_OVERFLOW] ? 0 : *S[s].n_pairs;
+    run += (int)((np + PAIR_CHUNK - 1) / PAIR_CHUNK);
+    chunk_pre[s + 1] = run;
+  }
+}
+
+// ---------------------------------------------------------------------------
+void launch_nblist(const Launch& L, bool always_rebuild_check) {
+  (void)always_rebuild_check;
+  hipLaunchKernelGGL(k_check, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  hipLaunchKernelGGL(k_bin_count, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  hipLaunchKernelGGL(k_col_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.d_sys, L.nsys);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  hipLaunchKernelGGL(k_col_sort, dim3(L.w_cols.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_cols.pre);
+  hipLaunchKernelGGL(k_pair_search, dim3(L.w_clus.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_clus.pre);
+  hipLaunchKernelGGL(k_pair_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.d_sys, L.nsys);
+  hipLaunchKernelGGL(k_pair_fill, dim3(L.w_clus.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_clus.pre);
+  hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(64), 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+}
+
+}  // namespace fz

@@ -1,0 +1,115 @@
+// fz_sys.h -- per-system device descriptor shared by the host engine and the
+// gfx950 kernels.  One SysDev describes one periodic system resident in HBM;
+// a batch is an array of them plus per-launch block-prefix tables, so every
+// kernel runs unchanged on one system or on many.
+#pragma once
+#include <stdint.h>
+
+namespace fz {
+
+constexpr int CLUSTER = 8;          // atoms per cluster (8 x 8 pair tile = one wave64)
+constexpr int PAIR_CHUNK = 32;      // cluster-pair entries per pair-kernel work item
+constexpr int SEARCH_TMP_CAP = 512;  // max J clusters per I cluster in the search buffer
+constexpr int MAX_COLUMN_ATOMS = 2048; // LDS sort capacity per column
+constexpr int MAX_FFT = 256;        // largest PME grid dimension
+
+// Energy / virial accumulator slots (int64 fixed point, scale 2^32).
+enum AccSlot {
+  A_BOND = 0, A_ANGLE, A_UB, A_IMPR, A_DIHED, A_VDW, A_ELEC, A_PME, A_EWEXCL,
+  A_KIN, A_VIR /* 9 slots */, A_NPAIR = A_VIR + 9, A_MOM /* 3 slots */, A_FMAX = A_MOM + 3,
+  A_NSLOT = A_FMAX + 1
+};
+
+// Device-side status flags.
+enum FlagSlot { F_REBUILD = 0, F_OVERFLOW, F_SHAKE_FAIL, F_NBUILD, F_NSLOT };
+
+enum OverflowBits {
+  OV_SLOTS = 1, OV_COLUMN = 2, OV_SEARCH = 4, OV_PAIRS = 8
+};
+
+struct SysDev {
+  // ---- sizes -----------------------------------------------------------
+  int n_atoms;
+  int n_types;
+  int n_slots_cap;       // cluster slots (8 per cluster) capacity
+  int n_clusters_cap;
+  long long n_pairs_cap; // cluster-pair list capacity
+  int ncx, ncy, ncol;    // column grid
+  int kx_range, ky_range; // column search half-widths
+  unsigned long long colw_x, colw_y; // floor(2^64 / ncx), floor(2^64 / ncy)
+  int K[3];              // PME grid
+  int Kxh;               // K[0]/2 + 1
+  int use_excl_list;     // 1: exclusions by sorted list, 0: 64-bit window bit vectors
+  int n_ccl;             // constraint clusters
+  // bonded term counts (flat index order: bonds, angles, dihedrals, impropers, ew pairs, 1-4 pairs)
+  int n_bonds, n_angles, n_diheds, n_imprs, n_ew, n_p14;
+  // ---- geometry / method parameters ----------------------------------
+  double L[3];
+  double sc[3];          // L * 2^-64: fixed -> Angstrom
+  double isc[3];         // 2^64 / L
+  float sc32[3];         // L * 2^-32 (32-bit top words)
+  double rc2, ron2, rl2, skin2;   // cutoff^2, switch-on^2, pair-list^2, (0.5*(rl-rc))^2
+  double beta;
+  double sw_inv;         // 1 / (rc^2 - ron^2)^3
+  double dt;             // AKMA time step (TU * dt_ps)
+  double shake_tol;
+  double pme_pre;        // CC / (pi V)
+  double pi2_beta2;      // pi^2 / beta^2
+  // ---- per-atom state (atom id order) --------------------------------
+  unsigned long long *ux, *uy, *uz;   // fixed-point fractional positions
+  unsigned long long *rx, *ry, *rz;   // positions at the last list build
+  unsigned long long *ox, *oy, *oz;   // positions before the drift (SHAKE reference)
+  double *vx, *vy, *vz;
+  double *q, *minv;
+  int *type;
+  // ---- cluster-sorted arrays (slot order) ----------------------------
+  int *slot_atom;        // [n_slots_cap], -1 = padding
+  int *atom_slot;        // [n_atoms]
+  void *xs;              // packed positions+charge per slot (int4 / longlong4)
+  int *ts;               // type per slot
+  void *Fs;              // force accumulators, 3 x n_slots_cap (x block, y block, z block)
+  // ---- binning / search --------------------------------------------------
+  int *atom_col;         // [n_atoms]
+  int *col_count, *col_fill, *col_clu0; // [ncol], [ncol], [ncol + 1]
+  int *bin_tmp;          // [n_slots_cap] column-scattered atoms
+  int *clu_col;          // [n_clusters_cap]
+  unsigned long long *bb; // [6 * n_clusters_cap] lo/hi per axis
+  int *n_clusters;       // device scalar
+  int *search_cnt;       // [n_clusters_cap + 1]
+  int *search_tmp;       // [n_clusters_cap * SEARCH_TMP_CAP]
+  long long *pl_off;     // [n_clusters_cap + 1]
+  int *pl_i, *pl_j;      // [n_pairs_cap]
+  unsigned long long *pl_mask; // [n_pairs_cap]
+  long long *n_pairs;    // device scalar
+  // ---- exclusions ---------------------------------------------------------
+  unsigned long long *excl_bits; // [n_atoms]: bit (32 + j - i) set if j excluded, |j-i| <= 31
+  int *excl_off, *excl_list;     // CSR (used when use_excl_list)
+  // ---- bonded terms -------------------------------------------------------
+  int2 *bond_ij;   double2 *bond_p;       // (k, r0)
+  int4 *angle_ijk; double4 *angle_p;      // (k, theta0, k_ub, s0)
+  int4 *dihe_ijkl; double2 *dihe_p; int *dihe_n; // (k, delta)
+  int4 *impr_ijkl; double *impr_k;
+  int2 *ew_ij;                            // Ewald-corrected exclusions (1-2, 1-3)
+  int2 *p14_ij;                           // 1-4 pairs
+  // ---- LJ tables --------------------------------------------------------------
+  float2 *lj32;   double2 *lj64;          // (eps, sigma^2) per type pair
+  double2 *lj14;                          // 1-4 (eps, sigma^2)
+  // ---- PME --------------------------------------------------------------------
+  void *Qacc;      // K0*K1*K2 accumulators (double or fixed int64)
+  void *Qc;        // complex half grid [K2][K1][Kxh] (float2 / double2)
+  void *phi;       // real potential grid [K2][K1][K0] (float / double)
+  double *pme_bsp; // per-axis |b(m)|^2, [K0 + K1 + K2]
+  double *pme_ex;  // per-axis exp(-pi^2 m_d^2 / beta^2), [K0 + K1 + K2]
+  double *pme_m;   // per-axis signed m_d (1/A), [K0 + K1 + K2]
+  void *tw;        // twiddles e^{-2 pi i k / K_d}, [K0 + K1 + K2] complex
+  // ---- constraints --------------------------------------------------------
+  int4 *ccl_atoms;    // up to 4 atoms (-1 = none)
+  int4 *ccl_cons;     // x,y,z: constraint c as (local a | local b << 8); w: count
+  double4 *ccl_d2;    // squared lengths (up to 3)
+  // ---- accumulators / flags ---------------------------------------------
+  unsigned long long *acc;  // [A_NSLOT] int64 fixed point (A_FMAX: bits of max |F|^2)
+  int *flags;               // [F_NSLOT]
+  double *mstep;            // minimiser: largest atom displacement of the next move (A)
+};
+
+}  // namespace fz

@@ -1,0 +1,396 @@
+"""ctypes binding of libfenzi (include/fenzi.h) for tests, bench and scripts.
+
+The library is the product; this module only marshals numpy arrays across the
+C ABI.  It raises if libfenzi.so is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfenzi.so")
+
+FP64, MIXED = 0, 1
+E_FIELDS = ["bond", "angle", "urey_bradley", "improper", "dihedral", "vdw", "elec", "pme",
+            "ewald_excl", "ewald_self", "potential", "kinetic"]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint64)
+
+
+class Topology(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("n_types", C.c_int32),
+                ("charge", _dp), ("mass", _dp), ("type", _ip),
+                ("lj_eps", _dp), ("lj_sig2", _dp), ("lj14_eps", _dp), ("lj14_sig2", _dp),
+                ("n_bonds", C.c_int32), ("bond_atoms", _ip), ("bond_k", _dp), ("bond_r0", _dp),
+                ("n_angles", C.c_int32), ("angle_atoms", _ip), ("angle_k", _dp), ("angle_theta0", _dp),
+                ("ub_k", _dp), ("ub_s0", _dp),
+                ("n_dihedrals", C.c_int32), ("dihedral_atoms", _ip), ("dihedral_k", _dp),
+                ("dihedral_n", _ip), ("dihedral_delta", _dp),
+                ("n_impropers", C.c_int32), ("improper_atoms", _ip), ("improper_k", _dp),
+                ("improper_psi0", _dp),
+                ("n_constraints", C.c_int32), ("constraint_atoms", _ip), ("constraint_length", _dp)]
+
+
+class Params(C.Structure):
+    _fields_ = [("box", C.c_double * 3), ("cutoff", C.c_double), ("cuton", C.c_double),
+                ("pairlist", C.c_double), ("beta", C.c_double), ("pme_grid", C.c_int32 * 3),
+                ("pme_order", C.c_int32), ("dt_ps", C.c_double), ("shake_tol", C.c_double),
+                ("column_grid", C.c_int32 * 2)]
+
+
+class Energies(C.Structure):
+    _fields_ = [("step", C.c_int64), ("temperature", C.c_double)] + \
+               [(f, C.c_double) for f in E_FIELDS] + \
+               [("virial", C.c_double * 9), ("pairs_within_cutoff", C.c_int64), ("list_builds", C.c_int64)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f in E_FIELDS}
+        d["temperature"] = self.temperature
+        d["step"] = self.step
+        d["virial"] = np.array(self.virial[:]).reshape(3, 3)
+        d["pairs_within_cutoff"] = self.pairs_within_cutoff
+        d["list_builds"] = self.list_builds
+        return d
+
+
+class RunSettings(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("dynasteps", C.c_int64), ("minimizesteps", C.c_int64),
+                ("startstep", C.c_int64), ("tempreset_every", C.c_int64), ("output_every", C.c_int64),
+                ("seed", C.c_uint64), ("shake", C.c_int32), ("print_to_stdout", C.c_int32),
+                ("output_prefix", C.c_char * 256)]
+
+
+# every symbol include/fenzi.h declares
+EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create", "fz_system_destroy",
+           "fz_set_state", "fz_get_state", "fz_set_fixed_positions", "fz_get_fixed_positions",
+           "fz_reassign_velocities", "fz_compute_forces", "fz_step", "fz_get_energies", "fz_minimize",
+           "fz_system_info", "fz_get_cluster_layout", "fz_get_cluster_pairs", "fz_pme_convolve",
+           "fz_batch_create", "fz_batch_step", "fz_batch_minimize", "fz_batch_destroy", "fz_deck_load", "fz_deck_destroy",
+           "fz_deck_topology", "fz_deck_params", "fz_deck_coordinates", "fz_deck_settings",
+           "fz_deck_psf_counts", "fz_format_energy_header", "fz_format_energy_row"]
+
+_lib = None
+
+
+class FenziError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FenziError("libfenzi.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.fz_last_error.restype = C.c_char_p
+        L.fz_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.fz_ctx_destroy.argtypes = [vp]
+        L.fz_system_create.argtypes = [vp, C.POINTER(Topology), C.POINTER(Params), C.c_int, C.POINTER(vp)]
+        L.fz_system_destroy.argtypes = [vp]
+        L.fz_set_state.argtypes = [vp, _dp, _dp]
+        L.fz_get_state.argtypes = [vp, _dp, _dp]
+        L.fz_set_fixed_positions.argtypes = [vp, _up]
+        L.fz_get_fixed_positions.argtypes = [vp, _up]
+        L.fz_reassign_velocities.argtypes = [vp, C.c_double, C.c_uint64, C.c_uint64]
+        L.fz_compute_forces.argtypes = [vp, _dp, C.POINTER(Energies)]
+        L.fz_step.argtypes = [vp, C.c_int]
+        L.fz_get_energies.argtypes = [vp, C.POINTER(Energies)]
+        L.fz_minimize.argtypes = [vp, C.c_int, C.c_double]
+        L.fz_system_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+        L.fz_get_cluster_layout.argtypes = [vp, _ip, _ip, _ip, C.c_int64, C.POINTER(C.c_int64)]
+        L.fz_get_cluster_pairs.argtypes = [vp, _ip, _up, C.c_int64]
+        L.fz_get_cluster_pairs.restype = C.c_int64
+        L.fz_pme_convolve.argtypes = [vp, _ip, _dp, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.fz_batch_create.argtypes = [vp, C.POINTER(vp), C.c_int, C.POINTER(vp)]
+        L.fz_batch_step.argtypes = [vp, C.c_int]
+        L.fz_batch_minimize.argtypes = [vp, C.c_int, C.c_double]
+        L.fz_batch_destroy.argtypes = [vp]
+        L.fz_deck_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.fz_deck_destroy.argtypes = [vp]
+        L.fz_deck_topology.argtypes = [vp]
+        L.fz_deck_topology.restype = C.POINTER(Topology)
+        L.fz_deck_params.argtypes = [vp]
+        L.fz_deck_params.restype = C.POINTER(Params)
+        L.fz_deck_coordinates.argtypes = [vp]
+        L.fz_deck_coordinates.restype = _dp
+        L.fz_deck_settings.argtypes = [vp, C.POINTER(RunSettings)]
+        L.fz_deck_psf_counts.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.fz_format_energy_header.argtypes = [C.c_char_p, C.c_size_t]
+        L.fz_format_energy_row.argtypes = [C.POINTER(Energies), C.c_double, C.c_char_p, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise FenziError("%s: %s (code %d)" % (what, lib().fz_last_error().decode(), rc))
+    return rc
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def make_topology(topo, keep):
+    """fz_topology over numpy arrays of a topology dict (fz_synth layout)."""
+    def arr(name, dt):
+        a = np.ascontiguousarray(topo[name], dtype=dt)
+        if a.size == 0:
+            a = np.zeros(1, dtype=dt)
+        keep.append(a)
+        return _p(a, C.c_int32 if dt == np.int32 else C.c_double)
+
+    t = Topology()
+    t.n_atoms, t.n_types = int(topo["n_atoms"]), int(topo["n_types"])
+    for f in ("charge", "mass", "lj_eps", "lj_sig2", "lj14_eps", "lj14_sig2", "bond_k", "bond_r0",
+              "angle_k", "angle_theta0", "ub_k", "ub_s0", "dihedral_k", "dihedral_delta",
+              "improper_k", "improper_psi0", "constraint_length"):
+        setattr(t, f, arr(f, np.float64))
+    for f in ("type", "bond_atoms", "angle_atoms", "dihedral_atoms", "dihedral_n", "improper_atoms",
+              "constraint_atoms"):
+        setattr(t, f, arr(f, np.int32))
+    for k in ("bonds", "angles", "dihedrals", "impropers", "constraints"):
+        setattr(t, "n_" + k, int(topo["n_" + k]))
+    return t
+
+
+def make_params(params):
+    p = Params()
+    p.box[:] = [float(x) for x in params["box"]]
+    p.cutoff, p.cuton, p.pairlist, p.beta = (float(params[k]) for k in ("cutoff", "cuton", "pairlist", "beta"))
+    p.pme_grid[:] = [int(k) for k in params["K"]]
+    p.pme_order = 4
+    p.dt_ps = float(params.get("dt_ps", 0.001))
+    p.shake_tol = float(params.get("shake_tol", 1e-8))
+    cg = params.get("column_grid", (0, 0))
+    p.column_grid[:] = [int(cg[0]), int(cg[1])]
+    return p
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().fz_ctx_create(device, C.byref(self.h)), "fz_ctx_create")
+
+    def close(self):
+        if self.h:
+            lib().fz_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pme_convolve(self, q, box, beta, precision=FP64, mode=0):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        K = np.array([q.shape[2], q.shape[1], q.shape[0]], np.int32)
+        phi = np.empty_like(q)
+        e = C.c_double(0)
+        b = np.asarray(box, np.float64)
+        _check(lib().fz_pme_convolve(self.h, _p(K, C.c_int32), _p(b, C.c_double), float(beta), precision, mode,
+                                     _p(q, C.c_double), _p(phi, C.c_double), C.byref(e)), "fz_pme_convolve")
+        return phi, e.value
+
+
+class System:
+    def __init__(self, ctx, topo, params, precision=FP64, xyz=None, vel=None):
+        self.ctx = ctx
+        self._keep = []
+        self.n = int(topo["n_atoms"])
+        t = make_topology(topo, self._keep)
+        p = make_params(params)
+        self.h = C.c_void_p()
+        _check(lib().fz_system_create(ctx.h, C.byref(t), C.byref(p), precision, C.byref(self.h)), "fz_system_create")
+        self.precision = precision
+        self.dt_ps = p.dt_ps
+        if xyz is not None:
+            self.set_state(xyz, vel)
+
+    def close(self):
+        if self.h:
+            lib().fz_system_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, xyz, vel=None):
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n, 3)
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(self.n, 3)
+        _check(lib().fz_set_state(self.h, _p(x, C.c_double), None if v is None else _p(v, C.c_double)),
+               "fz_set_state")
+
+    def get_state(self):
+        x = np.empty((self.n, 3))
+        v = np.empty((self.n, 3))
+        _check(lib().fz_get_state(self.h, _p(x, C.c_double), _p(v, C.c_double)), "fz_get_state")
+        return x, v
+
+    def set_fixed(self, u):
+        u = np.ascontiguousarray(u, dtype=np.uint64).reshape(self.n, 3)
+        _check(lib().fz_set_fixed_positions(self.h, _p(u, C.c_uint64)), "fz_set_fixed_positions")
+
+    def get_fixed(self):
+        u = np.empty((self.n, 3), np.uint64)
+        _check(lib().fz_get_fixed_positions(self.h, _p(u, C.c_uint64)), "fz_get_fixed_positions")
+        return u
+
+    def reassign_velocities(self, T, seed, counter=0):
+        _check(lib().fz_reassign_velocities(self.h, float(T), int(seed), int(counter)), "fz_reassign_velocities")
+
+    def forces(self):
+        f = np.empty((self.n, 3))
+        e = Energies()
+        _check(lib().fz_compute_forces(self.h, _p(f, C.c_double), C.byref(e)), "fz_compute_forces")
+        return f, e.as_dict()
+
+    def step(self, n):
+        _check(lib().fz_step(self.h, int(n)), "fz_step")
+
+    def minimize(self, n, max_step=0.05):
+        _check(lib().fz_minimize(self.h, int(n), float(max_step)), "fz_minimize")
+
+    def energies(self):
+        e = Energies()
+        _check(lib().fz_get_energies(self.h, C.byref(e)), "fz_get_energies")
+        return e.as_dict()
+
+    def info(self):
+        n, b, p = C.c_int64(), C.c_int64(), C.c_int32()
+        _check(lib().fz_system_info(self.h, C.byref(n), C.byref(b), C.byref(p)), "fz_system_info")
+        return dict(n_atoms=n.value, device_bytes=b.value, precision=p.value)
+
+    def cluster_layout(self):
+        g = np.zeros(2, np.int32)
+        col = np.zeros(self.n, np.int32)
+        ns = C.c_int64()
+        _check(lib().fz_get_cluster_layout(self.h, _p(g, C.c_int32), _p(col, C.c_int32), None, 0, C.byref(ns)),
+               "fz_get_cluster_layout")
+        slots = np.zeros(max(1, ns.value), np.int32)
+        _check(lib().fz_get_cluster_layout(self.h, None, None, _p(slots, C.c_int32), ns.value, C.byref(ns)),
+               "fz_get_cluster_layout")
+        return (int(g[0]), int(g[1])), col, slots[: ns.value]
+
+    def cluster_pairs(self):
+        n = _check(lib().fz_get_cluster_pairs(self.h, None, None, 0), "fz_get_cluster_pairs")
+        pairs = np.zeros((max(1, n), 2), np.int32)
+        masks = np.zeros(max(1, n), np.uint64)
+        n2 = _check(lib().fz_get_cluster_pairs(self.h, _p(pairs, C.c_int32), _p(masks, C.c_uint64), n),
+                    "fz_get_cluster_pairs")
+        return pairs[:n2], masks[:n2]
+
+
+class Batch:
+    def __init__(self, ctx, systems):
+        self.ctx = ctx
+        self.systems = list(systems)
+        arr = (C.c_void_p * len(self.systems))(*[s.h.value for s in self.systems])
+        self.h = C.c_void_p()
+        _check(lib().fz_batch_create(ctx.h, arr, len(self.systems), C.byref(self.h)), "fz_batch_create")
+
+    def step(self, n):
+        _check(lib().fz_batch_step(self.h, int(n)), "fz_batch_step")
+
+    def minimize(self, n, max_step=0.05):
+        _check(lib().fz_batch_minimize(self.h, int(n), float(max_step)), "fz_batch_minimize")
+
+    def close(self):
+        if self.h:
+            lib().fz_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Deck:
+    def __init__(self, path):
+        self.h = C.c_void_p()
+        _check(lib().fz_deck_load(path.encode(), C.byref(self.h)), "fz_deck_load")
+
+    def close(self):
+        if self.h:
+            lib().fz_deck_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def topology(self):
+        t = lib().fz_deck_topology(self.h).contents
+        n = t.n_atoms
+
+        def a(ptr, cnt, dt=np.float64):
+            if cnt == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(ptr, shape=(cnt,)).astype(dt).copy()
+        T = t.n_types
+        return dict(
+            n_atoms=n, n_types=T, charge=a(t.charge, n), mass=a(t.mass, n), type=a(t.type, n, np.int32),
+            lj_eps=a(t.lj_eps, T * T), lj_sig2=a(t.lj_sig2, T * T), lj14_eps=a(t.lj14_eps, T * T),
+            lj14_sig2=a(t.lj14_sig2, T * T),
+            n_bonds=t.n_bonds, bond_atoms=a(t.bond_atoms, 2 * t.n_bonds, np.int32), bond_k=a(t.bond_k, t.n_bonds),
+            bond_r0=a(t.bond_r0, t.n_bonds),
+            n_angles=t.n_angles, angle_atoms=a(t.angle_atoms, 3 * t.n_angles, np.int32),
+            angle_k=a(t.angle_k, t.n_angles), angle_theta0=a(t.angle_theta0, t.n_angles),
+            ub_k=a(t.ub_k, t.n_angles), ub_s0=a(t.ub_s0, t.n_angles),
+            n_dihedrals=t.n_dihedrals, dihedral_atoms=a(t.dihedral_atoms, 4 * t.n_dihedrals, np.int32),
+            dihedral_k=a(t.dihedral_k, t.n_dihedrals), dihedral_n=a(t.dihedral_n, t.n_dihedrals, np.int32),
+            dihedral_delta=a(t.dihedral_delta, t.n_dihedrals),
+            n_impropers=t.n_impropers, improper_atoms=a(t.improper_atoms, 4 * t.n_impropers, np.int32),
+            improper_k=a(t.improper_k, t.n_impropers), improper_psi0=a(t.improper_psi0, t.n_impropers),
+            n_constraints=t.n_constraints,
+            constraint_atoms=a(t.constraint_atoms, 2 * t.n_constraints, np.int32),
+            constraint_length=a(t.constraint_length, t.n_constraints))
+
+    def params(self):
+        p = lib().fz_deck_params(self.h).contents
+        return dict(box=tuple(p.box[:]), cutoff=p.cutoff, cuton=p.cuton, pairlist=p.pairlist, beta=p.beta,
+                    K=tuple(p.pme_grid[:]), dt_ps=p.dt_ps, shake_tol=p.shake_tol)
+
+    def coordinates(self):
+        n = lib().fz_deck_topology(self.h).contents.n_atoms
+        return np.ctypeslib.as_array(lib().fz_deck_coordinates(self.h), shape=(n * 3,)).reshape(n, 3).copy()
+
+    def settings(self):
+        s = RunSettings()
+        _check(lib().fz_deck_settings(self.h, C.byref(s)), "fz_deck_settings")
+        return {f: getattr(s, f) for f, _ in RunSettings._fields_}
+
+    def psf_counts(self):
+        c = np.zeros(4, np.int64)
+        _check(lib().fz_deck_psf_counts(self.h, _p(c, C.c_int64)), "fz_deck_psf_counts")
+        return dict(zip(["bonds", "angles", "dihedrals", "impropers"], c.tolist()))
+
+
+def energy_header():
+    b = C.create_string_buffer(1024)
+    lib().fz_format_energy_header(b, 1024)
+    return b.value.decode()
+
+
+def energy_row(e: Energies, dt_ps):
+    b = C.create_string_buffer(1024)
+    lib().fz_format_energy_row(C.byref(e), float(dt_ps), b, 1024)
+    return b.value.decode()
+
+
+def ns_per_day(ms_per_step, dt_ps):
+    return 86400.0 / (ms_per_step * 1e-3) * dt_ps * 1e-3

@@ -1,0 +1,196 @@
+/*
+ * fenzi.h -- C ABI of libfenzi, an MI355X (gfx950) implementation of FEN ZI's
+ * per-step force-and-integrate path.
+ *
+ * The reference has no library interface: its hot path is a set of
+ * `void f(void)` host functions over global state inside one nvcc translation
+ * unit.  Each entry point below names the reference function(s) it replaces
+ * (paths relative to the reference source tree).
+ *
+ * Conventions
+ *   - Units are AKMA: Angstrom, kcal/mol, amu, e.  Velocities are in Angstrom
+ *     per AKMA time unit (1/20.4548 ps), as the reference stores them
+ *     (src/mdcuda.cu:27-92).  Time steps are given in ps (`timestep` keyword).
+ *   - Arrays of vectors are [n][3] row-major (x0 y0 z0 x1 ...).
+ *   - Every call returns 0 on success or a negative FZ_ERR_* code; the message
+ *     of the last failure on the calling thread is fz_last_error().  Nothing
+ *     calls exit().
+ *   - Handles own their device memory; caller arrays are copied, never kept.
+ *     Calls on one handle must not overlap; different handles may be used from
+ *     different threads.
+ */
+#ifndef FENZI_H
+#define FENZI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FZ_OK = 0,
+  FZ_ERR_ARG = -1,      /* invalid argument or unsupported input */
+  FZ_ERR_HIP = -2,      /* HIP runtime error */
+  FZ_ERR_CAPACITY = -3, /* device capacity exceeded and could not be grown */
+  FZ_ERR_PARSE = -4,    /* input deck parse error */
+  FZ_ERR_NOMEM = -5,    /* out of device or host memory */
+  FZ_ERR_SHAKE = -6,    /* SHAKE did not converge */
+  FZ_ERR_STATE = -7,    /* call not valid in the handle's current state */
+  FZ_ERR_BUDGET = -8    /* serving: job exceeds the HBM budget */
+};
+
+typedef enum { FZ_FP64 = 0, FZ_MIXED = 1 } fz_precision;
+
+typedef struct fz_ctx fz_ctx;
+typedef struct fz_system fz_system;
+typedef struct fz_batch fz_batch;
+typedef struct fz_server fz_server;
+typedef struct fz_deck fz_deck;
+
+/* Parsed topology (the products of ReadPsf_CHARMM + ReadPrm_CHARMM,
+ * src/prmread.cu:1442,2181).  Pair tables are already combined:
+ * eps_ij = 4 sqrt(eps_i eps_j), sigma_ij = sigma_i/2 + sigma_j/2 (or NBFIX),
+ * E_LJ = eps_ij ((sigma_ij/r)^12 - (sigma_ij/r)^6).  Energies: bond
+ * k (r - r0)^2; angle k (theta - theta0)^2 (radians); Urey-Bradley
+ * k_ub (S - S0)^2; dihedral k (1 + cos(n phi - delta)) (one entry per term);
+ * improper k psi^2 (psi0 kept for reference but unused, src/bonded_forces.cu:196).
+ * Exclusions are derived: 1-2 from bonds, 1-3 from angle ends, 1-4 from
+ * dihedral ends (src/prmread.cu:2928-3062). */
+typedef struct fz_topology {
+  int32_t n_atoms, n_types;
+  const double *charge, *mass;
+  const int32_t *type;
+  const double *lj_eps, *lj_sig2, *lj14_eps, *lj14_sig2; /* [n_types * n_types] */
+  int32_t n_bonds;       const int32_t *bond_atoms;       const double *bond_k, *bond_r0;
+  int32_t n_angles;      const int32_t *angle_atoms;      const double *angle_k, *angle_theta0, *ub_k, *ub_s0;
+  int32_t n_dihedrals;   const int32_t *dihedral_atoms;   const double *dihedral_k; const int32_t *dihedral_n; const double *dihedral_delta;
+  int32_t n_impropers;   const int32_t *improper_atoms;   const double *improper_k, *improper_psi0;
+  int32_t n_constraints; const int32_t *constraint_atoms; const double *constraint_length;
+} fz_topology;
+
+/* Run parameters (the `.in` keywords of ReadInput, src/prmread.cu:491-913). */
+typedef struct fz_params {
+  double box[3];          /* cellbasisvector1/2/3 (orthorhombic) */
+  double cutoff;          /* cutoff */
+  double cuton;           /* cuton (LJ switch start) */
+  double pairlist;        /* pairlistdist */
+  double beta;            /* KAPPa (Ewald splitting, 1/A) */
+  int32_t pme_grid[3];    /* pmegridsizex/y/z (2,3,5-smooth, <= 256) */
+  int32_t pme_order;      /* must be 4 */
+  double dt_ps;           /* timestep (ps) */
+  double shake_tol;       /* shake on <tol>: max |r^2 - d^2| / (2 d^2) */
+  int32_t column_grid[2]; /* binning columns (x, y); 0 = chosen automatically */
+} fz_params;
+
+typedef struct fz_energies {
+  int64_t step;
+  double temperature;     /* 2 KE / (RC (3N - 6 - N_constraints)) (src/evalprops.cu:221-225) */
+  double bond, angle, urey_bradley, improper, dihedral;
+  double vdw;             /* switched LJ, including 1-4 pairs */
+  double elec;            /* real-space erfc Coulomb, including 1-4 pairs */
+  double pme;             /* reciprocal-space energy */
+  double ewald_excl;      /* -erf correction of 1-2 / 1-3 pairs */
+  double ewald_self;      /* -CC beta / sqrt(pi) sum q^2 */
+  double potential;       /* sum of the terms above */
+  double kinetic;
+  double virial[9];       /* sum r_ij (x) F_ij + reciprocal tensor (no constraint part) */
+  int64_t pairs_within_cutoff; /* non-excluded pairs inside the cutoff, this evaluation */
+  int64_t list_builds;    /* neighbour-list builds so far */
+} fz_energies;
+
+/* ---- errors / context ------------------------------------------------- */
+const char *fz_last_error(void);
+/* Replaces parseArgs' cudaSetDevice (src/fenzi.cu:59-110). */
+int fz_ctx_create(int device, fz_ctx **out);
+void fz_ctx_destroy(fz_ctx *ctx);
+
+/* ---- one system ------------------------------------------------------------ */
+/* Replaces InitHostMem + InitMem + InitDeviceConstants + GenExcl_minimal_new +
+ * build_constraints (src/prmread.cu:86,1303,2928; src/mem.cu:49;
+ * src/cell_based_kernels.cu:1831). */
+int fz_system_create(fz_ctx *ctx, const fz_topology *topo, const fz_params *par, int precision, fz_system **out);
+void fz_system_destroy(fz_system *sys);
+/* Positions (any image; wrapped on input) and velocities (nullable = zero).
+ * Replaces the coordinate upload of InitMem and LoadCheckpoint's H2D copies
+ * (src/fenzi.cu:481-501).  Recomputes forces. */
+int fz_set_state(fz_system *sys, const double *xyz, const double *vel);
+/* Positions wrapped to [-L/2, L/2), velocities (either nullable). */
+int fz_get_state(fz_system *sys, double *xyz, double *vel);
+/* Exact 64-bit fixed-point fractional positions u = frac(x / L) * 2^64. */
+int fz_set_fixed_positions(fz_system *sys, const uint64_t *u);
+int fz_get_fixed_positions(fz_system *sys, uint64_t *u);
+/* Maxwell velocities at T (Philox4x32-10 stream (seed, counter)), total
+ * momentum removed, constraint-projected.  Replaces InitVelocity
+ * (src/init.cu:19) and its time(NULL) seeding. */
+int fz_reassign_velocities(fz_system *sys, double temperature, uint64_t seed, uint64_t counter);
+/* Forces (nullable, [n][3]) and energies (nullable) at the current positions.
+ * Replaces ComputeAccelGPU (src/mdcuda.cu:557) + EvalProps' reductions
+ * (src/evalprops.cu:17). */
+int fz_compute_forces(fz_system *sys, double *forces, fz_energies *e);
+/* nsteps of velocity Verlet (+ SHAKE/RATTLE); neighbour rebuilds decided on
+ * the device.  Replaces SingleStep (src/dynamics.cu:623). */
+int fz_step(fz_system *sys, int nsteps);
+/* Energies of the last evaluated step (temperature from current velocities). */
+int fz_get_energies(fz_system *sys, fz_energies *e);
+/* Steepest-descent / conjugate-gradient minimisation, nsteps evaluations.
+ * Replaces MinimizeStep (src/dynamics.cu:491). */
+int fz_minimize(fz_system *sys, int nsteps, double max_step);
+int fz_system_info(fz_system *sys, int64_t *n_atoms, int64_t *device_bytes, int32_t *precision);
+
+/* ---- inspection (parity tests) ------------------------------------------------ */
+/* Column grid and cluster layout of the last neighbour build: column of each
+ * atom, atom of each slot (-1 = padding).  Replaces the cell lists of
+ * CellBuild (src/cell_based_kernels.cu:198). */
+int fz_get_cluster_layout(fz_system *sys, int32_t *ncx_ncy, int32_t *atom_column, int32_t *slot_atom,
+                          int64_t slot_cap, int64_t *n_slots);
+/* Cluster-pair list (I, J) and 64-bit tile masks; returns the number of pairs
+ * (negative FZ_ERR_* on failure).  Replaces nblistd (src/cell_based_kernels.cu:1048). */
+int64_t fz_get_cluster_pairs(fz_system *sys, int32_t *pairs, uint64_t *masks, int64_t cap);
+
+/* PME convolution test entry: real grid q[K2][K1][K0] -> phi (unnormalised
+ * inverse DFT of G * DFT(q)); mode 0 = PME kernel G for (box, beta), mode 1 = G = 1.
+ * Returns the reciprocal energy 1/2 sum G |DFT(q)|^2 in *energy (mode 0). */
+int fz_pme_convolve(fz_ctx *ctx, const int32_t K[3], const double box[3], double beta, int precision,
+                    int mode, const double *q, double *phi, double *energy);
+
+/* ---- batch: many systems per launch ---------------------------------------------- */
+/* Systems stay owned by their handles; the batch advances them in place with
+ * one launch per phase per step (hipGraph-captured). */
+int fz_batch_create(fz_ctx *ctx, fz_system *const *systems, int n, fz_batch **out);
+int fz_batch_step(fz_batch *b, int nsteps);
+/* Steepest descent on every member (per-system step control), one launch per phase. */
+int fz_batch_minimize(fz_batch *b, int nsteps, double max_step);
+void fz_batch_destroy(fz_batch *b);
+
+/* ---- input deck (.in / .psf / .prm / .rtf / .crd) --------------------------------- */
+/* Replaces ReadInput, ReadCrd/ReadPdb/ReadXYZ, ReadPsf_CHARMM, ReadPrm_CHARMM
+ * (src/prmread.cu:491,331,410,285,1442,2181). */
+int fz_deck_load(const char *in_path, fz_deck **out);
+void fz_deck_destroy(fz_deck *deck);
+const fz_topology *fz_deck_topology(const fz_deck *deck);
+const fz_params *fz_deck_params(const fz_deck *deck);
+const double *fz_deck_coordinates(const fz_deck *deck);
+/* Run settings of the deck: temperature, dynasteps, minimizesteps,
+ * tempresetTiming, outputTiming, seed, shake flag, printtostdout. */
+typedef struct fz_run_settings {
+  double temperature;
+  int64_t dynasteps, minimizesteps, startstep, tempreset_every, output_every;
+  uint64_t seed;
+  int32_t shake, print_to_stdout;
+  char output_prefix[256];
+} fz_run_settings;
+int fz_deck_settings(const fz_deck *deck, fz_run_settings *out);
+/* Section counts of the PSF as read: bonds, angles, dihedrals, impropers. */
+int fz_deck_psf_counts(const fz_deck *deck, int64_t *counts);
+
+/* Energy table in the reference's format (print_output_headers / EvalProps,
+ * src/evalprops.cu:248-267,311-343).  Return the number of characters written. */
+int fz_format_energy_header(char *buf, size_t len);
+int fz_format_energy_row(const fz_energies *e, double dt_ps, char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

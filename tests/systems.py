@@ -1,0 +1,22 @@
+"""Small seeded test systems shared by the CPU and GPU tests."""
+import numpy as np
+
+import fz_synth
+
+
+def water_box(n_waters=300, L=21.0, seed=11):
+    topo, xyz = fz_synth.make_system(0, n_waters, (L, L, L), seed=seed)
+    params = dict(box=(L, L, L), cutoff=9.0, cuton=7.0, pairlist=10.5, beta=0.34, K=(24, 24, 24),
+                  dt_ps=0.001, shake_tol=1e-10)
+    return topo, xyz, params
+
+
+def slab(n_chains=6, n_waters=400, L=26.0, seed=5, K=(27, 27, 27)):
+    topo, xyz = fz_synth.make_system(n_chains, n_waters, (L, L, L), seed=seed)
+    params = dict(box=(L, L, L), cutoff=10.0, cuton=8.0, pairlist=11.5, beta=0.32, K=K,
+                  dt_ps=0.001, shake_tol=1e-10)
+    return topo, xyz, params
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))

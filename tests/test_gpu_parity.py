@@ -1,0 +1,186 @@
+"""GPU parity: libfenzi's HIP path against the FP64 oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import fenzi
+import oracle
+import systems
+
+pytestmark = pytest.mark.gpu
+
+
+def _pme_numpy(q, box, beta):
+    """Reference PME convolution with numpy.fft (a library routine)."""
+    K2, K1, K0 = q.shape
+    V = box[0] * box[1] * box[2]
+    Q = np.fft.fftn(q)
+    m = []
+    bsp = []
+    for K, L in ((K2, box[2]), (K1, box[1]), (K0, box[0])):
+        j = np.arange(K)
+        jj = np.where(j <= K // 2, j, j - K)
+        m.append(jj / L)
+        bsp.append(9.0 / (2.0 + np.cos(2 * np.pi * j / K)) ** 2)
+    mz, my, mx = np.meshgrid(m[0], m[1], m[2], indexing="ij")
+    m2 = mx ** 2 + my ** 2 + mz ** 2
+    B = bsp[0][:, None, None] * bsp[1][None, :, None] * bsp[2][None, None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        G = 332.0716 / (np.pi * V) * np.exp(-np.pi ** 2 * m2 / beta ** 2) / m2 * B
+    G[0, 0, 0] = 0.0
+    E = 0.5 * (G * np.abs(Q) ** 2).sum()
+    phi = np.real(np.fft.ifftn(G * Q)) * q.size
+    return phi, E
+
+
+@pytest.mark.parametrize("K", [(16, 16, 16), (12, 20, 30), (48, 48, 72), (32, 27, 25), (128, 40, 36)])
+@pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
+def test_pme_fft_identity_roundtrip(ctx, K, prec):
+    rng = np.random.default_rng(sum(K))
+    q = rng.standard_normal((K[2], K[1], K[0]))
+    phi, _ = ctx.pme_convolve(q, (30.0, 31.0, 32.0), 0.3, prec, mode=1)
+    tol = 1e-12 if prec == fenzi.FP64 else 2e-6
+    err = np.abs(phi - q * q.size).max() / (np.abs(q).max() * q.size)
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("K", [(16, 16, 16), (24, 20, 30), (48, 48, 72)])
+@pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
+def test_pme_convolution_vs_numpy(ctx, K, prec):
+    rng = np.random.default_rng(7 + K[0])
+    q = rng.standard_normal((K[2], K[1], K[0]))
+    box = (25.0, 27.0, 33.0)
+    phi, E = ctx.pme_convolve(q, box, 0.32, prec, mode=0)
+    phi_ref, E_ref = _pme_numpy(q, box, 0.32)
+    tol = 1e-11 if prec == fenzi.FP64 else 2e-6
+    assert np.abs(phi - phi_ref).max() / np.abs(phi_ref).max() < tol
+    assert abs(E - E_ref) / abs(E_ref) < tol
+
+
+def _system(topo, params, prec, ctx, xyz, orc):
+    s = fenzi.System(ctx, topo, params, prec)
+    s.set_fixed(orc.to_fixed(xyz))
+    return s
+
+
+@pytest.mark.parametrize("make", [systems.water_box, systems.slab])
+def test_cluster_layout_and_pairs_bitexact(ctx, make):
+    topo, xyz, params = make()
+    orc = oracle.Oracle(topo, params)
+    u = orc.to_fixed(xyz)
+    s = _system(topo, params, fenzi.FP64, ctx, xyz, orc)
+    (ncx, ncy), col, slots = s.cluster_layout()
+    pairs, masks = s.cluster_pairs()
+    ocol, oslots, opairs, omasks = orc.cluster_pairs(u, ncx, ncy)
+    assert np.array_equal(col, ocol)
+    assert np.array_equal(slots, oslots)
+    key = lambda p, m: sorted(zip(p[:, 0].tolist(), p[:, 1].tolist(), m.tolist()))
+    assert key(pairs, masks) == key(opairs, omasks)
+
+
+@pytest.mark.parametrize("make", [systems.water_box, systems.slab])
+def test_pair_list_covers_every_pair_within_cutoff(ctx, make):
+    """Every non-excluded pair within the cutoff appears in exactly one tile lane."""
+    topo, xyz, params = make()
+    orc = oracle.Oracle(topo, params)
+    u = orc.to_fixed(xyz)
+    s = _system(topo, params, fenzi.MIXED, ctx, xyz, orc)
+    _, e = s.forces()
+    assert e["pairs_within_cutoff"] == orc.count_pairs_within(u, params["cutoff"])
+
+
+@pytest.mark.parametrize("make", [systems.water_box, systems.slab])
+@pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
+def test_forces_energies_virial_vs_oracle(ctx, make, prec):
+    topo, xyz, params = make()
+    orc = oracle.Oracle(topo, params)
+    u = orc.to_fixed(xyz)
+    f_ref, e_ref, vir_ref = orc.forces(u)
+    s = _system(topo, params, prec, ctx, xyz, orc)
+    f, e = s.forces()
+    # tolerances: FP64 form forces 1e-10 relative RMS, energy terms 1e-9 relative
+    # (absolute floor 1e-6 kcal/mol: energies are summed in 2^-28 fixed point);
+    # mixed form 1e-5 for both
+    ftol, etol = (1e-10, 1e-9) if prec == fenzi.FP64 else (1e-5, 1e-5)
+    assert systems.rel_rms(f, f_ref) < ftol
+    names = dict(bond="bond", angle="angle", ub="urey_bradley", impr="improper", dihed="dihedral",
+                 vdw="vdw", elec="elec", pme="pme", ewexcl="ewald_excl", ewself="ewald_self")
+    scale = sum(abs(v) for v in e_ref.values())
+    for k, v in e_ref.items():
+        assert abs(e[names[k]] - v) <= max(1e-6, etol * max(abs(v), 1e-3 * scale)), (k, e[names[k]], v)
+    assert np.abs(e["virial"] - vir_ref).max() <= etol * 10 * np.abs(vir_ref).max()
+
+
+@pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
+def test_md_steps_track_oracle(ctx, prec):
+    topo, xyz, params = systems.slab()
+    orc = oracle.Oracle(topo, params)
+    u = orc.to_fixed(xyz)
+    v0 = orc.maxwell(u, 300.0, seed=3, counter=0)
+    s = _system(topo, params, prec, ctx, xyz, orc)
+    s.reassign_velocities(300.0, 3, 0)
+    _, v_dev = s.get_state()
+    # same Philox draw; differences are libm ulps and the 2^-28 fixed-point momentum sum
+    assert systems.rel_rms(v_dev, v0) < 1e-10
+    u_ref, v_ref, rc = orc.step(u, v0, 10)
+    assert rc == 0
+    s.step(10)
+    x_dev, v_dev = s.get_state()
+    x_ref = orc.from_fixed(u_ref)
+    d = x_dev - x_ref
+    d -= np.array(params["box"]) * np.round(d / np.array(params["box"]))
+    tol = 1e-9 if prec == fenzi.FP64 else 2e-5
+    assert np.abs(d).max() < tol
+    assert systems.rel_rms(v_dev, v_ref) < (1e-8 if prec == fenzi.FP64 else 1e-4)
+    res, vres = orc.constraint_residual(s.get_fixed(), v_dev)
+    assert res < 1e-9 and vres < 1e-6
+
+
+def test_mixed_is_bitwise_reproducible(ctx):
+    topo, xyz, params = systems.slab()
+    runs = []
+    for _ in range(2):
+        s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+        s.reassign_velocities(300.0, 9, 0)
+        s.step(25)
+        runs.append((s.get_fixed(), s.get_state()[1]))
+        s.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_batch_matches_alone_bitwise(ctx):
+    cfgs = [systems.slab(seed=1), systems.water_box(seed=2), systems.slab(n_chains=3, n_waters=300, L=24.0, seed=3,
+                                                                        K=(24, 24, 24))]
+    alone = []
+    for topo, xyz, params in cfgs:
+        s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+        s.reassign_velocities(300.0, 4, 0)
+        s.step(20)
+        alone.append(s.get_fixed())
+        s.close()
+    mix = []
+    for topo, xyz, params in cfgs:
+        s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+        s.reassign_velocities(300.0, 4, 0)
+        mix.append(s)
+    b = fenzi.Batch(ctx, mix)
+    b.step(20)
+    for s, ref in zip(mix, alone):
+        assert np.array_equal(s.get_fixed(), ref)
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
+def test_minimize_lowers_energy_and_keeps_constraints(ctx, prec):
+    topo, xyz, params = systems.slab()
+    orc = oracle.Oracle(topo, params)
+    s = fenzi.System(ctx, topo, params, prec, xyz=xyz)
+    _, e0 = s.forces()
+    s.minimize(100, 0.05)
+    f1, e1 = s.forces()
+    assert e1["potential"] < e0["potential"]
+    res, _ = orc.constraint_residual(s.get_fixed())
+    assert res < 1e-9
+    # energies after minimisation agree with the oracle at the new positions
+    f_ref, e_ref, _ = orc.forces(s.get_fixed())
+    assert systems.rel_rms(f1, f_ref) < (1e-10 if prec == fenzi.FP64 else 1e-5)
