@@ -156,6 +156,14 @@ int fz_step(fz_system* s, int nsteps) {
   FZ_GUARD_END
 }
 
+int fz_phase_times(fz_system* s, int nsteps, double* ms) {
+  FZ_GUARD_BEGIN
+  if (!s || nsteps <= 0 || !ms) return set_error(FZ_ERR_ARG, "invalid argument");
+  FZ_HIP(hipSetDevice(s->ctx->device));
+  return s->eng->phase_times(nsteps, ms);
+  FZ_GUARD_END
+}
+
 int fz_get_energies(fz_system* s, fz_energies* e) {
   if (!s || !e) return set_error(FZ_ERR_ARG, "null argument");
   return s->sys.energies(e);

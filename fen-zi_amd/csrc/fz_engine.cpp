@@ -791,6 +791,52 @@ int Engine::minimize(int nsteps, double max_step) {
   return read_status();
 }
 
+int Engine::phase_times(int nsteps, double* out) {
+  FZ_TRY(ensure());
+  for (auto* s : sys_)
+    if (s->need_rebuild) FZ_TRY(forces(true));
+  const int NP = 9;
+  hipEvent_t ev[NP + 1];
+  for (int k = 0; k <= NP; k++) FZ_HIP(hipEventCreate(&ev[k]));
+  for (int k = 0; k < NP; k++) out[k] = 0.0;
+  int rc = FZ_OK;
+  for (int st = 0; st < nsteps && rc == FZ_OK; st++) {
+    hipEventRecord(ev[0], L_.stream);
+    launch_kick_drift(L_);
+    launch_shake(L_);
+    hipEventRecord(ev[1], L_.stream);
+    launch_nblist(L_, true);
+    hipEventRecord(ev[2], L_.stream);
+    launch_gather_sorted(L_);
+    hipEventRecord(ev[3], L_.stream);
+    launch_pair(L_, false);
+    hipEventRecord(ev[4], L_.stream);
+    launch_bonded(L_, false);
+    hipEventRecord(ev[5], L_.stream);
+    launch_pme_part(L_, 0, false);
+    hipEventRecord(ev[6], L_.stream);
+    launch_pme_part(L_, 1, false);
+    hipEventRecord(ev[7], L_.stream);
+    launch_pme_part(L_, 2, false);
+    hipEventRecord(ev[8], L_.stream);
+    launch_kick(L_);
+    launch_rattle(L_);
+    hipEventRecord(ev[9], L_.stream);
+    if (hipEventSynchronize(ev[9]) != hipSuccess) rc = set_error(FZ_ERR_HIP, "event synchronize failed");
+    for (int k = 0; k < NP && rc == FZ_OK; k++) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      out[k] += ms;
+    }
+  }
+  for (int k = 0; k <= NP; k++) (void)hipEventDestroy(ev[k]);
+  FZ_TRY(rc);
+  for (int k = 0; k < NP; k++) out[k] /= std::max(1, nsteps);
+  FZ_TRY(read_status());
+  for (auto* s : sys_) s->step += nsteps;
+  return FZ_OK;
+}
+
 int Engine::maxwell(double T, uint64_t seed, uint64_t counter) {
   FZ_TRY(ensure());
   launch_maxwell(L_, T, seed, counter);

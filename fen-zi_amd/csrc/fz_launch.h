@@ -38,6 +38,7 @@ void launch_pair(const Launch& L, bool ev);
 void launch_bonded(const Launch& L, bool ev);
 void launch_pme(const Launch& L, bool ev, int conv_mode);
 void launch_pme_fft_only(const Launch& L, int conv_mode);
+void launch_pme_part(const Launch& L, int part, bool ev);  // 0 spread, 1 FFT + convolution, 2 gather
 void launch_kick_drift(const Launch& L);
 void launch_shake(const Launch& L);
 void launch_kick(const Launch& L);

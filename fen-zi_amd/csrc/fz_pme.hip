@@ -437,6 +437,20 @@ void launch_pme(const Launch& L, bool ev, int conv_mode) {
   }
 }
 
+void launch_pme_part(const Launch& L, int part, bool ev) {
+  if (L.w_fftx.total == 0) return;
+  if (part == 0) {
+    if (L.mixed) hipLaunchKernelGGL(k_spread<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    else hipLaunchKernelGGL(k_spread<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+  } else if (part == 1) {
+    if (L.mixed) launch_fft_passes<true>(L, ev, 0);
+    else launch_fft_passes<false>(L, ev, 0);
+  } else {
+    if (L.mixed) hipLaunchKernelGGL(k_pme_gather<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    else hipLaunchKernelGGL(k_pme_gather<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+  }
+}
+
 void launch_pme_fft_only(const Launch& L, int conv_mode) {
   if (L.mixed) launch_fft_passes<true>(L, true, conv_mode);
   else launch_fft_passes<false>(L, true, conv_mode);

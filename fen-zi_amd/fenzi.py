@@ -69,7 +69,7 @@ class RunSettings(C.Structure):
 EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create", "fz_system_destroy",
            "fz_set_state", "fz_get_state", "fz_set_fixed_positions", "fz_get_fixed_positions",
            "fz_reassign_velocities", "fz_compute_forces", "fz_step", "fz_get_energies", "fz_minimize",
-           "fz_system_info", "fz_get_cluster_layout", "fz_get_cluster_pairs", "fz_pme_convolve",
+           "fz_system_info", "fz_phase_times", "fz_get_cluster_layout", "fz_get_cluster_pairs", "fz_pme_convolve",
            "fz_batch_create", "fz_batch_step", "fz_batch_minimize", "fz_batch_destroy", "fz_deck_load", "fz_deck_destroy",
            "fz_deck_topology", "fz_deck_params", "fz_deck_coordinates", "fz_deck_settings",
            "fz_deck_psf_counts", "fz_format_energy_header", "fz_format_energy_row"]
@@ -103,6 +103,7 @@ def lib():
         L.fz_get_energies.argtypes = [vp, C.POINTER(Energies)]
         L.fz_minimize.argtypes = [vp, C.c_int, C.c_double]
         L.fz_system_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+        L.fz_phase_times.argtypes = [vp, C.c_int, _dp]
         L.fz_get_cluster_layout.argtypes = [vp, _ip, _ip, _ip, C.c_int64, C.POINTER(C.c_int64)]
         L.fz_get_cluster_pairs.argtypes = [vp, _ip, _up, C.c_int64]
         L.fz_get_cluster_pairs.restype = C.c_int64
@@ -265,6 +266,14 @@ class System:
         e = Energies()
         _check(lib().fz_get_energies(self.h, C.byref(e)), "fz_get_energies")
         return e.as_dict()
+
+    PHASES = ["kick_drift_shake", "nblist", "gather_sorted", "pair", "bonded", "pme_spread", "pme_fft_conv",
+              "pme_gather", "kick_rattle"]
+
+    def phase_times(self, nsteps):
+        ms = np.zeros(len(self.PHASES))
+        _check(lib().fz_phase_times(self.h, int(nsteps), _p(ms, C.c_double)), "fz_phase_times")
+        return dict(zip(self.PHASES, ms.tolist()))
 
     def info(self):
         n, b, p = C.c_int64(), C.c_int64(), C.c_int32()

@@ -138,6 +138,15 @@ int fz_get_energies(fz_system *sys, fz_energies *e);
  * Replaces MinimizeStep (src/dynamics.cu:491). */
 int fz_minimize(fz_system *sys, int nsteps, double max_step);
 int fz_system_info(fz_system *sys, int64_t *n_atoms, int64_t *device_bytes, int32_t *precision);
+/* Per-phase device time: runs nsteps steps without graph capture, with HIP
+ * events between phases on the system's stream; mean ms per step per phase in
+ * ms[FZ_NPHASE], phases: 0 first kick + drift + SHAKE, 1 neighbour check and
+ * rebuild, 2 sorted-coordinate gather, 3 pair kernel, 4 bonded + exclusion
+ * terms, 5 PME spread, 6 PME FFT + convolution, 7 PME force gather,
+ * 8 second kick + RATTLE.  Replaces the reference's PROFILING clock() buckets
+ * (src/globals.h:796-823). */
+#define FZ_NPHASE 9
+int fz_phase_times(fz_system *sys, int nsteps, double *ms);
 
 /* ---- inspection (parity tests) ------------------------------------------------ */
 /* Column grid and cluster layout of the last neighbour build: column of each
