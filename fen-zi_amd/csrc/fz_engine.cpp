@@ -311,7 +311,7 @@ int System::alloc_device() {
   FZ_TRY(up(d.q, h.q)); FZ_TRY(up(d.minv, h.minv)); FZ_TRY(up(d.type, h.type));
   FZ_TRY(dalloc(&d.atom_slot, n)); FZ_TRY(dalloc(&d.atom_col, n));
   FZ_TRY(dalloc(&d.col_count, d.ncol)); FZ_TRY(dalloc(&d.col_fill, d.ncol)); FZ_TRY(dalloc(&d.col_clu0, d.ncol + 1));
-  FZ_TRY(dalloc(&d.n_clusters, 1)); FZ_TRY(dalloc(&d.n_pairs, 1));
+  FZ_TRY(dalloc(&d.n_clusters, 2)); FZ_TRY(dalloc(&d.n_pairs, 1));
   FZ_TRY(dalloc(&d.excl_bits, n)); FZ_TRY(up(d.excl_bits, h.excl_bits));
   FZ_TRY(dalloc(&d.excl_off, n + 1)); FZ_TRY(up(d.excl_off, h.excl_off));
   FZ_TRY(dalloc(&d.excl_list, h.excl_list.size())); FZ_TRY(up(d.excl_list, h.excl_list));
@@ -392,6 +392,7 @@ int System::alloc_device() {
 
 int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   SysDev& d = dv;
+  nb_alloc0 = allocs.size();
   d.n_clusters_cap = clusters_cap;
   d.n_slots_cap = clusters_cap * CLUSTER;
   d.n_pairs_cap = pairs_cap;
@@ -405,6 +406,7 @@ int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   FZ_TRY(dalloc(&d.bb, (size_t)6 * clusters_cap));
   FZ_TRY(dalloc(&d.search_cnt, clusters_cap + 1));
   FZ_TRY(dalloc(&d.search_tmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));
+  FZ_TRY(dalloc(&d.search_mtmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));
   FZ_TRY(dalloc(&d.pl_off, clusters_cap + 1));
   FZ_TRY(dalloc(&d.pl_i, pairs_cap, false));
   FZ_TRY(dalloc(&d.pl_j, pairs_cap, false));
@@ -656,7 +658,7 @@ int Engine::grow_and_reset(int si, int bits) {
     return set_error(FZ_ERR_CAPACITY, "neighbour search capacity exceeded (column occupancy or J list > %d); "
                      "use a finer column grid", SEARCH_TMP_CAP);
   // the neighbour arrays are the last allocations of the system: free them
-  const size_t keep = s->allocs.size() - 13;
+  const size_t keep = s->nb_alloc0;
   for (size_t k = keep; k < s->allocs.size(); k++) (void)hipFree(s->allocs[k]);
   s->allocs.resize(keep);
   FZ_TRY(s->alloc_neighbor(pairs, clus));

@@ -54,6 +54,7 @@ struct System {
   HostTopo topo;
   SysDev dv{};
   std::vector<void*> allocs;
+  size_t nb_alloc0 = 0;  // index in allocs of the first neighbour-list buffer
   size_t bytes = 0;
   int64_t step = 0;
   int gen = 0;  // bumped whenever device buffers are reallocated

@@ -91,30 +91,51 @@ __device__ __forceinline__ CCl load_ccl(const SysDev& d, int c) {
   return k;
 }
 
-// in-place solve of the n x n (n <= 3) system A x = b, partial pivoting
-__device__ __forceinline__ bool solve3(double A[3][3], double* b, int n) {
-  for (int k = 0; k < 3; k++) {
-    if (k >= n) break;
-    int piv = k;
-    for (int r = k + 1; r < n; r++)
-      if (fabs(A[r][k]) > fabs(A[piv][k])) piv = r;
-    if (A[piv][k] == 0.0) return false;
-    if (piv != k) {
-      for (int c = 0; c < 3; c++) { double t = A[k][c]; A[k][c] = A[piv][c]; A[piv][c] = t; }
-      double t = b[k]; b[k] = b[piv]; b[piv] = t;
-    }
-    for (int r = k + 1; r < n; r++) {
-      const double m = A[r][k] / A[k][k];
-      for (int c = k; c < n; c++) A[r][c] -= m * A[k][c];
-      b[r] -= m * b[k];
-    }
-  }
-  for (int k = n - 1; k >= 0; k--) {
-    double x = b[k];
-    for (int c = k + 1; c < n; c++) x -= A[k][c] * b[c];
-    b[k] = x / A[k][k];
-  }
+// b <- A^{-1} b for a 3 x 3 system (cofactor expansion; the constraint
+// matrices are small and well conditioned); false if singular
+__device__ __forceinline__ bool solve3(const double (&A)[3][3], double (&b)[3], int n) {
+  (void)n;
+  const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+  const double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+  const double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+  const double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+  if (det == 0.0 || !isfinite(det)) return false;
+  const double id = 1.0 / det;
+  const double x0 = (b[0] * c00 + b[1] * (A[0][2] * A[2][1] - A[0][1] * A[2][2]) +
+                     b[2] * (A[0][1] * A[1][2] - A[0][2] * A[1][1])) * id;
+  const double x1 = (b[0] * c01 + b[1] * (A[0][0] * A[2][2] - A[0][2] * A[2][0]) +
+                     b[2] * (A[0][2] * A[1][0] - A[0][0] * A[1][2])) * id;
+  const double x2 = (b[0] * c02 + b[1] * (A[0][1] * A[2][0] - A[0][0] * A[2][1]) +
+                     b[2] * (A[0][0] * A[1][1] - A[0][1] * A[1][0])) * id;
+  b[0] = x0; b[1] = x1; b[2] = x2;
   return true;
+}
+
+// Small vectors indexed by run-time cluster-local indices are read and written
+// through select chains over fully unrolled loops, so the cluster state stays
+// in registers (no scratch).
+struct V3 { double x, y, z; };
+__device__ __forceinline__ V3 v3sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ double v3dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 pick4(const V3 (&p)[4], int i) {
+  V3 r = p[0];
+  if (i == 1) r = p[1];
+  if (i == 2) r = p[2];
+  if (i == 3) r = p[3];
+  return r;
+}
+__device__ __forceinline__ double pick4(const double (&p)[4], int i) {
+  return i == 0 ? p[0] : i == 1 ? p[1] : i == 2 ? p[2] : p[3];
+}
+
+__device__ __forceinline__ void load_rel(const SysDev& d, const CCl& k, const unsigned long long* ux,
+                                         const unsigned long long* uy, const unsigned long long* uz, V3 (&r)[4]) {
+  const int a0 = k.at[0];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int a = k.at[m] >= 0 ? k.at[m] : a0;
+    r[m] = V3{fx_d(ux[a0], ux[a], d.sc[0]), fx_d(uy[a0], uy[a], d.sc[1]), fx_d(uz[a0], uz[a], d.sc[2])};
+  }
 }
 
 __global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
@@ -123,63 +144,74 @@ __global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int
   const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
   if (c >= d.n_ccl) return;
   const CCl k = load_ccl(d, c);
-  const int a0 = k.at[0];
-  double rn[4][3], ro[4][3], mi[4], dr[4][3];
-  for (int m = 0; m < k.na; m++) {
-    const int a = k.at[m];
-    rn[m][0] = fx_d(d.ux[a0], d.ux[a], d.sc[0]);
-    rn[m][1] = fx_d(d.uy[a0], d.uy[a], d.sc[1]);
-    rn[m][2] = fx_d(d.uz[a0], d.uz[a], d.sc[2]);
-    ro[m][0] = fx_d(d.ox[a0], d.ox[a], d.sc[0]);
-    ro[m][1] = fx_d(d.oy[a0], d.oy[a], d.sc[1]);
-    ro[m][2] = fx_d(d.oz[a0], d.oz[a], d.sc[2]);
-    mi[m] = d.minv[a];
-    dr[m][0] = dr[m][1] = dr[m][2] = 0.0;
+  V3 rn[4], ro[4], dr[4];
+  double mi[4];
+  load_rel(d, k, d.ux, d.uy, d.uz, rn);
+  load_rel(d, k, d.ox, d.oy, d.oz, ro);
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    mi[m] = k.at[m] >= 0 ? d.minv[k.at[m]] : 0.0;
+    dr[m] = V3{0.0, 0.0, 0.0};
   }
-  double g[3][3], lam[3] = {0.0, 0.0, 0.0};
-  for (int j = 0; j < k.nc; j++)
-    for (int x = 0; x < 3; x++) g[j][x] = ro[k.c1[j]][x] - ro[k.c2[j]][x];
+  V3 g[3];
+  double lam[3] = {0.0, 0.0, 0.0};
+  // coupling weights w[j][e]: d(r_c1j - r_c2j)/d(lambda_e) = w[j][e] * g_e
+  double w[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    g[j] = v3sub(pick4(ro, k.c1[j]), pick4(ro, k.c2[j]));
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+      const int p = k.c1[j], q = k.c2[j], ep = k.c1[e], eq = k.c2[e];
+      w[j][e] = ((p == ep) - (p == eq)) * pick4(mi, p) - ((q == ep) - (q == eq)) * pick4(mi, q);
+    }
+  }
   bool ok = false;
   for (int it = 0; it < 200; it++) {
-    double rc[3][3], sig[3], worst = 0.0;
-    for (int j = 0; j < k.nc; j++) {
-      const int p = k.c1[j], q = k.c2[j];
-      for (int x = 0; x < 3; x++) rc[j][x] = (rn[p][x] + dr[p][x]) - (rn[q][x] + dr[q][x]);
-      sig[j] = rc[j][0] * rc[j][0] + rc[j][1] * rc[j][1] + rc[j][2] * rc[j][2] - k.d2[j];
-      worst = fmax(worst, fabs(sig[j]) / (2.0 * k.d2[j]));
+    V3 rc[3];
+    double sig[3], worst = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      rc[j] = v3sub(v3sub(pick4(rn, k.c1[j]), pick4(rn, k.c2[j])), v3sub(pick4(dr, k.c2[j]), pick4(dr, k.c1[j])));
+      sig[j] = j < k.nc ? v3dot(rc[j], rc[j]) - k.d2[j] : 0.0;
+      if (j < k.nc) worst = fmax(worst, fabs(sig[j]) / (2.0 * k.d2[j]));
     }
     if (worst < d.shake_tol) { ok = true; break; }
     double J[3][3];
+#pragma unroll
     for (int j = 0; j < 3; j++)
-      for (int e = 0; e < 3; e++) J[j][e] = 0.0;
-    for (int j = 0; j < k.nc; j++)
-      for (int e = 0; e < k.nc; e++) {
-        const int p = k.c1[j], q = k.c2[j], ep = k.c1[e], eq = k.c2[e];
-        const double wp = ((p == ep) - (p == eq)) * mi[p], wq = ((q == ep) - (q == eq)) * mi[q];
-        J[j][e] = 2.0 * (rc[j][0] * g[e][0] + rc[j][1] * g[e][1] + rc[j][2] * g[e][2]) * (wp - wq);
+#pragma unroll
+      for (int e = 0; e < 3; e++)
+        J[j][e] = (j < k.nc && e < k.nc) ? 2.0 * v3dot(rc[j], g[e]) * w[j][e] : (j == e ? 1.0 : 0.0);
+#pragma unroll
+    for (int j = 0; j < 3; j++) sig[j] = -sig[j];
+    if (!solve3(J, sig, 3)) break;
+#pragma unroll
+    for (int j = 0; j < 3; j++) lam[j] += j < k.nc ? sig[j] : 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      V3 s = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const double f = (k.c1[j] == m ? 1.0 : 0.0) - (k.c2[j] == m ? 1.0 : 0.0);
+        const double l = j < k.nc ? lam[j] * f * mi[m] : 0.0;
+        s.x += l * g[j].x; s.y += l * g[j].y; s.z += l * g[j].z;
       }
-    for (int j = 0; j < k.nc; j++) sig[j] = -sig[j];
-    if (!solve3(J, sig, k.nc)) break;
-    for (int j = 0; j < k.nc; j++) lam[j] += sig[j];
-    for (int m = 0; m < k.na; m++) dr[m][0] = dr[m][1] = dr[m][2] = 0.0;
-    for (int j = 0; j < k.nc; j++) {
-      const int p = k.c1[j], q = k.c2[j];
-      for (int x = 0; x < 3; x++) {
-        dr[p][x] += lam[j] * g[j][x] * mi[p];
-        dr[q][x] -= lam[j] * g[j][x] * mi[q];
-      }
+      dr[m] = s;
     }
   }
   if (!ok) d.flags[F_SHAKE_FAIL] = 1;
   const double idt = 1.0 / d.dt;
-  for (int m = 0; m < k.na; m++) {
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
     const int a = k.at[m];
-    d.ux[a] += (unsigned long long)__double2ll_rn(dr[m][0] * d.isc[0]);
-    d.uy[a] += (unsigned long long)__double2ll_rn(dr[m][1] * d.isc[1]);
-    d.uz[a] += (unsigned long long)__double2ll_rn(dr[m][2] * d.isc[2]);
-    d.vx[a] += dr[m][0] * idt;
-    d.vy[a] += dr[m][1] * idt;
-    d.vz[a] += dr[m][2] * idt;
+    if (a < 0) continue;
+    d.ux[a] += (unsigned long long)__double2ll_rn(dr[m].x * d.isc[0]);
+    d.uy[a] += (unsigned long long)__double2ll_rn(dr[m].y * d.isc[1]);
+    d.uz[a] += (unsigned long long)__double2ll_rn(dr[m].z * d.isc[2]);
+    d.vx[a] += dr[m].x * idt;
+    d.vy[a] += dr[m].y * idt;
+    d.vz[a] += dr[m].z * idt;
   }
 }
 
@@ -189,41 +221,43 @@ __global__ __launch_bounds__(128) void k_rattle(const SysDev* __restrict__ S, in
   const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
   if (c >= d.n_ccl) return;
   const CCl k = load_ccl(d, c);
-  const int a0 = k.at[0];
-  double r[4][3], v[4][3], mi[4];
-  for (int m = 0; m < k.na; m++) {
-    const int a = k.at[m];
-    r[m][0] = fx_d(d.ux[a0], d.ux[a], d.sc[0]);
-    r[m][1] = fx_d(d.uy[a0], d.uy[a], d.sc[1]);
-    r[m][2] = fx_d(d.uz[a0], d.uz[a], d.sc[2]);
-    v[m][0] = d.vx[a]; v[m][1] = d.vy[a]; v[m][2] = d.vz[a];
-    mi[m] = d.minv[a];
+  V3 r[4], v[4];
+  double mi[4];
+  load_rel(d, k, d.ux, d.uy, d.uz, r);
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int a = k.at[m] >= 0 ? k.at[m] : k.at[0];
+    v[m] = V3{d.vx[a], d.vy[a], d.vz[a]};
+    mi[m] = k.at[m] >= 0 ? d.minv[a] : 0.0;
   }
-  double rc[3][3], A[3][3], b[3];
-  for (int j = 0; j < 3; j++)
-    for (int e = 0; e < 3; e++) A[j][e] = 0.0;
-  for (int j = 0; j < k.nc; j++)
-    for (int x = 0; x < 3; x++) rc[j][x] = r[k.c1[j]][x] - r[k.c2[j]][x];
-  for (int j = 0; j < k.nc; j++) {
+  V3 rc[3];
+  double A[3][3], b[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) rc[j] = v3sub(pick4(r, k.c1[j]), pick4(r, k.c2[j]));
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
     const int p = k.c1[j], q = k.c2[j];
-    b[j] = -((v[p][0] - v[q][0]) * rc[j][0] + (v[p][1] - v[q][1]) * rc[j][1] + (v[p][2] - v[q][2]) * rc[j][2]);
-    for (int e = 0; e < k.nc; e++) {
+    b[j] = j < k.nc ? -v3dot(v3sub(pick4(v, p), pick4(v, q)), rc[j]) : 0.0;
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
       const int ep = k.c1[e], eq = k.c2[e];
-      const double wp = ((p == ep) - (p == eq)) * mi[p], wq = ((q == ep) - (q == eq)) * mi[q];
-      A[j][e] = (rc[j][0] * rc[e][0] + rc[j][1] * rc[e][1] + rc[j][2] * rc[e][2]) * (wp - wq);
+      const double w = ((p == ep) - (p == eq)) * pick4(mi, p) - ((q == ep) - (q == eq)) * pick4(mi, q);
+      A[j][e] = (j < k.nc && e < k.nc) ? v3dot(rc[j], rc[e]) * w : (j == e ? 1.0 : 0.0);
     }
   }
-  if (!solve3(A, b, k.nc)) return;
-  for (int j = 0; j < k.nc; j++) {
-    const int p = k.c1[j], q = k.c2[j];
-    for (int x = 0; x < 3; x++) {
-      v[p][x] += b[j] * rc[j][x] * mi[p];
-      v[q][x] -= b[j] * rc[j][x] * mi[q];
-    }
-  }
-  for (int m = 0; m < k.na; m++) {
+  if (!solve3(A, b, 3)) return;
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
     const int a = k.at[m];
-    d.vx[a] = v[m][0]; d.vy[a] = v[m][1]; d.vz[a] = v[m][2];
+    if (a < 0) continue;
+    V3 s = v[m];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double f = (k.c1[j] == m ? 1.0 : 0.0) - (k.c2[j] == m ? 1.0 : 0.0);
+      const double l = j < k.nc ? b[j] * f * mi[m] : 0.0;
+      s.x += l * rc[j].x; s.y += l * rc[j].y; s.z += l * rc[j].z;
+    }
+    d.vx[a] = s.x; d.vy[a] = s.y; d.vz[a] = s.z;
   }
 }
 

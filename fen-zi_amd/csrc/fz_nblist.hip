@@ -98,6 +98,16 @@ __global__ __launch_bounds__(1024) void k_col_scan(const SysDev* __restrict__ S,
     int total = part[1023];
     d.col_clu0[n] = total;
     *d.n_clusters = total;
+    // slot-order multiplier for the PME spread: odd, near 0.618 ns, coprime to ns
+    const unsigned ns = (unsigned)total * CLUSTER;
+    unsigned m = ((unsigned)(0.6180339887 * ns)) | 1u;
+    for (;;) {
+      unsigned a = m, b = ns > 0 ? ns : 1u;
+      while (b) { unsigned r = a % b; a = b; b = r; }
+      if (a == 1u || ns == 0u) break;
+      m += 2u;
+    }
+    d.n_clusters[1] = (int)m;
     int ov = 0;
     if (bad) ov |= OV_COLUMN;
     if (total > d.n_clusters_cap || total * CLUSTER > d.n_slots_cap) ov |= OV_SLOTS;
@@ -221,13 +231,51 @@ __device__ __forceinline__ double col_gap(double a0, double a1, int c, double w,
   return fmax(0.0, fmin(d1, d2) - 1e-9);
 }
 
-// one wave per I cluster: J >= I with bounding-box gap < pair-list distance
+__device__ __forceinline__ bool is_excluded(const SysDev& d, int a, int b) {
+  if (!d.use_excl_list) {
+    int off = b - a;
+    if (off < -32 || off > 31) return false;
+    return (d.excl_bits[a] >> (off + 32)) & 1ull;
+  }
+  int lo = d.excl_off[a], hi = d.excl_off[a + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    int v = d.excl_list[mid];
+    if (v < b) lo = mid + 1; else hi = mid;
+  }
+  return lo < d.excl_off[a + 1] && d.excl_list[lo] == b;
+}
+
+// squared distance of two atoms from their fixed-point positions, rounded
+// exactly as the oracle does (no contraction)
+__device__ __forceinline__ double r2_exact(const SysDev& d, unsigned long long ax, unsigned long long ay,
+                                           unsigned long long az, int b) {
+#pragma clang fp contract(off)
+  const double dx = (double)(long long)(d.ux[b] - ax) * d.sc[0];
+  const double dy = (double)(long long)(d.uy[b] - ay) * d.sc[1];
+  const double dz = (double)(long long)(d.uz[b] - az) * d.sc[2];
+  double r2 = dx * dx + dy * dy;
+  r2 = r2 + dz * dz;
+  return r2;
+}
+
+// One wave per I cluster.  Candidates J >= I whose bounding-box gap is below
+// the pair-list distance are tested atom by atom: lane l (i = l & 7, j = l >> 3)
+// keeps its pair when both slots hold atoms, the pair is not excluded, i < j
+// inside a self pair, and r < pair-list distance at build time.  Pairs farther
+// than that cannot reach the cutoff before the next build, so an entry whose
+// 64-bit lane mask is empty is dropped.
 __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
   const int lane = threadIdx.x & 63;
+  const int il = lane & 7, jl = lane >> 3;
   const int I = lb * 4 + (threadIdx.x >> 6);
   if (I >= *d.n_clusters) return;
+  const int ai = d.slot_atom[I * CLUSTER + il];
+  unsigned long long aix = 0, aiy = 0, aiz = 0;
+  if (ai >= 0) { aix = d.ux[ai]; aiy = d.uy[ai]; aiz = d.uz[ai]; }
+  unsigned long long* mout = d.search_mtmp + (size_t)I * SEARCH_TMP_CAP;
   unsigned long long bi[6];
 #pragma unroll
   for (int e = 0; e < 6; e++) bi[e] = d.bb[6 * (size_t)I + e];
@@ -263,20 +311,31 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
           hit = bbox_within(bi, bj, d);
         }
         unsigned long long bal = __ballot(hit);
-        int pos = count + __popcll(bal & ((1ull << lane) - 1ull));
-        if (hit) {
-          if (pos < SEARCH_TMP_CAP) out[pos] = J;
-          else over = true;
+        while (bal) {
+          const int k = __ffsll((long long)bal) - 1;
+          bal &= bal - 1ull;
+          const int Jk = jb + k;
+          const int b = d.slot_atom[Jk * CLUSTER + jl];
+          bool on = ai >= 0 && b >= 0 && (I != Jk || il < jl);
+          if (on) on = !is_excluded(d, ai, b);
+          if (on) on = r2_exact(d, aix, aiy, aiz, b) < d.rl2;
+          const unsigned long long m = __ballot(on);
+          if (m) {
+            if (count < SEARCH_TMP_CAP) {
+              if (lane == 0) { out[count] = Jk; mout[count] = m; }
+            } else {
+              over = true;
+            }
+            count++;
+          }
         }
-        count += __popcll(bal);
       }
     }
   }
   if (lane == 0) {
     d.search_cnt[I] = count < SEARCH_TMP_CAP ? count : SEARCH_TMP_CAP;
-    if (count > SEARCH_TMP_CAP) atomicOr(d.flags + F_OVERFLOW, OV_SEARCH);
+    if (over) atomicOr(d.flags + F_OVERFLOW, OV_SEARCH);
   }
-  (void)over;
 }
 
 // one block per system: exclusive scan of per-cluster counts into list offsets
@@ -315,44 +374,21 @@ __global__ __launch_bounds__(1024) void k_pair_scan(const SysDev* __restrict__ S
   }
 }
 
-__device__ __forceinline__ bool is_excluded(const SysDev& d, int a, int b) {
-  if (!d.use_excl_list) {
-    int off = b - a;
-    if (off < -32 || off > 31) return false;
-    return (d.excl_bits[a] >> (off + 32)) & 1ull;
-  }
-  int lo = d.excl_off[a], hi = d.excl_off[a + 1];
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    int v = d.excl_list[mid];
-    if (v < b) lo = mid + 1; else hi = mid;
-  }
-  return lo < d.excl_off[a + 1] && d.excl_list[lo] == b;
-}
-
-// one wave per I cluster: copy the J list into place and build the lane masks
+// one wave per I cluster: copy its entries into place
 __global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
   const int lane = threadIdx.x & 63;
   const int I = lb * 4 + (threadIdx.x >> 6);
   if (I >= *d.n_clusters) return;
-  const int il = lane & 7, jl = lane >> 3;
-  const int a = d.slot_atom[I * CLUSTER + il];
   const long long off = d.pl_off[I];
   const int cnt = d.search_cnt[I];
   const int* src = d.search_tmp + (size_t)I * SEARCH_TMP_CAP;
-  for (int k = 0; k < cnt; k++) {
-    const int J = src[k];
-    const int b = d.slot_atom[J * CLUSTER + jl];
-    bool on = a >= 0 && b >= 0 && (I != J || il < jl);
-    if (on) on = !is_excluded(d, a, b);
-    unsigned long long m = __ballot(on);
-    if (lane == 0) {
-      d.pl_i[off + k] = I;
-      d.pl_j[off + k] = J;
-      d.pl_mask[off + k] = m;
-    }
+  const unsigned long long* msrc = d.search_mtmp + (size_t)I * SEARCH_TMP_CAP;
+  for (int k = lane; k < cnt; k += 64) {
+    d.pl_i[off + k] = I;
+    d.pl_j[off + k] = src[k];
+    d.pl_mask[off + k] = msrc[k];
   }
 }
 

@@ -53,45 +53,50 @@ __device__ __forceinline__ int spline_pos(unsigned long long u, int K, R& f) {
   return g < 0 ? g + K : g;
 }
 
+// weight k (0..3) of the order-4 cardinal B-spline at fraction f
+template <class R>
+__device__ __forceinline__ R bspline4_k(R f, int k) {
+  const R f2 = f * f, f3 = f2 * f, g = R(1) - f;
+  const R w0 = g * g * g, w1 = R(3) * f3 - R(6) * f2 + R(4), w2 = R(-3) * f3 + R(3) * f2 + R(3) * f + R(1);
+  const R w = k == 0 ? w0 : k == 1 ? w1 : k == 2 ? w2 : f3;
+  return w * R(1.0 / 6.0);
+}
+
+// One atom per wave iteration, one of its 4x4x4 grid points per lane: the 64
+// atomics of an instruction fall on 16 rows of 4 consecutive x points (a few
+// cache lines) instead of 64 unrelated lines.  Slots are visited with a
+// multiplier coprime to their count (near 0.618 ns, set at each list build; a
+// bijection of [0, ns)), so that concurrent waves work on distant atoms.
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_spread(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using R = typename std::conditional<MIXED, float, double>::type;
   const int s_ = locate(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
-  if (sl >= *d.n_clusters * CLUSTER) return;
-  const int a = d.slot_atom[sl];
-  if (a < 0) return;
-  const double qd = d.q[a];
-  if (qd == 0.0) return;
+  const int lane = threadIdx.x & 63;
+  const int t0 = ((blockIdx.x - pre[s_]) * 4 + (threadIdx.x >> 6)) * 64;
+  const int ns = *d.n_clusters * CLUSTER;
+  const unsigned long long mult = (unsigned long long)d.n_clusters[1];
   const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
-  R fx, fy, fz, tx[4], ty[4], tz[4];
-  const int gx = spline_pos<R>(d.ux[a], K0, fx);
-  const int gy = spline_pos<R>(d.uy[a], K1, fy);
-  const int gz = spline_pos<R>(d.uz[a], K2, fz);
-  bspline4<R>(fx, tx, nullptr);
-  bspline4<R>(fy, ty, nullptr);
-  bspline4<R>(fz, tz, nullptr);
-  const R q = (R)qd;
-  int ix[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) ix[k] = gx + k >= K0 ? gx + k - K0 : gx + k;
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    const int z = gz + c >= K2 ? gz + c - K2 : gz + c;
-    const R qz = q * tz[c];
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-      const int y = gy + b >= K1 ? gy + b - K1 : gy + b;
-      const R qzy = qz * ty[b];
-      const size_t row = ((size_t)z * K1 + y) * K0;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const R v = qzy * tx[k];
-        if (MIXED) atomicAdd((unsigned long long*)d.Qacc + row + ix[k], (unsigned long long)__float2ll_rn((float)v * 4294967296.0f));
-        else unsafeAtomicAdd((double*)d.Qacc + row + ix[k], (double)v);
-      }
-    }
+  const int ca = lane & 3, cb = (lane >> 2) & 3, cc = lane >> 4;
+  for (int k = 0; k < 64; k++) {
+    const int t = t0 + k;
+    if (t >= ns) break;
+    const int sl = (int)(((unsigned long long)t * mult) % (unsigned long long)ns);
+    const int a = d.slot_atom[sl];
+    if (a < 0) continue;
+    const double qd = d.q[a];
+    if (qd == 0.0) continue;
+    R fx, fy, fz;
+    int gx = spline_pos<R>(d.ux[a], K0, fx) + ca;
+    int gy = spline_pos<R>(d.uy[a], K1, fy) + cb;
+    int gz = spline_pos<R>(d.uz[a], K2, fz) + cc;
+    gx -= gx >= K0 ? K0 : 0;
+    gy -= gy >= K1 ? K1 : 0;
+    gz -= gz >= K2 ? K2 : 0;
+    const R v = (R)qd * bspline4_k<R>(fx, ca) * bspline4_k<R>(fy, cb) * bspline4_k<R>(fz, cc);
+    const size_t g = ((size_t)gz * K1 + gy) * K0 + gx;
+    if (MIXED) atomicAdd((unsigned long long*)d.Qacc + g, (unsigned long long)__float2ll_rn((float)v * 4294967296.0f));
+    else unsafeAtomicAdd((double*)d.Qacc + g, (double)v);
   }
 }
 

@@ -74,9 +74,10 @@ struct SysDev {
   int *bin_tmp;          // [n_slots_cap] column-scattered atoms
   int *clu_col;          // [n_clusters_cap]
   unsigned long long *bb; // [6 * n_clusters_cap] lo/hi per axis
-  int *n_clusters;       // device scalar
+  int *n_clusters;       // [0] clusters of the last build, [1] spread slot multiplier
   int *search_cnt;       // [n_clusters_cap + 1]
-  int *search_tmp;       // [n_clusters_cap * SEARCH_TMP_CAP]
+  int *search_tmp;       // [n_clusters_cap * SEARCH_TMP_CAP] J of each kept entry
+  unsigned long long *search_mtmp; // [n_clusters_cap * SEARCH_TMP_CAP] lane mask of each kept entry
   long long *pl_off;     // [n_clusters_cap + 1]
   int *pl_i, *pl_j;      // [n_pairs_cap]
   unsigned long long *pl_mask; // [n_pairs_cap]

@@ -852,48 +852,61 @@ int64_t or_cluster_pairs(or_sys *s, const uint64_t *u, int32_t nx, int32_t ny,
   }
   double sc[3] = {s->p.box[0] * TWO_POW_M64, s->p.box[1] * TWO_POW_M64, s->p.box[2] * TWO_POW_M64};
   double rl2 = s->p.pairlist * s->p.pairlist;
-  /* count per I, then fill (brute force over J >= I) */
+  /* Entry (I, J >= I) exists iff the bounding-box gap is below the pair-list
+   * distance and at least one lane pair qualifies: both slots hold atoms, not
+   * excluded, i < j inside a self pair, r < pair-list distance.  The lane mask
+   * holds exactly the qualifying pairs.  Per I: (J, mask) in ascending J. */
   int64_t *cntI = (int64_t *)calloc((size_t)ncl + 1, sizeof(int64_t));
+  int32_t **lj = (int32_t **)calloc((size_t)ncl + 1, sizeof(int32_t *));
+  uint64_t **lm = (uint64_t **)calloc((size_t)ncl + 1, sizeof(uint64_t *));
 #pragma omp parallel for schedule(dynamic, 16)
   for (int64_t I = 0; I < ncl; I++) {
-    int64_t c = 0;
+    int64_t c = 0, capI = 64;
+    int32_t *vj = (int32_t *)malloc(sizeof(int32_t) * capI);
+    uint64_t *vm = (uint64_t *)malloc(sizeof(uint64_t) * capI);
     for (int64_t J = I; J < ncl; J++) {
       double g[3];
       for (int d = 0; d < 3; d++)
         g[d] = (double)axis_gap(bb[6 * I + 2 * d], bb[6 * I + 2 * d + 1], bb[6 * J + 2 * d], bb[6 * J + 2 * d + 1]) * sc[d];
       double d2 = g[0] * g[0] + g[1] * g[1];
       d2 = d2 + g[2] * g[2];
-      if (d2 < rl2) c++;
+      if (!(d2 < rl2)) continue;
+      uint64_t mask = 0;
+      for (int l = 0; l < 64; l++) {
+        int il = l & 7, jl = l >> 3;
+        int a = slot_atom[I * M + il], b = slot_atom[J * M + jl];
+        if (a < 0 || b < 0) continue;
+        if (I == J && il >= jl) continue;
+        if (excluded(s, a, b)) continue;
+        double dx = (double)(int64_t)(u[3 * b] - u[3 * a]) * sc[0];
+        double dy = (double)(int64_t)(u[3 * b + 1] - u[3 * a + 1]) * sc[1];
+        double dz = (double)(int64_t)(u[3 * b + 2] - u[3 * a + 2]) * sc[2];
+        double r2 = dx * dx + dy * dy;
+        r2 = r2 + dz * dz;
+        if (!(r2 < rl2)) continue;
+        mask |= 1ull << l;
+      }
+      if (!mask) continue;
+      if (c == capI) {
+        capI *= 2;
+        vj = (int32_t *)realloc(vj, sizeof(int32_t) * capI);
+        vm = (uint64_t *)realloc(vm, sizeof(uint64_t) * capI);
+      }
+      vj[c] = (int32_t)J; vm[c] = mask; c++;
     }
-    cntI[I + 1] = c;
+    cntI[I + 1] = c; lj[I] = vj; lm[I] = vm;
   }
   for (int64_t I = 0; I < ncl; I++) cntI[I + 1] += cntI[I];
   int64_t np = cntI[ncl];
   if (np <= cap) {
-#pragma omp parallel for schedule(dynamic, 16)
-    for (int64_t I = 0; I < ncl; I++) {
-      int64_t w = cntI[I];
-      for (int64_t J = I; J < ncl; J++) {
-        double g[3];
-        for (int d = 0; d < 3; d++)
-          g[d] = (double)axis_gap(bb[6 * I + 2 * d], bb[6 * I + 2 * d + 1], bb[6 * J + 2 * d], bb[6 * J + 2 * d + 1]) * sc[d];
-        double d2 = g[0] * g[0] + g[1] * g[1];
-        d2 = d2 + g[2] * g[2];
-        if (!(d2 < rl2)) continue;
-        uint64_t mask = 0;
-        for (int l = 0; l < 64; l++) {
-          int il = l & 7, jl = l >> 3;
-          int a = slot_atom[I * M + il], b = slot_atom[J * M + jl];
-          if (a < 0 || b < 0) continue;
-          if (I == J && il >= jl) continue;
-          if (excluded(s, a, b)) continue;
-          mask |= 1ull << l;
-        }
-        pairs[2 * w] = (int32_t)I; pairs[2 * w + 1] = (int32_t)J; masks[w] = mask;
-        w++;
+    for (int64_t I = 0; I < ncl; I++)
+      for (int64_t k = 0; k < cntI[I + 1] - cntI[I]; k++) {
+        int64_t w = cntI[I] + k;
+        pairs[2 * w] = (int32_t)I; pairs[2 * w + 1] = lj[I][k]; masks[w] = lm[I][k];
       }
-    }
   }
+  for (int64_t I = 0; I < ncl; I++) { free(lj[I]); free(lm[I]); }
+  free(lj); free(lm);
   for (int c = 0; c < ncol; c++) free(col[c]);
   free(col); free(fill); free(cnt); free(clu0); free(bb); free(cntI);
   return np <= cap ? np : -np;
