@@ -12,10 +12,6 @@ namespace fz {
 const char* last_error();
 }
 
-struct fz_ctx {
-  int device;
-};
-
 struct fz_system {
   fz_ctx* ctx;
   System sys;
@@ -161,6 +157,22 @@ int fz_phase_times(fz_system* s, int nsteps, double* ms) {
   if (!s || nsteps <= 0 || !ms) return set_error(FZ_ERR_ARG, "invalid argument");
   FZ_HIP(hipSetDevice(s->ctx->device));
   return s->eng->phase_times(nsteps, ms);
+  FZ_GUARD_END
+}
+
+int fz_autotune(fz_system* s, int nsteps, double* out) {
+  FZ_GUARD_BEGIN
+  if (!s || nsteps <= 0 || !out) return set_error(FZ_ERR_ARG, "invalid argument");
+  FZ_HIP(hipSetDevice(s->ctx->device));
+  return s->eng->autotune(nsteps, out);
+  FZ_GUARD_END
+}
+
+int fz_batch_autotune(fz_batch* b, int nsteps, double* out) {
+  FZ_GUARD_BEGIN
+  if (!b || nsteps <= 0 || !out) return set_error(FZ_ERR_ARG, "invalid argument");
+  FZ_HIP(hipSetDevice(b->ctx->device));
+  return b->eng->autotune(nsteps, out);
   FZ_GUARD_END
 }
 

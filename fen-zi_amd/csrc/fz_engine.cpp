@@ -282,18 +282,6 @@ int System::alloc_device() {
   d.shake_tol = par.shake_tol;
   d.pme_pre = kCC / (kPi * V);
   d.pi2_beta2 = kPi * kPi / (par.beta * par.beta);
-  // column grid: columns about cbrt(8 / rho) wide so that a cluster of 8 is
-  // roughly cubic
-  const double rho = n / V;
-  const double w = std::cbrt(CLUSTER / rho);
-  int ncx = par.column_grid[0] > 0 ? par.column_grid[0] : std::max(1, (int)std::floor(par.box[0] / w));
-  int ncy = par.column_grid[1] > 0 ? par.column_grid[1] : std::max(1, (int)std::floor(par.box[1] / w));
-  while ((double)n / (ncx * ncy) > 0.5 * MAX_COLUMN_ATOMS) { ncx++; ncy++; }
-  d.ncx = ncx; d.ncy = ncy; d.ncol = ncx * ncy;
-  d.kx_range = (int)std::ceil(par.pairlist / (par.box[0] / ncx)) + 1;
-  d.ky_range = (int)std::ceil(par.pairlist / (par.box[1] / ncy)) + 1;
-  d.colw_x = (unsigned long long)(kTwo64 / ncx);
-  d.colw_y = (unsigned long long)(kTwo64 / ncy);
   d.use_excl_list = h.use_excl_list;
   d.n_ccl = (int)h.ccl_atoms.size();
   d.n_bonds = (int)h.bond_ij.size();
@@ -310,7 +298,6 @@ int System::alloc_device() {
   FZ_TRY(dalloc(&d.q, n)); FZ_TRY(dalloc(&d.minv, n)); FZ_TRY(dalloc(&d.type, n));
   FZ_TRY(up(d.q, h.q)); FZ_TRY(up(d.minv, h.minv)); FZ_TRY(up(d.type, h.type));
   FZ_TRY(dalloc(&d.atom_slot, n)); FZ_TRY(dalloc(&d.atom_col, n));
-  FZ_TRY(dalloc(&d.col_count, d.ncol)); FZ_TRY(dalloc(&d.col_fill, d.ncol)); FZ_TRY(dalloc(&d.col_clu0, d.ncol + 1));
   FZ_TRY(dalloc(&d.n_clusters, 2)); FZ_TRY(dalloc(&d.n_pairs, 1));
   FZ_TRY(dalloc(&d.excl_bits, n)); FZ_TRY(up(d.excl_bits, h.excl_bits));
   FZ_TRY(dalloc(&d.excl_off, n + 1)); FZ_TRY(up(d.excl_off, h.excl_off));
@@ -381,18 +368,46 @@ int System::alloc_device() {
   FZ_TRY(dalloc(&d.acc, A_NSLOT));
   FZ_TRY(dalloc(&d.flags, F_NSLOT));
   FZ_TRY(dalloc(&d.mstep, 1));
-  FZ_TRY(dalloc(&snap, (size_t)6 * n));
-  // neighbour structures
+  FZ_TRY(dalloc(&snap, (size_t)9 * n));
+  return set_column_grid(par.column_grid[0], par.column_grid[1], 1.0);
+}
+
+// Column grid (binning) and the neighbour structures sized from it.  Columns
+// are about f * cbrt(8 / rho) wide, so that a cluster of 8 atoms is roughly
+// cubic at f = 1; (ncx, ncy) > 0 override.  Replaces the compile-time
+// CELL_ATOMS / MAXNB presets of the reference (src/configure:89-137).
+int System::set_column_grid(int ncx, int ncy, double f) {
+  SysDev& d = dv;
+  const int n = topo.n;
+  const double V = par.box[0] * par.box[1] * par.box[2];
+  const double rho = n / V;
+  const double w = f * std::cbrt(CLUSTER / rho);
+  if (ncx <= 0) ncx = std::max(1, (int)std::floor(par.box[0] / w));
+  if (ncy <= 0) ncy = std::max(1, (int)std::floor(par.box[1] / w));
+  while ((double)n / (ncx * ncy) > 0.5 * MAX_COLUMN_ATOMS) { ncx++; ncy++; }
+  if (nb_alloc0 > 0) {  // release the previous neighbour block
+    for (size_t k = nb_alloc0; k < allocs.size(); k++) (void)hipFree(allocs[k]);
+    allocs.resize(nb_alloc0);
+  }
+  d.ncx = ncx; d.ncy = ncy; d.ncol = ncx * ncy;
+  d.kx_range = (int)std::ceil(par.pairlist / (par.box[0] / ncx)) + 1;
+  d.ky_range = (int)std::ceil(par.pairlist / (par.box[1] / ncy)) + 1;
+  d.colw_x = (unsigned long long)(kTwo64 / ncx);
+  d.colw_y = (unsigned long long)(kTwo64 / ncy);
   const int clusters_cap = n / CLUSTER + d.ncol + 64;
-  const double hw = 0.5 * w;
-  const double est = 4.0 / 3.0 * kPi * std::pow(par.pairlist + 2.0 * hw + 1.0, 3) * rho / CLUSTER * 0.5 + 2.0;
+  const double cw = std::sqrt(par.box[0] / ncx * par.box[1] / ncy);
+  const double est = 4.0 / 3.0 * kPi * std::pow(par.pairlist + cw + 1.0, 3) * rho / CLUSTER * 0.5 + 2.0;
   long long per = std::min<long long>(clusters_cap, (long long)std::ceil(1.6 * est));
-  return alloc_neighbor((long long)clusters_cap * per + 4096, clusters_cap);
+  FZ_TRY(alloc_neighbor((long long)clusters_cap * per + 4096, clusters_cap));
+  gen++;
+  need_rebuild = true;
+  return FZ_OK;
 }
 
 int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   SysDev& d = dv;
   nb_alloc0 = allocs.size();
+  FZ_TRY(dalloc(&d.col_count, d.ncol)); FZ_TRY(dalloc(&d.col_fill, d.ncol)); FZ_TRY(dalloc(&d.col_clu0, d.ncol + 1));
   d.n_clusters_cap = clusters_cap;
   d.n_slots_cap = clusters_cap * CLUSTER;
   d.n_pairs_cap = pairs_cap;
@@ -423,6 +438,7 @@ int System::upload_fixed(const uint64_t* u) {
     FZ_HIP(hipMemcpy(dst[k], c.data(), n * sizeof(unsigned long long), hipMemcpyHostToDevice));
   }
   need_rebuild = true;
+  min_step = 0.0;
   return FZ_OK;
 }
 
@@ -565,7 +581,7 @@ int Engine::prepare() {
   int kmax = 4;
   for (auto* s : sys_) kmax = std::max({kmax, s->dv.K[0], s->dv.K[1], s->dv.K[2]});
   const int csz = L_.mixed ? 8 : 16;
-  int NL = 16;
+  int NL = tuned_nl_ > 0 ? tuned_nl_ : 16;
   while (NL > 1 && (size_t)NL * 2 * kmax * csz > 49152) NL >>= 1;
   L_.fft_nl = NL;
   L_.fft_lds = NL * 2 * kmax * csz;
@@ -592,7 +608,7 @@ int Engine::prepare() {
   FZ_TRY(make_work(L_.w_fftx, fx_));
   FZ_TRY(make_work(L_.w_ffty, fy_));
   FZ_TRY(make_work(L_.w_fftz, fz_));
-  L_.pair_grid = 2048;
+  L_.pair_grid = tuned_grid_ > 0 ? tuned_grid_ : 2048;
   prepared_ = true;
   gens_.clear();
   for (auto* s : sys_) gens_.push_back(s->gen);
@@ -707,32 +723,32 @@ int Engine::steps(int n) {
   FZ_TRY(ensure());
   for (auto* s : sys_)
     if (s->need_rebuild) FZ_TRY(forces(true));
+  auto copy3 = [&](unsigned long long* const* dst, unsigned long long* const* src, size_t nb) -> int {
+    for (int c = 0; c < 3; c++) FZ_HIP(hipMemcpyAsync(dst[c], src[c], nb, hipMemcpyDeviceToDevice, L_.stream));
+    return FZ_OK;
+  };
   for (int attempt = 0; attempt < 4; attempt++) {
-    // snapshot u, v for replay after a capacity overflow
+    // snapshot positions, velocities and the positions of the last list build
+    // for replay after a capacity overflow
     for (auto* s : sys_) {
-      const size_t nb = (size_t)s->topo.n * 8;
-      void* src[6] = {s->dv.ux, s->dv.uy, s->dv.uz, s->dv.vx, s->dv.vy, s->dv.vz};
-      for (int k = 0; k < 6; k++)
-        FZ_HIP(hipMemcpyAsync((char*)s->snap + k * nb, src[k], nb, hipMemcpyDeviceToDevice, L_.stream));
+      const size_t n3 = s->topo.n;
+      unsigned long long* sn[3] = {s->snap, s->snap + n3, s->snap + 2 * n3};
+      unsigned long long* sv[3] = {s->snap + 3 * n3, s->snap + 4 * n3, s->snap + 5 * n3};
+      unsigned long long* sr[3] = {s->snap + 6 * n3, s->snap + 7 * n3, s->snap + 8 * n3};
+      unsigned long long* u[3] = {s->dv.ux, s->dv.uy, s->dv.uz};
+      unsigned long long* v[3] = {(unsigned long long*)s->dv.vx, (unsigned long long*)s->dv.vy,
+                                  (unsigned long long*)s->dv.vz};
+      unsigned long long* r[3] = {s->dv.rx, s->dv.ry, s->dv.rz};
+      FZ_TRY(copy3(sn, u, n3 * 8));
+      FZ_TRY(copy3(sv, v, n3 * 8));
+      FZ_TRY(copy3(sr, r, n3 * 8));
     }
     if (!g_plain_) FZ_TRY(capture(false, &g_plain_));
     if (!g_ev_) FZ_TRY(capture(true, &g_ev_));
     for (int k = 0; k < n; k++) FZ_HIP(hipGraphLaunch(k == n - 1 ? g_ev_ : g_plain_, L_.stream));
     FZ_TRY(read_status());
     bool again = false;
-    for (size_t k = 0; k < sys_.size(); k++) {
-      System* s = sys_[k];
-      if (s->flags_host[F_OVERFLOW]) {
-        // restore the snapshot, grow, re-evaluate forces, replay
-        const size_t nb = (size_t)s->topo.n * 8;
-        void* dst[6] = {s->dv.ux, s->dv.uy, s->dv.uz, s->dv.vx, s->dv.vy, s->dv.vz};
-        for (int c = 0; c < 6; c++)
-          FZ_HIP(hipMemcpyAsync(dst[c], (char*)s->snap + c * nb, nb, hipMemcpyDeviceToDevice, L_.stream));
-        FZ_HIP(hipStreamSynchronize(L_.stream));
-        FZ_TRY(grow_and_reset((int)k, s->flags_host[F_OVERFLOW]));
-        again = true;
-      }
-    }
+    for (auto* s : sys_) again = again || s->flags_host[F_OVERFLOW];
     if (!again) {
       for (auto* s : sys_) {
         if (s->flags_host[F_SHAKE_FAIL]) return set_error(FZ_ERR_SHAKE, "SHAKE did not converge");
@@ -740,8 +756,41 @@ int Engine::steps(int n) {
       }
       return FZ_OK;
     }
+    // Every member returns to the snapshot.  Members that overflowed grow and
+    // rebuild at the snapshot positions; the others rebuild their list at the
+    // snapshot's build positions, which restores the exact list they held, so
+    // their replay repeats what they would have computed without the overflow.
+    std::vector<int> grown(sys_.size(), 0);
+    for (size_t k = 0; k < sys_.size(); k++) {
+      System* s = sys_[k];
+      const size_t n3 = s->topo.n;
+      unsigned long long* sn[3] = {s->snap, s->snap + n3, s->snap + 2 * n3};
+      unsigned long long* sv[3] = {s->snap + 3 * n3, s->snap + 4 * n3, s->snap + 5 * n3};
+      unsigned long long* sr[3] = {s->snap + 6 * n3, s->snap + 7 * n3, s->snap + 8 * n3};
+      unsigned long long* u[3] = {s->dv.ux, s->dv.uy, s->dv.uz};
+      unsigned long long* v[3] = {(unsigned long long*)s->dv.vx, (unsigned long long*)s->dv.vy,
+                                  (unsigned long long*)s->dv.vz};
+      FZ_TRY(copy3(v, sv, n3 * 8));
+      FZ_TRY(copy3(u, s->flags_host[F_OVERFLOW] ? sn : sr, n3 * 8));
+      FZ_HIP(hipStreamSynchronize(L_.stream));
+      if (s->flags_host[F_OVERFLOW]) {
+        FZ_TRY(grow_and_reset((int)k, s->flags_host[F_OVERFLOW]));
+        grown[k] = 1;
+      } else {
+        s->need_rebuild = true;
+      }
+    }
     FZ_TRY(prepare());
-    FZ_TRY(forces(true));
+    FZ_TRY(forces(true));  // builds every list (grown members at their positions)
+    for (size_t k = 0; k < sys_.size(); k++) {
+      if (grown[k]) continue;
+      System* s = sys_[k];
+      const size_t n3 = s->topo.n;
+      unsigned long long* sn[3] = {s->snap, s->snap + n3, s->snap + 2 * n3};
+      unsigned long long* u[3] = {s->dv.ux, s->dv.uy, s->dv.uz};
+      FZ_TRY(copy3(u, sn, n3 * 8));
+    }
+    FZ_TRY(forces(true));  // forces at the snapshot positions with the restored lists
   }
   return set_error(FZ_ERR_CAPACITY, "neighbour capacity could not be grown");
 }
@@ -759,6 +808,8 @@ int Engine::minimize(int nsteps, double max_step) {
   FZ_TRY(ensure());
   const int ns = (int)sys_.size();
   std::vector<double> step(ns, max_step), e_old(ns);
+  for (int s = 0; s < ns; s++)
+    if (sys_[s]->min_step > 0.0) step[s] = std::min(sys_[s]->min_step, max_step);
   std::vector<int> reject(ns, 0);
   int* d_reject = nullptr;
   FZ_HIP(hipMalloc(&d_reject, ns * sizeof(int)));
@@ -785,12 +836,11 @@ int Engine::minimize(int nsteps, double max_step) {
       FZ_TRY(forces(true));
     }
   }
+  // the adaptive step carries over to the next call, so a minimisation split
+  // into several calls takes the same path as one call
+  for (int s = 0; s < ns; s++) sys_[s]->min_step = step[s];
   launch_zero_vel(L_);
-  launch_zero_acc(L_);
-  record_forces(true);
-  launch_kinetic(L_);
-  FZ_HIP(hipGetLastError());
-  return read_status();
+  return forces(true);
 }
 
 int Engine::phase_times(int nsteps, double* out) {
@@ -836,6 +886,82 @@ int Engine::phase_times(int nsteps, double* out) {
   for (int k = 0; k < NP; k++) out[k] /= std::max(1, nsteps);
   FZ_TRY(read_status());
   for (auto* s : sys_) s->step += nsteps;
+  return FZ_OK;
+}
+
+int Engine::time_steps(int nsteps, double* ms) {
+  FZ_TRY(steps(2));  // graph capture and warm-up outside the timed region
+  FZ_TRY(sync());
+  hipEvent_t a, b;
+  FZ_HIP(hipEventCreate(&a));
+  FZ_HIP(hipEventCreate(&b));
+  (void)hipEventRecord(a, L_.stream);
+  int rc = steps(nsteps);
+  (void)hipEventRecord(b, L_.stream);
+  (void)hipEventSynchronize(b);
+  float t = 0.f;
+  (void)hipEventElapsedTime(&t, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = t / std::max(1, nsteps);
+  return rc;
+}
+
+// Coordinate descent over the launch-shape knobs the reference fixes at
+// compile time (CELL_BLOCKSIZE / CELL_ATOMS presets, src/configure:89-137):
+// column width (atoms per column and cluster compactness), FFT lines per
+// workgroup, persistent pair-kernel grid.  Each candidate is timed on
+// graph-replayed steps; the systems' positions, velocities and step counters
+// are restored afterwards.
+int Engine::autotune(int nsteps, double* out) {
+  FZ_TRY(ensure());
+  const int ns = (int)sys_.size();
+  std::vector<std::vector<uint64_t>> u0(ns);
+  std::vector<std::vector<double>> v0(ns);
+  std::vector<int64_t> st0(ns);
+  for (int s = 0; s < ns; s++) {
+    u0[s].resize((size_t)3 * sys_[s]->topo.n);
+    v0[s].resize((size_t)3 * sys_[s]->topo.n);
+    FZ_TRY(sys_[s]->download_state(nullptr, v0[s].data(), u0[s].data()));
+    st0[s] = sys_[s]->step;
+  }
+  auto restore = [&]() -> int {
+    for (int s = 0; s < ns; s++) {
+      FZ_TRY(sys_[s]->upload_fixed(u0[s].data()));
+      FZ_TRY(sys_[s]->upload_velocities(v0[s].data()));
+      sys_[s]->step = st0[s];
+    }
+    return FZ_OK;
+  };
+  auto trial = [&](double f, int nl, int grid, double* ms) -> int {
+    for (auto* s : sys_) FZ_TRY(s->set_column_grid(0, 0, f));
+    tuned_nl_ = nl;
+    tuned_grid_ = grid;
+    FZ_TRY(prepare());
+    FZ_TRY(restore());
+    return time_steps(nsteps, ms);
+  };
+  double best_f = 1.0, best_ms = 1e30, ms = 0;
+  int best_nl = 16, best_grid = 2048;
+  for (double f : {0.8, 0.9, 1.0, 1.15, 1.3}) {
+    FZ_TRY(trial(f, best_nl, best_grid, &ms));
+    if (ms < best_ms) { best_ms = ms; best_f = f; }
+  }
+  for (int nl : {4, 8, 16}) {
+    FZ_TRY(trial(best_f, nl, best_grid, &ms));
+    if (ms < best_ms) { best_ms = ms; best_nl = nl; }
+  }
+  for (int grid : {512, 1024, 4096, 8192}) {
+    FZ_TRY(trial(best_f, best_nl, grid, &ms));
+    if (ms < best_ms) { best_ms = ms; best_grid = grid; }
+  }
+  for (auto* s : sys_) FZ_TRY(s->set_column_grid(0, 0, best_f));
+  tuned_nl_ = best_nl;
+  tuned_grid_ = best_grid;
+  FZ_TRY(prepare());
+  FZ_TRY(restore());
+  FZ_TRY(forces(true));
+  out[0] = best_f; out[1] = best_nl; out[2] = best_grid; out[3] = best_ms;
   return FZ_OK;
 }
 

@@ -59,6 +59,7 @@ struct System {
   int64_t step = 0;
   int gen = 0;  // bumped whenever device buffers are reallocated
   double self_energy = 0.0;
+  double min_step = 0.0;  // adaptive minimiser step carried between calls (0 = start at the maximum)
   bool need_rebuild = true;
   // snapshot buffers for capacity-overflow recovery
   unsigned long long* snap = nullptr;  // 3n u + 3n v (as bits)
@@ -69,6 +70,7 @@ struct System {
   int init(const fz_topology* t, const fz_params* p, int precision, int device);
   int alloc_device();
   int alloc_neighbor(long long pairs_cap, int clusters_cap);
+  int set_column_grid(int ncx, int ncy, double width_factor);
   void free_device();
   template <class T> int dalloc(T** p, size_t count, bool zero = true);
   int upload_positions(const double* xyz);
@@ -91,6 +93,11 @@ class Engine {
   // nsteps eager (non-graph) steps with HIP events between phases; mean ms per
   // phase per step into out[FZ_NPHASE] (order: see fz_phase_times in fenzi.h)
   int phase_times(int nsteps, double* out);
+  // Times graph-replayed steps over candidate column widths, FFT lines per
+  // workgroup and persistent pair-kernel grids (coordinate descent), keeps the
+  // fastest, and restores the systems' state.  out: {width factor, fft lines,
+  // pair grid, best ms per step}.
+  int autotune(int nsteps, double* out);
   int read_status();                   // flags + accumulators of every system -> host
   int sync();
   hipStream_t stream() const { return L_.stream; }
@@ -106,7 +113,9 @@ class Engine {
   hipGraphExec_t g_plain_ = nullptr, g_ev_ = nullptr;
   bool prepared_ = false;
   std::vector<int> gens_;
+  int tuned_nl_ = 0, tuned_grid_ = 0;  // autotuned launch shapes (0 = defaults)
   int ensure();  // (re)prepare when a member's buffers moved since the last prepare
+  int time_steps(int nsteps, double* ms);
   int make_work(Work& w, const std::vector<int>& blocks);
   int capture(bool ev, hipGraphExec_t* out);
   void record_step(bool ev);
@@ -120,3 +129,8 @@ int pme_convolve(int device, const int32_t K[3], const double box[3], double bet
                  const double* q, double* phi, double* energy);
 
 }  // namespace fz
+
+// device context handle of the C ABI
+struct fz_ctx {
+  int device;
+};

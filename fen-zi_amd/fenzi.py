@@ -69,10 +69,28 @@ class RunSettings(C.Structure):
 EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create", "fz_system_destroy",
            "fz_set_state", "fz_get_state", "fz_set_fixed_positions", "fz_get_fixed_positions",
            "fz_reassign_velocities", "fz_compute_forces", "fz_step", "fz_get_energies", "fz_minimize",
-           "fz_system_info", "fz_phase_times", "fz_get_cluster_layout", "fz_get_cluster_pairs", "fz_pme_convolve",
+           "fz_system_info", "fz_phase_times", "fz_autotune", "fz_batch_autotune", "fz_get_cluster_layout", "fz_get_cluster_pairs", "fz_pme_convolve",
            "fz_batch_create", "fz_batch_step", "fz_batch_minimize", "fz_batch_destroy", "fz_deck_load", "fz_deck_destroy",
            "fz_deck_topology", "fz_deck_params", "fz_deck_coordinates", "fz_deck_settings",
-           "fz_deck_psf_counts", "fz_format_energy_header", "fz_format_energy_row"]
+           "fz_deck_psf_counts", "fz_format_energy_header", "fz_format_energy_row",
+           "fz_server_create", "fz_server_submit", "fz_server_poll", "fz_server_job_info", "fz_server_job_result",
+           "fz_server_release", "fz_server_estimate_bytes", "fz_server_destroy"]
+
+JOB_QUEUED, JOB_MINIMIZING, JOB_DYNAMICS, JOB_DONE, JOB_FAILED = 0, 1, 2, 3, -1
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_double), ("max_active", C.c_int32), ("precision", C.c_int32),
+                ("chunk_steps", C.c_int32), ("min_max_step", C.c_double)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("id", C.c_int64), ("state", C.c_int32), ("n_atoms", C.c_int64), ("steps_done", C.c_int64),
+                ("min_done", C.c_int64), ("device_bytes", C.c_int64), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 _lib = None
 
@@ -104,6 +122,8 @@ def lib():
         L.fz_minimize.argtypes = [vp, C.c_int, C.c_double]
         L.fz_system_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
         L.fz_phase_times.argtypes = [vp, C.c_int, _dp]
+        L.fz_autotune.argtypes = [vp, C.c_int, _dp]
+        L.fz_batch_autotune.argtypes = [vp, C.c_int, _dp]
         L.fz_get_cluster_layout.argtypes = [vp, _ip, _ip, _ip, C.c_int64, C.POINTER(C.c_int64)]
         L.fz_get_cluster_pairs.argtypes = [vp, _ip, _up, C.c_int64]
         L.fz_get_cluster_pairs.restype = C.c_int64
@@ -124,6 +144,16 @@ def lib():
         L.fz_deck_psf_counts.argtypes = [vp, C.POINTER(C.c_int64)]
         L.fz_format_energy_header.argtypes = [C.c_char_p, C.c_size_t]
         L.fz_format_energy_row.argtypes = [C.POINTER(Energies), C.c_double, C.c_char_p, C.c_size_t]
+        L.fz_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
+        L.fz_server_submit.argtypes = [vp, C.POINTER(Topology), C.POINTER(Params), _dp, C.c_int32, C.c_int32,
+                                       C.c_double, C.c_int32, C.c_uint64, C.POINTER(C.c_int64)]
+        L.fz_server_poll.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.fz_server_job_info.argtypes = [vp, C.c_int64, C.POINTER(JobInfo)]
+        L.fz_server_job_result.argtypes = [vp, C.c_int64, _dp, _dp, _up, C.POINTER(Energies)]
+        L.fz_server_release.argtypes = [vp, C.c_int64]
+        L.fz_server_estimate_bytes.argtypes = [C.POINTER(Topology), C.POINTER(Params), C.c_int32]
+        L.fz_server_estimate_bytes.restype = C.c_int64
+        L.fz_server_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -275,6 +305,11 @@ class System:
         _check(lib().fz_phase_times(self.h, int(nsteps), _p(ms, C.c_double)), "fz_phase_times")
         return dict(zip(self.PHASES, ms.tolist()))
 
+    def autotune(self, nsteps=20):
+        out = np.zeros(4)
+        _check(lib().fz_autotune(self.h, int(nsteps), _p(out, C.c_double)), "fz_autotune")
+        return dict(column_width=out[0], fft_lines=int(out[1]), pair_grid=int(out[2]), ms_per_step=out[3])
+
     def info(self):
         n, b, p = C.c_int64(), C.c_int64(), C.c_int32()
         _check(lib().fz_system_info(self.h, C.byref(n), C.byref(b), C.byref(p)), "fz_system_info")
@@ -314,9 +349,88 @@ class Batch:
     def minimize(self, n, max_step=0.05):
         _check(lib().fz_batch_minimize(self.h, int(n), float(max_step)), "fz_batch_minimize")
 
+    def autotune(self, nsteps=10):
+        out = np.zeros(4)
+        _check(lib().fz_batch_autotune(self.h, int(nsteps), _p(out, C.c_double)), "fz_batch_autotune")
+        return dict(column_width=out[0], fft_lines=int(out[1]), pair_grid=int(out[2]), ms_per_step=out[3])
+
     def close(self):
         if self.h:
             lib().fz_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def estimate_bytes(topo, params, precision=MIXED):
+    """Device bytes the server budgets for one job of this topology."""
+    keep = []
+    t = make_topology(topo, keep)
+    p = make_params(params)
+    return _check(lib().fz_server_estimate_bytes(C.byref(t), C.byref(p), precision), "fz_server_estimate_bytes")
+
+
+class Server:
+    """Continuous-batching job server (fz_server_* in include/fenzi.h)."""
+
+    def __init__(self, ctx, hbm_budget_bytes, max_active=64, precision=MIXED, chunk_steps=100, min_max_step=0.05):
+        self.ctx = ctx
+        cfg = ServerConfig(float(hbm_budget_bytes), int(max_active), int(precision), int(chunk_steps),
+                           float(min_max_step))
+        self.h = C.c_void_p()
+        _check(lib().fz_server_create(ctx.h, C.byref(cfg), C.byref(self.h)), "fz_server_create")
+        self.n_atoms = {}
+
+    def submit(self, topo, params, xyz, min_steps, md_steps, temperature=300.0, reassign_every=0, seed=0):
+        keep = []
+        t = make_topology(topo, keep)
+        p = make_params(params)
+        x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        jid = C.c_int64()
+        _check(lib().fz_server_submit(self.h, C.byref(t), C.byref(p), _p(x, C.c_double), int(min_steps),
+                                      int(md_steps), float(temperature), int(reassign_every), int(seed),
+                                      C.byref(jid)), "fz_server_submit")
+        self.n_atoms[jid.value] = x.shape[0]
+        return jid.value
+
+    def poll(self):
+        a, q, d = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().fz_server_poll(self.h, C.byref(a), C.byref(q), C.byref(d)), "fz_server_poll")
+        return a.value, q.value, d.value
+
+    def info(self, jid):
+        i = JobInfo()
+        _check(lib().fz_server_job_info(self.h, int(jid), C.byref(i)), "fz_server_job_info")
+        return i.as_dict()
+
+    def result(self, jid):
+        n = self.n_atoms[jid]
+        x = np.empty((n, 3))
+        v = np.empty((n, 3))
+        u = np.empty((n, 3), np.uint64)
+        e = Energies()
+        _check(lib().fz_server_job_result(self.h, int(jid), _p(x, C.c_double), _p(v, C.c_double),
+                                          _p(u, C.c_uint64), C.byref(e)), "fz_server_job_result")
+        return dict(xyz=x, vel=v, fixed=u, energies=e.as_dict())
+
+    def release(self, jid):
+        _check(lib().fz_server_release(self.h, int(jid)), "fz_server_release")
+        self.n_atoms.pop(jid, None)
+
+    def run_until_done(self, max_rounds=1000000):
+        for _ in range(max_rounds):
+            a, q, _ = self.poll()
+            if a == 0 and q == 0:
+                return
+        raise FenziError("server did not drain")
+
+    def close(self):
+        if self.h:
+            lib().fz_server_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

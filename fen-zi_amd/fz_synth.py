@@ -362,6 +362,84 @@ def ensemble_member(n_target, chain_frac, seed, cutoff=10.0):
     return topo, xyz, params
 
 
+def write_deck(topo, xyz, params, directory, name="synth", temperature=300.0, steps=1000, shake=True,
+               reassign_every=0, output_every=100, seed=1):
+    """Write a FEN ZI input deck (.in / .psf / .prm / .crd) for a system built
+    by make_system, with string atom types (no .rtf), in CHARMM formats."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    n = int(topo["n_atoms"])
+    tnames = TYPE_NAMES[: int(topo["n_types"])]
+    res = ["CHN" if t == 2 else "TIP3" for t in topo["type"]]
+    with open(os.path.join(directory, name + ".psf"), "w") as f:
+        f.write("PSF\n\n       1 !NTITLE\n* synthetic system\n\n")
+        f.write("%8d !NATOM\n" % n)
+        for i in range(n):
+            t = int(topo["type"][i])
+            f.write("%8d %-4s %-4d %-4s %-4s %-4s %14.6f %14.6f %8d\n" % (
+                i + 1, "SYS", i // 3 + 1, res[i], tnames[t], tnames[t], topo["charge"][i], topo["mass"][i], 0))
+
+        def section(tag, arr, width, per_line):
+            arr = np.asarray(arr).reshape(-1, width) + 1
+            f.write("\n%8d %s\n" % (len(arr), tag))
+            for k in range(0, len(arr), per_line):
+                f.write("".join("%8d" % v for v in arr[k:k + per_line].ravel()) + "\n")
+        section("!NBOND: bonds", topo["bond_atoms"], 2, 4)
+        section("!NTHETA: angles", topo["angle_atoms"], 3, 3)
+        section("!NPHI: dihedrals", topo["dihedral_atoms"], 4, 2)
+        section("!NIMPHI: impropers", topo["improper_atoms"], 4, 2)
+    with open(os.path.join(directory, name + ".prm"), "w") as f:
+        f.write("* synthetic parameters\n*\n\nBONDS\n")
+        seen = set()
+        ba = np.asarray(topo["bond_atoms"]).reshape(-1, 2)
+        for (a, b), k, r0 in zip(ba, topo["bond_k"], topo["bond_r0"]):
+            key = (tnames[topo["type"][a]], tnames[topo["type"][b]])
+            if key not in seen:
+                seen.add(key)
+                f.write("%-6s %-6s %16.10f %16.10f\n" % (key + (k, r0)))
+        f.write("\nANGLES\n")
+        seen = set()
+        aa = np.asarray(topo["angle_atoms"]).reshape(-1, 3)
+        for (a, b, c), k, t0 in zip(aa, topo["angle_k"], topo["angle_theta0"]):
+            key = tuple(tnames[topo["type"][x]] for x in (a, b, c))
+            if key not in seen:
+                seen.add(key)
+                f.write("%-6s %-6s %-6s %16.10f %18.12f\n" % (key + (k, math.degrees(t0))))
+        f.write("\nDIHEDRALS\n")
+        seen = set()
+        da = np.asarray(topo["dihedral_atoms"]).reshape(-1, 4)
+        for q, k, nn, dl in zip(da, topo["dihedral_k"], topo["dihedral_n"], topo["dihedral_delta"]):
+            key = tuple(tnames[topo["type"][x]] for x in q) + (int(nn),)
+            if key not in seen:
+                seen.add(key)
+                f.write("%-6s %-6s %-6s %-6s %16.10f %3d %16.10f\n" % (key[:4] + (k, nn, math.degrees(dl))))
+        f.write("\nIMPROPER\n\nNONBONDED nbxmod  5 atom cdiel shift vatom vdistance vswitch -\n"
+                "cutnb 14.0 ctofnb 12.0 ctonnb 10.0 eps 1.0 e14fac 1.0 wmin 1.5\n")
+        for t, nm in enumerate(tnames):
+            f.write("%-6s %10.6f %16.10f %16.10f\n" % (nm, 0.0, -TYPE_EPS[t], TYPE_RMIN2[t]))
+        f.write("\nEND\n")
+    with open(os.path.join(directory, name + ".crd"), "w") as f:
+        f.write("* synthetic coordinates\n*\n%10d  EXT\n" % n)
+        for i in range(n):
+            t = int(topo["type"][i])
+            f.write("%10d%10d  %-8s  %-8s%20.10f%20.10f%20.10f  %-8s  %-8d%20.10f\n" % (
+                i + 1, i // 3 + 1, res[i], tnames[t], xyz[i, 0], xyz[i, 1], xyz[i, 2], "SYS", i // 3 + 1, 0.0))
+    box = params["box"]
+    K = params["K"]
+    with open(os.path.join(directory, name + ".in"), "w") as f:
+        f.write("# synthetic deck\ncoordinates %s.crd\nstructure %s.psf\nparameters %s.prm\n" % (name, name, name))
+        f.write("temperature %g\nseed %d\ntimestep %g\ndynasteps %d\noutputTiming %d\n" % (
+            temperature, seed, params["dt_ps"], steps, output_every))
+        if reassign_every:
+            f.write("tempresetTiming %d\n" % reassign_every)
+        f.write("cuton %g\ncutoff %g\npairlistdist %g\nKAPPa %g\n" % (
+            params["cuton"], params["cutoff"], params["pairlist"], params["beta"]))
+        f.write("cellBasisVector1 %.10f 0 0\ncellBasisVector2 0 %.10f 0\ncellBasisVector3 0 0 %.10f\n" % tuple(box))
+        f.write("PMEGridSizeX %d\nPMEGridSizeY %d\nPMEGridSizeZ %d\n" % tuple(K))
+        f.write("shake %s %g\nprinttostdout 1\n" % ("on" if shake else "off", params.get("shake_tol", 1e-8)))
+    return os.path.join(directory, name + ".in")
+
+
 def batch_sizes(n_systems=256, lo=1500, hi=24000, seed=3):
     rng = np.random.default_rng(seed)
     n = np.exp(rng.uniform(math.log(lo), math.log(hi), n_systems)).astype(int)

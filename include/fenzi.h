@@ -147,6 +147,13 @@ int fz_system_info(fz_system *sys, int64_t *n_atoms, int64_t *device_bytes, int3
  * (src/globals.h:796-823). */
 #define FZ_NPHASE 9
 int fz_phase_times(fz_system *sys, int nsteps, double *ms);
+/* Autotuner: times graph-replayed steps for candidate column widths (atoms per
+ * binning column and cluster compactness), FFT lines per workgroup and
+ * persistent pair-kernel grid sizes, keeps the fastest and restores the state.
+ * out[4] = {column width factor, FFT lines, pair grid blocks, ms per step}.
+ * Replaces the compile-time CELL_BLOCKSIZE / CELL_ATOMS presets of the
+ * reference's configure step (src/configure:89-137). */
+int fz_autotune(fz_system *sys, int nsteps, double *out);
 
 /* ---- inspection (parity tests) ------------------------------------------------ */
 /* Column grid and cluster layout of the last neighbour build: column of each
@@ -171,7 +178,53 @@ int fz_batch_create(fz_ctx *ctx, fz_system *const *systems, int n, fz_batch **ou
 int fz_batch_step(fz_batch *b, int nsteps);
 /* Steepest descent on every member (per-system step control), one launch per phase. */
 int fz_batch_minimize(fz_batch *b, int nsteps, double max_step);
+/* fz_autotune over the whole batch (one setting for all members). */
+int fz_batch_autotune(fz_batch *b, int nsteps, double *out);
 void fz_batch_destroy(fz_batch *b);
+
+/* ---- serving: continuous batching of MD jobs ------------------------------------------ */
+/* The reference runs one system per process (main, src/fenzi.cu:112); the
+ * server keeps many jobs resident in HBM.  Each job: steepest-descent
+ * minimisation, Maxwell velocities, NVT dynamics with velocity reassignment
+ * every reassign_every steps.  Jobs are admitted while the estimated HBM
+ * footprint of the active set stays within the budget and fewer than
+ * max_active are running; every fz_server_poll advances all minimising jobs
+ * as one batch and all dynamics jobs as one batch, then retires finished jobs
+ * and admits queued ones, so arrivals join running work without waiting for
+ * it to drain.  A job that fails is marked failed and removed; the others
+ * continue. */
+typedef struct fz_server_config {
+  double hbm_budget_bytes;  /* admission budget for resident job state */
+  int32_t max_active;       /* at most this many jobs on the device */
+  int32_t precision;        /* FZ_MIXED or FZ_FP64 */
+  int32_t chunk_steps;      /* steps (or minimiser iterations) per poll round */
+  double min_max_step;      /* steepest-descent step cap (A) */
+} fz_server_config;
+
+enum { FZ_JOB_QUEUED = 0, FZ_JOB_MINIMIZING = 1, FZ_JOB_DYNAMICS = 2, FZ_JOB_DONE = 3, FZ_JOB_FAILED = -1 };
+
+typedef struct fz_job_info {
+  int64_t id;
+  int32_t state;
+  int64_t n_atoms, steps_done, min_done, device_bytes;
+  double t_submit, t_admit, t_done; /* seconds since the server was created */
+} fz_job_info;
+
+int fz_server_create(fz_ctx *ctx, const fz_server_config *cfg, fz_server **out);
+/* Copies the topology, parameters and coordinates; returns the job id. */
+int fz_server_submit(fz_server *srv, const fz_topology *topo, const fz_params *par, const double *xyz,
+                     int32_t min_steps, int32_t md_steps, double temperature, int32_t reassign_every,
+                     uint64_t seed, int64_t *job_id);
+/* One round: admit, advance, retire.  Counts after the round (nullable). */
+int fz_server_poll(fz_server *srv, int32_t *n_active, int32_t *n_queued, int32_t *n_done);
+int fz_server_job_info(fz_server *srv, int64_t job_id, fz_job_info *info);
+/* Final state of a finished job (any pointer nullable). */
+int fz_server_job_result(fz_server *srv, int64_t job_id, double *xyz, double *vel, uint64_t *fixed, fz_energies *e);
+/* Releases a finished job's host-side result. */
+int fz_server_release(fz_server *srv, int64_t job_id);
+/* Estimated device bytes of one job (the admission figure). */
+int64_t fz_server_estimate_bytes(const fz_topology *topo, const fz_params *par, int32_t precision);
+void fz_server_destroy(fz_server *srv);
 
 /* ---- input deck (.in / .psf / .prm / .rtf / .crd) --------------------------------- */
 /* Replaces ReadInput, ReadCrd/ReadPdb/ReadXYZ, ReadPsf_CHARMM, ReadPrm_CHARMM

@@ -20,3 +20,19 @@ def slab(n_chains=6, n_waters=400, L=26.0, seed=5, K=(27, 27, 27)):
 
 def rel_rms(a, b):
     return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def dmpc_small_deck(tmpdir):
+    """Unpack the reference's dmpc_small input deck (tests/examples_fenzi/dmpc_small,
+    kept here as a gzip'd fixture) and return the path of its .in file."""
+    import gzip
+    import os
+    import shutil
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dmpc_small")
+    for f in os.listdir(src):
+        if f.endswith(".gz"):
+            with gzip.open(os.path.join(src, f), "rb") as a, open(os.path.join(tmpdir, f[:-3]), "wb") as b:
+                shutil.copyfileobj(a, b)
+        else:
+            shutil.copy(os.path.join(src, f), os.path.join(tmpdir, f))
+    return os.path.join(tmpdir, "fenzi.in")

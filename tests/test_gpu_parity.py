@@ -2,7 +2,10 @@
 import numpy as np
 import pytest
 
+import os
+
 import fenzi
+import fz_synth
 import oracle
 import systems
 
@@ -184,3 +187,53 @@ def test_minimize_lowers_energy_and_keeps_constraints(ctx, prec):
     # energies after minimisation agree with the oracle at the new positions
     f_ref, e_ref, _ = orc.forces(s.get_fixed())
     assert systems.rel_rms(f1, f_ref) < (1e-10 if prec == fenzi.FP64 else 1e-5)
+
+
+@pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
+def test_reference_deck_forces_vs_oracle(ctx, prec, tmp_path):
+    """The reference's own dmpc_small input (CHARMM lipid + SPC/Fw water, with
+    Urey-Bradley, impropers, NBFIX and multi-term dihedrals) through the parser,
+    GPU vs oracle."""
+    deck = fenzi.Deck(systems.dmpc_small_deck(str(tmp_path)))
+    topo, params, xyz = deck.topology(), deck.params(), deck.coordinates()
+    orc = oracle.Oracle(topo, params)
+    u = orc.to_fixed(xyz)
+    f_ref, e_ref, vir_ref = orc.forces(u)
+    s = fenzi.System(ctx, topo, params, prec)
+    s.set_fixed(u)
+    f, e = s.forces()
+    assert systems.rel_rms(f, f_ref) < (1e-10 if prec == fenzi.FP64 else 1e-5)
+    epot_ref = sum(e_ref.values())
+    assert abs(e["potential"] - epot_ref) < (1e-8 if prec == fenzi.FP64 else 1e-5) * sum(abs(v) for v in e_ref.values())
+    assert e["pairs_within_cutoff"] == orc.count_pairs_within(u, params["cutoff"])
+
+
+@pytest.mark.parametrize("cid,prec", [(1, fenzi.FP64), (1, fenzi.MIXED), (2, fenzi.MIXED)])
+def test_full_size_workloads_vs_oracle(ctx, cid, prec):
+    """The 17K and 230K slabs at their full size (generated coordinates)."""
+    topo, xyz, params = fz_synth.config(cid)
+    orc = oracle.Oracle(topo, params)
+    u = orc.to_fixed(xyz)
+    f_ref, e_ref, vir_ref = orc.forces(u)
+    s = fenzi.System(ctx, topo, params, prec)
+    s.set_fixed(u)
+    f, e = s.forces()
+    assert systems.rel_rms(f, f_ref) < (1e-10 if prec == fenzi.FP64 else 1e-5)
+    assert np.abs(e["virial"] - vir_ref).max() < (1e-8 if prec == fenzi.FP64 else 1e-5) * np.abs(vir_ref).max()
+    for k, name in (("vdw", "vdw"), ("elec", "elec"), ("pme", "pme"), ("ewexcl", "ewald_excl")):
+        assert abs(e[name] - e_ref[k]) < (1e-8 if prec == fenzi.FP64 else 1e-5) * max(1.0, abs(e_ref[k])), k
+
+
+def test_fenzi_driver_runs_a_deck(tmp_path):
+    import subprocess
+    topo, xyz, params = systems.slab()
+    path = fz_synth.write_deck(topo, xyz, params, str(tmp_path), "drv", steps=40, output_every=20, reassign_every=20)
+    exe = os.path.join(os.path.dirname(fenzi.LIB_PATH), "fenzi")
+    out = subprocess.run([exe, "-d", "0", "-f", path, "-p", "mixed"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    assert lines[0] == fenzi.energy_header().rstrip("\n")
+    rows = [l.split() for l in lines[1:] if l and l[0].isdigit()]
+    assert [int(r[0]) for r in rows] == [0, 20, 40]
+    assert all(len(r) == 16 for r in rows)
+    assert "Ns per day" in out.stdout

@@ -1,0 +1,320 @@
+"""Pins for the FP64 oracle (CPU only): known-answer vectors, closed forms,
+library routines and finite differences, each chosen so that a plausible
+mistake in the oracle (a dropped term, a wrong sign or index, a transposed
+operand) fails at least one of them."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import fz_synth
+import oracle
+import systems
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---- Philox4x32-10: Random123 known-answer vectors ---------------------------
+@pytest.mark.parametrize("ctr,key,out", [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_known_answers(ctr, key, out):
+    import ctypes as C
+    L = oracle.lib()
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    L.or_philox(c, k, o)
+    assert list(o) == out
+
+
+# ---- FFT: committed numpy.fft vectors, then live numpy for more shapes -------
+def test_fft_golden_vectors():
+    g = np.load(os.path.join(GOLDEN, "fft_vectors.npz"))
+    for name in ("a", "b"):
+        x = g["in_" + name]
+        assert np.abs(oracle.fft3d(x, -1) - g["fwd_" + name]).max() < 1e-12 * np.abs(g["fwd_" + name]).max()
+        assert np.abs(oracle.fft3d(x, +1) - g["inv_" + name]).max() < 1e-12 * np.abs(g["inv_" + name]).max()
+
+
+@pytest.mark.parametrize("shape", [(5, 6, 7), (16, 9, 10), (3, 25, 8), (2, 3, 49)])
+def test_fft_matches_numpy(shape):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    assert np.allclose(oracle.fft3d(x, -1), np.fft.fftn(x), rtol=0, atol=1e-11 * np.abs(x).sum())
+
+
+# ---- Ewald: NaCl Madelung constant --------------------------------------------
+def _ions(charges, xyz, box, cutoff, beta, K=(16, 16, 16)):
+    n = len(charges)
+    topo = dict(n_atoms=n, n_types=1, charge=np.asarray(charges, float), mass=np.full(n, 10.0),
+                type=np.zeros(n, np.int32), lj_eps=np.zeros(1), lj_sig2=np.ones(1), lj14_eps=np.zeros(1),
+                lj14_sig2=np.ones(1), n_bonds=0, bond_atoms=np.zeros(0, np.int32), bond_k=np.zeros(0),
+                bond_r0=np.zeros(0), n_angles=0, angle_atoms=np.zeros(0, np.int32), angle_k=np.zeros(0),
+                angle_theta0=np.zeros(0), ub_k=np.zeros(0), ub_s0=np.zeros(0), n_dihedrals=0,
+                dihedral_atoms=np.zeros(0, np.int32), dihedral_k=np.zeros(0), dihedral_n=np.zeros(0, np.int32),
+                dihedral_delta=np.zeros(0), n_impropers=0, improper_atoms=np.zeros(0, np.int32),
+                improper_k=np.zeros(0), improper_psi0=np.zeros(0), n_constraints=0,
+                constraint_atoms=np.zeros(0, np.int32), constraint_length=np.zeros(0))
+    params = dict(box=box, cutoff=cutoff, cuton=cutoff - 1.0, pairlist=cutoff + 1.0, beta=beta, K=K)
+    o = oracle.Oracle(topo, params)
+    return o, o.to_fixed(np.asarray(xyz, float))
+
+
+def test_nacl_madelung_constant():
+    r0, m = 2.8, 3                       # nearest-neighbour distance, 3x3x3 conventional cells
+    pts, q = [], []
+    for i in range(2 * m):
+        for j in range(2 * m):
+            for k in range(2 * m):
+                pts.append((i * r0, j * r0, k * r0))
+                q.append(1.0 if (i + j + k) % 2 == 0 else -1.0)
+    L = 2 * m * r0
+    o, u = _ions(q, pts, (L, L, L), cutoff=8.0, beta=0.6)
+    _, e, _ = o.forces(u, oracle.REAL | oracle.EWALD | oracle.SELF)
+    total = e["elec"] + e["pme"] + e["ewself"]
+    madelung = -total / (len(q) / 2) * r0 / 332.0716
+    assert abs(madelung - 1.747564594633182) < 1e-8
+
+
+def test_ewald_total_is_beta_independent():
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0, 18.0, (40, 3))
+    q = np.tile([1.0, -1.0], 20)
+    e = []
+    for beta in (0.45, 0.55):
+        o, u = _ions(q, x, (18.0, 18.0, 18.0), cutoff=8.9, beta=beta)
+        _, ee, _ = o.forces(u, oracle.REAL | oracle.EWALD | oracle.SELF)
+        e.append(ee["elec"] + ee["pme"] + ee["ewself"])
+    assert abs(e[0] - e[1]) < 1e-7 * abs(e[0])
+
+
+def test_pme_converges_to_direct_ewald():
+    topo, xyz, p = systems.water_box(100, 15.0, seed=3)
+    errs = []
+    fe = None
+    for K in (12, 16, 24, 32):
+        params = dict(p, K=(K, K, K), cutoff=7.0, cuton=5.0, pairlist=8.0)
+        o = oracle.Oracle(topo, params)
+        u = o.to_fixed(xyz)
+        if fe is None:
+            fe, ee, _ = o.forces(u, oracle.EWALD)
+        fp, ep, _ = o.forces(u, oracle.PME)
+        errs.append(systems.rel_rms(fp, fe))
+    # order-4 splines: force error falls roughly as h^3 (12 -> 32: ~95x here)
+    assert errs[-1] < 3e-4 and errs[0] / errs[-1] > 30
+    assert all(a > b for a, b in zip(errs, errs[1:]))
+
+
+# ---- forces are the negative gradient of the energy ---------------------------
+def _fd_system():
+    topo, xyz = fz_synth.make_system(3, 60, (22.0, 22.0, 22.0), seed=2)
+    # add Urey-Bradley terms and impropers so that every term type is exercised
+    na = topo["n_angles"]
+    topo["ub_k"] = np.where(np.arange(na) % 3 == 0, 15.0, 0.0)
+    topo["ub_s0"] = np.full(na, 2.4)
+    imp = np.array([[0, 1, 2, 3], [40, 41, 42, 43], [80, 81, 82, 83]], np.int32)
+    topo["improper_atoms"] = imp.ravel()
+    topo["improper_k"] = np.array([20.0, 35.0, 10.0])
+    topo["improper_psi0"] = np.zeros(3)
+    topo["n_impropers"] = 3
+    topo["dihedral_delta"] = np.where(np.arange(topo["n_dihedrals"]) % 2 == 0, 0.0, math.radians(35.0))
+    params = dict(box=(22.0, 22.0, 22.0), cutoff=9.0, cuton=7.0, pairlist=10.0, beta=0.35, K=(24, 24, 24))
+    return topo, xyz, params
+
+
+@pytest.mark.parametrize("flags", [oracle.REAL, oracle.EXCL, oracle.BONDED, oracle.PME, oracle.EWALD])
+def test_force_is_minus_energy_gradient(flags):
+    topo, xyz, params = _fd_system()
+    o = oracle.Oracle(topo, params)
+    u = o.to_fixed(xyz)
+    f, _, _ = o.forces(u, flags)
+    L = params["box"][0]
+    rng = np.random.default_rng(flags)
+    atoms = rng.choice(topo["n_atoms"], 12, replace=False).tolist() + [0, 1, 2, 41]
+    for i in atoms:
+        for d in range(3):
+            du = np.uint64(int(1e-5 / L * 2 ** 64))
+            up, um = u.copy(), u.copy()
+            up[i, d] += du
+            um[i, d] -= du
+            h = int(du) * L / 2 ** 64
+            ep = sum(o.forces(up, flags)[1].values())
+            em = sum(o.forces(um, flags)[1].values())
+            fd = -(ep - em) / (2 * h)
+            assert abs(fd - f[i, d]) < 2e-5 * max(1.0, abs(f[i, d])), (flags, i, d, fd, f[i, d])
+
+
+def test_virial_matches_box_scaling_derivative():
+    """W_aa = -L_a dE/dL_a at fixed fractional coordinates, for every term together."""
+    topo, xyz, params = _fd_system()
+    o = oracle.Oracle(topo, params)
+    u = o.to_fixed(xyz)
+    flags = oracle.REAL | oracle.EXCL | oracle.BONDED | oracle.EWALD
+    _, _, vir = o.forces(u, flags)
+    for a in range(3):
+        e = []
+        for s in (1 + 1e-6, 1 - 1e-6):
+            box = list(params["box"])
+            box[a] *= s
+            ob = oracle.Oracle(topo, dict(params, box=tuple(box)))
+            e.append(sum(ob.forces(u, flags)[1].values()))
+        dEdL = (e[0] - e[1]) / (2e-6 * params["box"][a])
+        assert abs(-params["box"][a] * dEdL - vir[a, a]) < 1e-4 * np.abs(vir).max()
+
+
+def test_pair_forces_obey_newton_third_law():
+    topo, xyz, params = _fd_system()
+    o = oracle.Oracle(topo, params)
+    f, _, _ = o.forces(o.to_fixed(xyz), oracle.REAL | oracle.EXCL | oracle.BONDED)
+    assert np.abs(f.sum(axis=0)).max() < 1e-9 * np.abs(f).max()
+
+
+def test_dihedral_sign_convention():
+    """phi = +60 deg in the IUPAC convention: E = K(1 + cos(n phi - delta)) for
+    delta = 90 deg, n = 1 is K(1 + sin 60)."""
+    topo, xyz, params = _fd_system()
+    th = math.radians(60.0)
+    pos = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 1.5], [math.cos(th), math.sin(th), 1.5]]) + 5.0
+    t = dict(topo)
+    t["n_atoms"] = 4
+    for k in ("charge", "mass", "type"):
+        t[k] = topo[k][:4]
+    for k in ("bonds", "angles", "impropers", "constraints"):
+        t["n_" + k] = 0
+    t["n_dihedrals"] = 1
+    t["dihedral_atoms"] = np.array([0, 1, 2, 3], np.int32)
+    t["dihedral_k"], t["dihedral_n"], t["dihedral_delta"] = np.array([2.0]), np.array([1], np.int32), np.array([math.pi / 2])
+    o = oracle.Oracle(t, params)
+    _, e, _ = o.forces(o.to_fixed(pos), oracle.BONDED)
+    assert abs(e["dihed"] - 2.0 * (1.0 + math.sin(th))) < 1e-12
+
+
+def test_angle_and_bond_closed_forms():
+    topo, xyz, params = _fd_system()
+    t = dict(topo)
+    t["n_atoms"] = 3
+    for k in ("charge", "mass", "type"):
+        t[k] = topo[k][:3]
+    t["n_dihedrals"] = t["n_impropers"] = t["n_constraints"] = 0
+    t["n_bonds"], t["bond_atoms"], t["bond_k"], t["bond_r0"] = 1, np.array([0, 1], np.int32), np.array([300.0]), np.array([1.5])
+    t["n_angles"], t["angle_atoms"] = 1, np.array([0, 1, 2], np.int32)
+    t["angle_k"], t["angle_theta0"] = np.array([50.0]), np.array([math.radians(100.0)])
+    t["ub_k"], t["ub_s0"] = np.zeros(1), np.zeros(1)
+    pos = np.array([[1.7, 0, 0], [0, 0, 0], [0, 2.0, 0]]) + 4.0    # right angle, bond 1.7
+    o = oracle.Oracle(t, params)
+    _, e, _ = o.forces(o.to_fixed(pos), oracle.BONDED)
+    assert abs(e["bond"] - 300.0 * 0.2 ** 2) < 1e-10
+    assert abs(e["angle"] - 50.0 * math.radians(10.0) ** 2) < 1e-10
+
+
+# ---- exclusions, pair sets, cluster pairs ----------------------------------------
+def test_exclusion_sets_follow_bonds_angles_dihedrals():
+    topo, xyz = fz_synth.make_system(2, 5, (30.0, 30.0, 30.0), seed=1)
+    params = dict(box=(30.0,) * 3, cutoff=9.0, cuton=7.0, pairlist=10.0, beta=0.3, K=(24, 24, 24))
+    o = oracle.Oracle(topo, params)
+    ew = {tuple(p) for p in o.pairs(1).tolist()}
+    p14 = {tuple(p) for p in o.pairs(2).tolist()}
+    # chain of 38: 37 bonded + 36 1-3 pairs, 35 1-4 pairs; water: 3 pairs (O-H, O-H, H-H)
+    assert len(ew) == 2 * (37 + 36) + 5 * 3
+    assert len(p14) == 2 * 35
+    assert (0, 3) in p14 and (0, 2) in ew and (0, 1) in ew and (0, 4) not in p14 | ew
+
+
+def test_cluster_pairs_cover_exactly_the_pairs_within_list_distance():
+    topo, xyz, params = systems.water_box(200, 19.0, seed=8)
+    o = oracle.Oracle(topo, params)
+    u = o.to_fixed(xyz)
+    col, slots, pairs, masks = o.cluster_pairs(u, 4, 4)
+    covered = set()
+    for (I, J), m in zip(pairs.tolist(), masks.tolist()):
+        assert I <= J and m != 0
+        for l in range(64):
+            if m >> l & 1:
+                a, b = slots[8 * I + (l & 7)], slots[8 * J + (l >> 3)]
+                key = (min(a, b), max(a, b))
+                assert key not in covered
+                covered.add(key)
+    assert len(covered) == o.count_pairs_within(u, params["pairlist"])
+    # every atom sits in exactly one slot, in the column its x, y fractions name
+    real = slots[slots >= 0]
+    assert sorted(real.tolist()) == list(range(topo["n_atoms"]))
+
+
+# ---- integrator, constraints, velocities ---------------------------------------------
+def test_shake_keeps_rigid_water_geometry():
+    topo, xyz, params = systems.water_box(150, 17.0, seed=4)
+    params = dict(params, cutoff=8.0, cuton=6.0, pairlist=9.0, K=(20, 20, 20), dt_ps=0.001)
+    o = oracle.Oracle(topo, params)
+    u = o.to_fixed(xyz)
+    v = o.maxwell(u, 300.0, 1, 0)
+    u, v, rc = o.step(u, v, 5)
+    assert rc == 0
+    res, vres = o.constraint_residual(u, v)
+    assert res < params["shake_tol"] and vres < 1e-9
+    x = o.from_fixed(u)
+    box = np.array(params["box"])
+    for w in range(0, 30, 3):
+        d1 = x[w + 1] - x[w]; d1 -= box * np.round(d1 / box)
+        d2 = x[w + 2] - x[w + 1]; d2 -= box * np.round(d2 / box)
+        assert abs(np.linalg.norm(d1) - fz_synth.R_OH) < 1e-8
+        assert abs(np.linalg.norm(d2) - fz_synth.R_HH) < 1e-8
+
+
+def test_harmonic_bond_oscillates_at_the_closed_form_frequency():
+    k, r0, m = 100.0, 1.0, 12.0
+    t = dict(n_atoms=2, n_types=1, charge=np.zeros(2), mass=np.full(2, m), type=np.zeros(2, np.int32),
+             lj_eps=np.zeros(1), lj_sig2=np.ones(1), lj14_eps=np.zeros(1), lj14_sig2=np.ones(1),
+             n_bonds=1, bond_atoms=np.array([0, 1], np.int32), bond_k=np.array([k]), bond_r0=np.array([r0]),
+             n_angles=0, angle_atoms=np.zeros(0, np.int32), angle_k=np.zeros(0), angle_theta0=np.zeros(0),
+             ub_k=np.zeros(0), ub_s0=np.zeros(0), n_dihedrals=0, dihedral_atoms=np.zeros(0, np.int32),
+             dihedral_k=np.zeros(0), dihedral_n=np.zeros(0, np.int32), dihedral_delta=np.zeros(0),
+             n_impropers=0, improper_atoms=np.zeros(0, np.int32), improper_k=np.zeros(0), improper_psi0=np.zeros(0),
+             n_constraints=0, constraint_atoms=np.zeros(0, np.int32), constraint_length=np.zeros(0))
+    params = dict(box=(30.0,) * 3, cutoff=9.0, cuton=8.0, pairlist=10.0, beta=0.3, K=(16, 16, 16), dt_ps=0.0005)
+    o = oracle.Oracle(t, params)
+    x = np.array([[0.0, 0.0, 0.0], [r0 + 0.1, 0.0, 0.0]])
+    u = o.to_fixed(x)
+    v = np.zeros((2, 3))
+    mu = m / 2.0
+    omega = math.sqrt(2.0 * k / mu)       # E = k (r - r0)^2
+    nsteps = 400
+    u2, v2, _ = o.step(u, v, nsteps)
+    x2 = o.from_fixed(u2)
+    h = params["dt_ps"] * 20.4548
+    # velocity Verlet on a harmonic oscillator is exactly a rotation with
+    # frequency (2/h) asin(omega h / 2)
+    omega_h = 2.0 / h * math.asin(omega * h / 2.0)
+    assert abs((x2[1, 0] - x2[0, 0] - r0) - 0.1 * math.cos(omega_h * nsteps * h)) < 1e-9
+
+
+def test_maxwell_velocities_statistics():
+    topo, xyz, params = systems.water_box(3000, 31.0, seed=1)
+    o = oracle.Oracle(topo, params)
+    u = o.to_fixed(xyz)
+    v = o.maxwell(u, 300.0, 12, 0)
+    p = (topo["mass"][:, None] * v).sum(axis=0)
+    assert np.abs(p).max() < 1e-9
+    assert abs(o.temperature(v) - 300.0) < 10.0
+    v2 = o.maxwell(u, 300.0, 12, 1)
+    assert not np.allclose(v, v2)
+
+
+# ---- FP32 erfc fit --------------------------------------------------------------------
+def test_erfc_fit_relative_error():
+    from scipy.special import erfc
+    hdr = open(os.path.join(os.path.dirname(GOLDEN), "..", "fen-zi_amd", "csrc", "fz_erfc_fit.h")).read()
+    vals = {}
+    for line in hdr.splitlines():
+        if line.startswith("#define FZ_ERFC_"):
+            _, k, v = line.split()
+            vals[k] = float(v.rstrip("f"))
+    c = [vals["FZ_ERFC_C%d" % i] for i in range(int(vals["FZ_ERFC_DEG"]) + 1)]
+    x = np.linspace(0, 4.0, 20001)
+    t = 1 / (1 + vals["FZ_ERFC_P"] * x)
+    y = np.exp(-x * x) * t * np.polyval(c[::-1], t)
+    assert np.abs(y / erfc(x) - 1).max() < 2e-7
