@@ -3,17 +3,27 @@
 
 Headline line (value): ns/day of the ~17K-atom membrane-like slab (128 chains
 x 38 beads + 4,046 rigid waters, 48x48x72 A, cutoff 12 A, PME 48x48x72,
-dt 2 fs, NVT) in the mixed-precision form.  Beside it: the same system in
-the plain FP64 form, the ~230K-atom slab (1,536 chains + 57,216 waters,
-132^3 A, PME 128^3) in both forms and their ratio, mixed-vs-FP64 force /
-energy / virial errors, per-phase device times, the pair kernel's roofline
-and the FP64 CPU oracle timed on the host.
+dt 2 fs, NVT) in the mixed-precision form.  Beside it, under "extra":
 
-Systems are synthetic (seeded), relaxed with libfenzi's own steepest descent
-before timing (untimed), then given Maxwell velocities at 300 K.  The timed
-region is K steps of fz_step (graph-replayed steps, device-side neighbour
-rebuilds) bracketed by device synchronisation; ns/day = 86400 / t_step[s] *
-dt[fs] * 1e-6.
+* the same system in the plain FP64 form, and the ~230K-atom slab (1,536
+  chains + 57,216 waters, 132^3 A, PME 128^3) in both forms with their ratio,
+  mixed-vs-FP64 force / energy / virial errors, NVE energy drift, per-phase
+  device times;
+* the 2,400-water box (30 A, cutoff 10 A, PME 32^3, dt 1 fs, NVE) in FP64;
+* the batched ensemble: 256 systems of 1,500-24,000 atoms (log-uniform) run as
+  one batch (one launch per phase per step, graph-replayed) against the same
+  systems run one at a time, aggregate ns/day of both;
+* the serving loop: jobs arriving as a Poisson stream at 20 jobs/s, 1,000-60,000
+  atoms, 200 minimiser iterations then 500-5,000 NVT steps, admitted by HBM
+  footprint into a continuously batched server; p50/p99 completion latency and
+  aggregate ns/day;
+* the pair kernel's roofline and the FP64 CPU oracle timed on the host.
+
+Systems are synthetic (seeded) and relaxed with libfenzi's own steepest
+descent before timing (untimed), then given Maxwell velocities at 300 K.  Timed
+regions are fz_step / fz_batch_step calls (graph-replayed steps, device-side
+neighbour rebuilds) bracketed by device synchronisation; ns/day = 86400 /
+t_step[s] * dt[fs] * 1e-6.
 """
 import argparse
 import json
@@ -30,10 +40,10 @@ import fenzi  # noqa: E402
 import fz_synth  # noqa: E402
 
 METRIC = "ns/day at 17K & 230K atoms; batched agg ns/day + p99 latency, mixed vs FP64"
-FP32_PEAK = 157.3e12   # MI355X FP32 vector, published
-FP64_PEAK = 78.6e12    # MI355X FP64 vector, published
-HBM_PEAK = 8.0e12
+FP32_PEAK = 157.3e12   # MI355X FP32, published
+FP64_PEAK = 78.6e12    # MI355X FP64, published
 PAIR_FLOP = 66         # algorithmic flop per non-excluded half-pair inside the cutoff
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pair_mixed_slab17k_traffic.json")
 
 
 def log(*a):
@@ -57,12 +67,29 @@ def prepare(ctx, cid, min_steps):
     return topo, x, p
 
 
+def run_steps(s, n, done, reassign_every, T, seed):
+    """n NVT steps: velocities redrawn whenever the step count reaches a multiple
+    of reassign_every (0 = never)."""
+    while n > 0:
+        c = n
+        if reassign_every:
+            c = min(c, reassign_every - done % reassign_every)
+        s.step(c)
+        n -= c
+        done += c
+        if reassign_every and done % reassign_every == 0:
+            s.reassign_velocities(T, seed, done // reassign_every)
+    return done
+
+
 def md(ctx, topo, x, p, prec, steps, warmup, seed=1):
     s = fenzi.System(ctx, topo, p, prec, xyz=x)
-    s.reassign_velocities(p["temperature"], seed, 0)
-    s.step(warmup)
+    T, every = p["temperature"], int(p.get("reassign_every", 0))
+    s.reassign_velocities(T, seed, 0)
+    done = run_steps(s, warmup, 0, every, T, seed)
+    # fz_step returns once the device has finished (it reads the status flags)
     t0 = time.perf_counter()
-    s.step(steps)
+    run_steps(s, steps, done, every, T, seed)
     dt = time.perf_counter() - t0
     ms = dt * 1e3 / steps
     e = s.energies()
@@ -70,8 +97,21 @@ def md(ctx, topo, x, p, prec, steps, warmup, seed=1):
     out = dict(ms_per_step=ms, ns_per_day=ns_day(ms, p["dt_ps"]), temperature=e["temperature"],
                pairs_within_cutoff=int(e["pairs_within_cutoff"]), list_builds=int(e["list_builds"]), phases_ms=ph,
                device_bytes=s.info()["device_bytes"])
-    s.close()
-    return out
+    return out, s
+
+
+def nve_drift(s, n_atoms, dt_ps, segments=10, seg_steps=50):
+    """Total-energy slope of an NVE segment, kcal/mol per ns per atom (least squares)."""
+    t, e = [], []
+    for k in range(segments + 1):
+        if k:
+            s.step(seg_steps)
+        en = s.energies()
+        t.append(k * seg_steps * dt_ps * 1e-3)
+        e.append(en["potential"] + en["kinetic"])
+    slope = np.polyfit(np.array(t), np.array(e), 1)[0]
+    return {"kcal_per_mol_ns_atom": float(slope / n_atoms), "steps": segments * seg_steps,
+            "e_total_first": e[0], "e_total_last": e[-1]}
 
 
 def compare_forms(ctx, topo, x, p):
@@ -92,14 +132,31 @@ def compare_forms(ctx, topo, x, p):
     return res
 
 
+def traffic_per_launch():
+    """HBM bytes per pair-kernel launch from the committed FETCH_SIZE / WRITE_SIZE
+    counter runs (scaled by the copy-kernel calibration recorded beside them)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def roofline(run, prec):
     t = run["phases_ms"]["pair"] * 1e-3
     flops = PAIR_FLOP * run["pairs_within_cutoff"]
     ach = flops / t
     peak = FP32_PEAK if prec == fenzi.MIXED else FP64_PEAK
-    return {"kernel": "pair (LJ + real-space Ewald, cluster-pair tiles)", "bound": "valu",
-            "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
-            "flop_per_launch": flops, "launch_ms": run["phases_ms"]["pair"]}
+    tr = traffic_per_launch()
+    return {"kernel": "k_pair_mixed (LJ + real-space Ewald over cluster-pair tiles)", "bound": "mfma",
+            "note": "compute-bound on the FP32 vector ALU (no dense contraction to put on the matrix cores); "
+                    "MI355X's FP32 peak is 157.3 TFLOP/s for both",
+            "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s", "frac": ach / peak,
+            "traffic": None if tr is None else tr["bytes_per_launch"],
+            "traffic_source": None if tr is None else tr["source"],
+            "flop_per_launch": flops, "pairs_per_launch": run["pairs_within_cutoff"],
+            "launch_ms": run["phases_ms"]["pair"],
+            "timing": "mean of 10 eagerly launched steps, HIP events on the launch stream around each phase"}
 
 
 def cpu_baseline(topo, x, p, budget_s=20.0):
@@ -122,6 +179,91 @@ def cpu_baseline(topo, x, p, budget_s=20.0):
                       "order-4 PME with own FFT, bonded, SHAKE/RATTLE); %.1f ms/step" % (n, topo["n_atoms"], ms)}
 
 
+# ---- batched ensemble ----------------------------------------------------------
+
+def batched(ctx, members, steps, warmup, alone_steps):
+    t0 = time.perf_counter()
+    systems = [fenzi.System(ctx, t, p, fenzi.MIXED, xyz=x) for t, x, p in members]
+    b = fenzi.Batch(ctx, systems)
+    b.minimize(100, 0.05)
+    for k, s in enumerate(systems):
+        s.reassign_velocities(300.0, 1000 + k, 0)
+    log("batch: %d systems, %d atoms set up + relaxed in %.1fs" % (len(systems), sum(s.n for s in systems),
+                                                                   time.perf_counter() - t0))
+    tune = b.autotune(6)
+    log("batch autotune: %s" % tune)
+    b.step(warmup)
+    t0 = time.perf_counter()
+    b.step(steps)
+    t_batch = (time.perf_counter() - t0) / steps
+    dt_ps = members[0][2]["dt_ps"]
+    agg_batch = ns_day(t_batch * 1e3, dt_ps) * len(systems)
+    log("batch: %.3f ms per batch step, aggregate %.1f ns/day" % (t_batch * 1e3, agg_batch))
+    # one at a time: the same systems, each through its own engine (batch of one)
+    t_alone = 0.0
+    for s in systems:
+        s.step(3)
+        t0 = time.perf_counter()
+        s.step(alone_steps)
+        t_alone += (time.perf_counter() - t0) / alone_steps
+    agg_alone = ns_day(t_alone * 1e3, dt_ps) * len(systems)
+    log("one at a time: %.3f ms per round of single steps, aggregate %.1f ns/day" % (t_alone * 1e3, agg_alone))
+    e = [s.energies()["temperature"] for s in systems]
+    b.close()
+    for s in systems:
+        s.close()
+    return {"systems": len(members), "atoms": int(sum(m[0]["n_atoms"] for m in members)),
+            "steps": steps, "ms_per_batch_step": t_batch * 1e3, "aggregate_ns_per_day": agg_batch,
+            "one_at_a_time_ms_per_round": t_alone * 1e3, "one_at_a_time_aggregate_ns_per_day": agg_alone,
+            "one_at_a_time_steps_per_system": alone_steps, "speedup": agg_batch / agg_alone,
+            "autotune": tune, "temperature_mean": float(np.mean(e)),
+            "workload": "256 systems, N log-uniform in [1500, 24000] (seed 3), chain fraction U[0, 0.3], "
+                        "cubic boxes at ~1 g/cm^3, cutoff 10 A, PME K in [16, 48], dt 2 fs, NVE, mixed"}
+
+
+# ---- serving ---------------------------------------------------------------------
+
+def serving(ctx, jobs, arrival, md_steps, budget, chunk):
+    srv = fenzi.Server(ctx, budget, max_active=100000, precision=fenzi.MIXED, chunk_steps=chunk)
+    t_start = time.perf_counter()
+    ids = []
+    nxt = 0
+    rounds = 0
+    peak_active = 0
+    while True:
+        now = time.perf_counter() - t_start
+        while nxt < len(jobs) and arrival[nxt] <= now:
+            t, x, p = jobs[nxt]
+            ids.append(srv.submit(t, p, x, 200, int(md_steps[nxt]), 300.0, 500, 5000 + nxt))
+            nxt += 1
+        a, q, d = srv.poll()
+        rounds += 1
+        peak_active = max(peak_active, a)
+        if nxt == len(jobs) and a == 0 and q == 0:
+            break
+        if a == 0 and q == 0:
+            time.sleep(max(0.0, min(0.01, arrival[nxt] - (time.perf_counter() - t_start))))
+    wall = time.perf_counter() - t_start
+    lat, sim_ns, failed = [], 0.0, 0
+    for k, jid in enumerate(ids):
+        info = srv.info(jid)
+        if info["state"] != fenzi.JOB_DONE:
+            failed += 1
+            continue
+        lat.append(info["t_done"] - arrival[k])
+        sim_ns += info["steps_done"] * jobs[k][2]["dt_ps"] * 1e-3
+    srv.close()
+    lat = np.array(lat)
+    return {"jobs": len(jobs), "failed": failed, "wall_s": wall, "rounds": rounds, "peak_active": peak_active,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "latency_max_s": float(lat.max()),
+            "aggregate_ns_per_day": sim_ns / wall * 86400.0,
+            "atoms_mean": float(np.mean([j[0]["n_atoms"] for j in jobs])),
+            "workload": "Poisson arrivals at 20 jobs/s (seed 4), N log-uniform in [1000, 60000], 200 steepest-"
+                        "descent iterations then U{500..5000} NVT steps at 300 K (reassignment every 500), "
+                        "mixed, admission by estimated footprint within %.0f GB" % (budget / 1e9)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,17 +271,37 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-large", action="store_true", help="skip the 230K-atom system")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-water", action="store_true", help="skip the 2,400-water FP64 box")
+    ap.add_argument("--batch-systems", type=int, default=256, help="0 skips the batched ensemble")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--serve-jobs", type=int, default=100, help="0 skips the serving loop")
+    ap.add_argument("--serve-chunk", type=int, default=100)
     ap.add_argument("--min-steps", type=int, default=300)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs one process on one MI355X; "
               "multi-GPU runs are not part of it (--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    # ensemble and serving inputs are generated in forked workers before the GPU is opened
+    members, jobs = [], []
+    if a.batch_systems:
+        t0 = time.perf_counter()
+        n, f = fz_synth.batch_sizes(a.batch_systems)
+        members = fz_synth.ensemble_members([(n[k], f[k], 1000 + k, 10.0) for k in range(a.batch_systems)])
+        log("generated %d ensemble members in %.1fs" % (len(members), time.perf_counter() - t0))
+    if a.serve_jobs:
+        t0 = time.perf_counter()
+        arrival, n, f, md_steps = fz_synth.serving_jobs(a.serve_jobs)
+        jobs = fz_synth.ensemble_members([(n[k], f[k], 7000 + k, 10.0) for k in range(a.serve_jobs)])
+        log("generated %d serving jobs in %.1fs" % (len(jobs), time.perf_counter() - t0))
+
     ctx = fenzi.Context(0)
     topo1, x1, p1 = prepare(ctx, 1, a.min_steps)
-    r1m = md(ctx, topo1, x1, p1, fenzi.MIXED, a.steps, a.warmup)
+    r1m, s = md(ctx, topo1, x1, p1, fenzi.MIXED, a.steps, a.warmup)
+    s.close()
     log("17K mixed: %.3f ms/step %.1f ns/day" % (r1m["ms_per_step"], r1m["ns_per_day"]))
-    r1d = md(ctx, topo1, x1, p1, fenzi.FP64, a.steps, a.warmup)
+    r1d, s = md(ctx, topo1, x1, p1, fenzi.FP64, a.steps, a.warmup)
+    s.close()
     log("17K fp64 : %.3f ms/step %.1f ns/day" % (r1d["ms_per_step"], r1d["ns_per_day"]))
     err1 = compare_forms(ctx, topo1, x1, p1)
     extra = {"slab17k_mixed": r1m, "slab17k_fp64": r1d, "slab17k_mixed_vs_fp64": err1,
@@ -147,13 +309,32 @@ def main():
     if not a.no_large:
         topo2, x2, p2 = prepare(ctx, 2, a.min_steps)
         s2 = max(50, a.steps // 3)
-        r2m = md(ctx, topo2, x2, p2, fenzi.MIXED, s2, max(10, a.warmup // 2))
-        log("230K mixed: %.3f ms/step %.1f ns/day" % (r2m["ms_per_step"], r2m["ns_per_day"]))
-        r2d = md(ctx, topo2, x2, p2, fenzi.FP64, s2, max(10, a.warmup // 2))
+        r2m, s = md(ctx, topo2, x2, p2, fenzi.MIXED, s2, max(10, a.warmup // 2))
+        r2m["nve_drift"] = nve_drift(s, topo2["n_atoms"], p2["dt_ps"])
+        s.close()
+        log("230K mixed: %.3f ms/step %.1f ns/day, drift %s" % (r2m["ms_per_step"], r2m["ns_per_day"],
+                                                               r2m["nve_drift"]["kcal_per_mol_ns_atom"]))
+        r2d, s = md(ctx, topo2, x2, p2, fenzi.FP64, s2, max(10, a.warmup // 2))
+        r2d["nve_drift"] = nve_drift(s, topo2["n_atoms"], p2["dt_ps"])
+        s.close()
         log("230K fp64 : %.3f ms/step %.1f ns/day" % (r2d["ms_per_step"], r2d["ns_per_day"]))
         extra.update({"slab230k_mixed": r2m, "slab230k_fp64": r2d,
                       "slab230k_mixed_vs_fp64": compare_forms(ctx, topo2, x2, p2),
                       "slab230k_mixed_over_fp64": r2d["ms_per_step"] / r2m["ms_per_step"]})
+    if not a.no_water:
+        topo0, x0, p0 = prepare(ctx, 0, 100)
+        r0, s = md(ctx, topo0, x0, p0, fenzi.FP64, 1000, 50)
+        r0["nve_drift"] = nve_drift(s, topo0["n_atoms"], p0["dt_ps"], 10, 100)
+        s.close()
+        log("2400 waters fp64: %.3f ms/step %.1f ns/day" % (r0["ms_per_step"], r0["ns_per_day"]))
+        extra["water2400_fp64"] = r0
+    if members:
+        extra["batch256_mixed"] = batched(ctx, members, a.batch_steps, 20, 30)
+    if jobs:
+        extra["serving_mixed"] = serving(ctx, jobs, arrival, md_steps, 200e9, a.serve_chunk)
+        sv = extra["serving_mixed"]
+        log("serving: %d jobs, p50 %.2fs p99 %.2fs, aggregate %.1f ns/day" % (
+            sv["jobs"], sv["latency_p50_s"], sv["latency_p99_s"], sv["aggregate_ns_per_day"]))
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(topo1, x1, p1)

@@ -445,3 +445,31 @@ def batch_sizes(n_systems=256, lo=1500, hi=24000, seed=3):
     n = np.exp(rng.uniform(math.log(lo), math.log(hi), n_systems)).astype(int)
     frac = rng.uniform(0.0, 0.3, n_systems)
     return n, frac
+
+
+def serving_jobs(n_jobs=100, rate=20.0, lo=1000, hi=60000, seed=4):
+    """Job stream of the serving workload: Poisson arrivals at `rate` jobs/s,
+    N ~ LogUniform[lo, hi], chain fraction ~ U[0, 0.3], NVT length U{500..5000}."""
+    rng = np.random.default_rng(seed)
+    arrival = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    n = np.exp(rng.uniform(math.log(lo), math.log(hi), n_jobs)).astype(int)
+    frac = rng.uniform(0.0, 0.3, n_jobs)
+    md_steps = rng.integers(500, 5001, n_jobs)
+    return arrival, n, frac, md_steps
+
+
+def _member_job(spec):
+    n, frac, seed, cutoff = spec
+    return ensemble_member(int(n), float(frac), int(seed), cutoff)
+
+
+def ensemble_members(specs, workers=None):
+    """ensemble_member over (n_target, chain_frac, seed, cutoff) specs, in worker
+    processes (forked: call this before the process opens a GPU context)."""
+    import multiprocessing as mp
+    import os
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(specs) < 4:
+        return [_member_job(s) for s in specs]
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_member_job, specs, chunksize=1)
