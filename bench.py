@@ -104,8 +104,7 @@ def nve_drift(s, n_atoms, dt_ps, segments=10, seg_steps=50):
     """Total-energy slope of an NVE segment, kcal/mol per ns per atom (least squares)."""
     t, e = [], []
     for k in range(segments + 1):
-        if k:
-            s.step(seg_steps)
+        s.step(seg_steps if k else 1)  # energies are evaluated on the last step of each call
         en = s.energies()
         t.append(k * seg_steps * dt_ps * 1e-3)
         e.append(en["potential"] + en["kinetic"])

@@ -728,6 +728,7 @@ int Engine::steps(int n) {
     return FZ_OK;
   };
   for (int attempt = 0; attempt < 4; attempt++) {
+    launch_clear_shake_flag(L_);  // the flag reports this attempt's steps only
     // snapshot positions, velocities and the positions of the last list build
     // for replay after a capacity overflow
     for (auto* s : sys_) {
@@ -750,10 +751,10 @@ int Engine::steps(int n) {
     bool again = false;
     for (auto* s : sys_) again = again || s->flags_host[F_OVERFLOW];
     if (!again) {
-      for (auto* s : sys_) {
-        if (s->flags_host[F_SHAKE_FAIL]) return set_error(FZ_ERR_SHAKE, "SHAKE did not converge");
-        s->step += n;
-      }
+      for (size_t k = 0; k < sys_.size(); k++)
+        if (sys_[k]->flags_host[F_SHAKE_FAIL])
+          return set_error(FZ_ERR_SHAKE, "SHAKE did not converge (system %d of the set)", (int)k);
+      for (auto* s : sys_) s->step += n;
       return FZ_OK;
     }
     // Every member returns to the snapshot.  Members that overflowed grow and

@@ -430,6 +430,13 @@ void launch_kinetic(const Launch& L) {
 void launch_zero_acc(const Launch& L) {
   hipLaunchKernelGGL(k_zero_acc, dim3(L.nsys), dim3(64), 0, L.stream, L.d_sys, L.nsys);
 }
+
+__global__ void k_clear_shake_flag(const SysDev* __restrict__ S, int nsys) {
+  if (threadIdx.x == 0) S[blockIdx.x].flags[F_SHAKE_FAIL] = 0;
+}
+void launch_clear_shake_flag(const Launch& L) {
+  hipLaunchKernelGGL(k_clear_shake_flag, dim3(L.nsys), dim3(64), 0, L.stream, L.d_sys, L.nsys);
+}
 void launch_zero_forces(const Launch& L) {
   hipLaunchKernelGGL(k_zero_forces, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
 }

@@ -46,6 +46,7 @@ void launch_rattle(const Launch& L);
 void launch_kinetic(const Launch& L);
 void launch_maxwell(const Launch& L, double T, unsigned long long seed, unsigned long long counter);
 void launch_zero_acc(const Launch& L);
+void launch_clear_shake_flag(const Launch& L);
 void launch_zero_forces(const Launch& L);
 
 }  // namespace fz

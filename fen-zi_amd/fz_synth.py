@@ -226,6 +226,7 @@ def make_system(n_chains, n_waters, box, seed=0, jitter=0.3, water_grid=None,
     if n_chains:
         # slab thickness from the natural volumes of a CH2 bead (27 A^3) and a water (30 A^3)
         h = 0.5 * Lz * (27.0 * n_beads) / (27.0 * n_beads + 30.0 * n_waters)
+        h = min(max(h, 5.0), 0.45 * Lz)  # room for a few chains in a mostly-water box
         per = [n_chains // 2 + n_chains % 2, n_chains // 2] if leaflets == 2 else [n_chains]
         for li, m in enumerate(per):
             if m == 0:
@@ -250,11 +251,24 @@ def make_system(n_chains, n_waters, box, seed=0, jitter=0.3, water_grid=None,
             ix, iy, iz = idx % nside, (idx // nside) % nside, idx // (nside * nside)
             sites = np.stack([ix, iy, iz], axis=1) * spacing + spacing / 2.0 - box / 2.0
         elif n_chains:
-            # water above and below the slab: a lattice over the wrapped range [h, Lz - h)
-            pts = _lattice_sites(box, h + 1.0, Lz - h - 1.0, n_waters)
+            # water above and below the slab: a lattice over the wrapped range [h, Lz - h),
+            # without the sites that fall within 2.8 A of a bead
+            from scipy.spatial import cKDTree
+            beads = np.mod(xyz_parts[0] + box / 2, box)
+            beads = np.minimum(beads, np.nextafter(box, 0))
+            tree = cKDTree(beads, boxsize=box)
+            want = n_waters
+            while True:
+                pts = _lattice_sites(box, h + 1.0, Lz - h - 1.0, want)
+                pts[:, 2] = (pts[:, 2] + Lz / 2) % Lz - Lz / 2
+                w = np.minimum(np.mod(pts + box / 2, box), np.nextafter(box, 0))
+                d, _ = tree.query(w)
+                pts = pts[d > 2.8]
+                if len(pts) >= n_waters:
+                    break
+                want = int(want * 1.1) + 8
             sel = rng.permutation(len(pts))[:n_waters]
             sites = pts[np.sort(sel)]
-            sites[:, 2] = (sites[:, 2] + Lz / 2) % Lz - Lz / 2
         else:
             pts = _lattice_sites(box, -Lz / 2, Lz / 2, n_waters)
             sel = rng.permutation(len(pts))[:n_waters]
