@@ -263,6 +263,37 @@ def serving(ctx, jobs, arrival, md_steps, budget, chunk):
                         "mixed, admission by estimated footprint within %.0f GB" % (budget / 1e9)}
 
 
+def profile_run(a):
+    """17K mixed leg (graph-replayed steps, then the eager phase timing) followed by
+    one device-to-device copy of a known size, so that the same counter run gives
+    the counter's reading for a plain copy."""
+    ctx = fenzi.Context(0)
+    topo1, x1, p1 = prepare(ctx, 1, a.min_steps)
+    r1m, s = md(ctx, topo1, x1, p1, fenzi.MIXED, a.steps, a.warmup)
+    s.close()
+    log("17K mixed: %.3f ms/step; pair %.4f ms per launch, %d pairs" % (r1m["ms_per_step"], r1m["phases_ms"]["pair"],
+                                                                         r1m["pairs_within_cutoff"]))
+    # through the HIP runtime libfenzi itself links (one runtime per process)
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    n = a.copy_mb << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(dst, src, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed")
+    log("copy calibration: %d bytes read, %d bytes written" % (n, n))
+    print(json.dumps({"profile": "slab17k_mixed", "ms_per_step": r1m["ms_per_step"], "phases_ms": r1m["phases_ms"],
+                      "pairs_within_cutoff": r1m["pairs_within_cutoff"], "copy_bytes": n}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,11 +307,18 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=100, help="0 skips the serving loop")
     ap.add_argument("--serve-chunk", type=int, default=100)
     ap.add_argument("--min-steps", type=int, default=300)
+    ap.add_argument("--profile", action="store_true",
+                    help="profiler run: only the 17K mixed leg, then one device copy of --copy-mb MiB "
+                         "(the byte-counter calibration)")
+    ap.add_argument("--copy-mb", type=int, default=512)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs one process on one MI355X; "
               "multi-GPU runs are not part of it (--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.profile:
+        profile_run(a)
+        return
     # ensemble and serving inputs are generated in forked workers before the GPU is opened
     members, jobs = [], []
     if a.batch_systems:
