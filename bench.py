@@ -264,15 +264,16 @@ def serving(ctx, jobs, arrival, md_steps, budget, chunk):
 
 
 def profile_run(a):
-    """17K mixed leg (graph-replayed steps, then the eager phase timing) followed by
-    one device-to-device copy of a known size, so that the same counter run gives
-    the counter's reading for a plain copy."""
+    """Mixed leg of one slab (--profile-workload 1: 17K, 2: 230K; graph-replayed
+    steps, then the eager phase timing) followed by one device-to-device copy of
+    a known size, so that the same counter run gives the counter's reading for a
+    plain copy."""
     ctx = fenzi.Context(0)
-    topo1, x1, p1 = prepare(ctx, 1, a.min_steps)
+    topo1, x1, p1 = prepare(ctx, a.profile_workload, a.min_steps)
     r1m, s = md(ctx, topo1, x1, p1, fenzi.MIXED, a.steps, a.warmup)
     s.close()
-    log("17K mixed: %.3f ms/step; pair %.4f ms per launch, %d pairs" % (r1m["ms_per_step"], r1m["phases_ms"]["pair"],
-                                                                         r1m["pairs_within_cutoff"]))
+    log("%d atoms mixed: %.3f ms/step; pair %.4f ms per launch, %d pairs" % (
+        topo1["n_atoms"], r1m["ms_per_step"], r1m["phases_ms"]["pair"], r1m["pairs_within_cutoff"]))
     # through the HIP runtime libfenzi itself links (one runtime per process)
     import ctypes as C
     hip = C.CDLL("libamdhip64.so.7")
@@ -290,7 +291,7 @@ def profile_run(a):
     if rc != 0:
         raise RuntimeError("hipMemcpy failed")
     log("copy calibration: %d bytes read, %d bytes written" % (n, n))
-    print(json.dumps({"profile": "slab17k_mixed", "ms_per_step": r1m["ms_per_step"], "phases_ms": r1m["phases_ms"],
+    print(json.dumps({"profile": "slab%d_mixed" % topo1["n_atoms"], "ms_per_step": r1m["ms_per_step"], "phases_ms": r1m["phases_ms"],
                       "pairs_within_cutoff": r1m["pairs_within_cutoff"], "copy_bytes": n}), flush=True)
 
 
@@ -311,6 +312,7 @@ def main():
                     help="profiler run: only the 17K mixed leg, then one device copy of --copy-mb MiB "
                          "(the byte-counter calibration)")
     ap.add_argument("--copy-mb", type=int, default=512)
+    ap.add_argument("--profile-workload", type=int, default=1, choices=(1, 2))
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build runs one process on one MI355X; "

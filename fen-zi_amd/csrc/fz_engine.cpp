@@ -325,6 +325,25 @@ int System::alloc_device() {
     FZ_TRY(dalloc(&d.lj32, T2)); FZ_TRY(up(d.lj32, l32));
     FZ_TRY(dalloc(&d.lj64, T2)); FZ_TRY(up(d.lj64, l64));
     FZ_TRY(dalloc(&d.lj14, T2)); FZ_TRY(up(d.lj14, l14));
+    // combination-rule test (pairs with zero well depth carry no sigma information)
+    const int T = h.ntypes;
+    bool sep = true;
+    for (int i = 0; i < T && sep; i++)
+      for (int j = 0; j < T && sep; j++) {
+        const double eii = h.lj_eps[i * T + i], ejj = h.lj_eps[j * T + j], eij = h.lj_eps[i * T + j];
+        if (!(eii >= 0.0 && ejj >= 0.0)) { sep = false; break; }
+        const double ec = std::sqrt(eii * ejj);
+        if (std::fabs(eij - ec) > 1e-12 * std::max(1.0, std::fabs(ec))) sep = false;
+        if (eij == 0.0) continue;
+        const double sc = 0.5 * (std::sqrt(h.lj_sig2[i * T + i]) + std::sqrt(h.lj_sig2[j * T + j]));
+        if (std::fabs(std::sqrt(h.lj_sig2[i * T + j]) - sc) > 1e-12 * std::max(1.0, sc)) sep = false;
+      }
+    std::vector<float2> ah(T);
+    for (int t = 0; t < T; t++)
+      ah[t] = make_float2((float)std::sqrt(std::max(0.0, h.lj_eps[t * T + t])),
+                          (float)(0.5 * std::sqrt(std::max(0.0, h.lj_sig2[t * T + t]))));
+    d.lj_sep = sep ? 1 : 0;
+    FZ_TRY(dalloc(&d.lj_ah, T)); FZ_TRY(up(d.lj_ah, ah));
   }
   // constraints
   FZ_TRY(dalloc(&d.ccl_atoms, h.ccl_atoms.size())); FZ_TRY(up(d.ccl_atoms, h.ccl_atoms));
@@ -415,10 +434,12 @@ int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   FZ_TRY(dalloc(&d.slot_atom, ns));
   FZ_TRY(dalloc((char**)&d.xs, (size_t)ns * (mixed ? 16 : 32)));
   FZ_TRY(dalloc(&d.ts, ns));
+  FZ_TRY(dalloc(&d.lp, ns));
   FZ_TRY(dalloc((unsigned long long**)&d.Fs, (size_t)3 * ns));
   FZ_TRY(dalloc(&d.bin_tmp, ns));
   FZ_TRY(dalloc(&d.clu_col, clusters_cap));
   FZ_TRY(dalloc(&d.bb, (size_t)6 * clusters_cap));
+  FZ_TRY(dalloc(&d.su, (size_t)3 * ns));
   FZ_TRY(dalloc(&d.search_cnt, clusters_cap + 1));
   FZ_TRY(dalloc(&d.search_tmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));
   FZ_TRY(dalloc(&d.search_mtmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));

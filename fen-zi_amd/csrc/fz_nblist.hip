@@ -53,14 +53,36 @@ __device__ __forceinline__ int column_of(const SysDev& d, int i) {
   return cy * d.ncx + cx;
 }
 
+// Column histograms are formed in LDS per block of 256 atoms (consecutive atoms
+// are mostly neighbours, so a block touches few columns) and added to the
+// global counts once per touched column: the global atomics no longer queue
+// up by the hundred on each column's counter.  Systems with more columns than
+// the LDS table holds use global atomics directly.
+constexpr int BIN_LDS = 4096;
+
 __global__ __launch_bounds__(256) void k_bin_count(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
+  __shared__ int hist[BIN_LDS];
+  const bool lds = d.ncol <= BIN_LDS;  // uniform over the block
   const int i = lb * 256 + threadIdx.x;
-  if (i >= d.n_atoms) return;
-  int c = column_of(d, i);
-  d.atom_col[i] = c;
-  atomicAdd(d.col_count + c, 1);
+  if (lds) {
+    for (int k = threadIdx.x; k < d.ncol; k += 256) hist[k] = 0;
+    __syncthreads();
+  }
+  if (i < d.n_atoms) {
+    const int c = column_of(d, i);
+    d.atom_col[i] = c;
+    if (lds) atomicAdd(hist + c, 1);
+    else atomicAdd(d.col_count + c, 1);
+  }
+  if (lds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < d.ncol; k += 256) {
+      const int v = hist[k];
+      if (v) atomicAdd(d.col_count + k, v);
+    }
+  }
 }
 
 // one block per system: clusters per column, exclusive scan, capacity checks
@@ -115,14 +137,33 @@ __global__ __launch_bounds__(1024) void k_col_scan(const SysDev* __restrict__ S,
   }
 }
 
+// places each atom in its column's range of bin_tmp (order inside a column is
+// arbitrary here; k_col_sort fixes it)
 __global__ __launch_bounds__(256) void k_bin_scatter(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
+  __shared__ int hist[BIN_LDS];
+  const bool lds = d.ncol <= BIN_LDS;
   const int i = lb * 256 + threadIdx.x;
-  if (i >= d.n_atoms) return;
-  int c = d.atom_col[i];
-  int p = atomicAdd(d.col_fill + c, 1);
-  d.bin_tmp[d.col_clu0[c] * CLUSTER + p] = i;
+  if (lds) {
+    for (int k = threadIdx.x; k < d.ncol; k += 256) hist[k] = 0;
+    __syncthreads();
+  }
+  int c = -1, r = 0;
+  if (i < d.n_atoms) {
+    c = d.atom_col[i];
+    r = lds ? atomicAdd(hist + c, 1) : atomicAdd(d.col_fill + c, 1);
+  }
+  if (lds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < d.ncol; k += 256) {
+      const int v = hist[k];
+      if (v) hist[k] = atomicAdd(d.col_fill + k, v);  // this block's base in column k
+    }
+    __syncthreads();
+    if (c >= 0) r += hist[c];
+  }
+  if (c >= 0) d.bin_tmp[d.col_clu0[c] * CLUSTER + r] = i;
 }
 
 // one block per column: sort by (u_z, id), lay out slots, snapshot positions,
@@ -168,15 +209,23 @@ __global__ __launch_bounds__(256) void k_col_sort(const SysDev* __restrict__ S, 
       __syncthreads();
     }
   }
+  const int nsc = d.n_slots_cap;
   for (int k = threadIdx.x; k < nclu * CLUSTER; k += 256) {
     int a = k < n ? ids[k] : -1;
     d.slot_atom[base + k] = a;
+    unsigned long long x = 0, y = 0, z = 0;
     if (a >= 0) {
       d.atom_slot[a] = base + k;
-      d.rx[a] = d.ux[a];
-      d.ry[a] = d.uy[a];
-      d.rz[a] = d.uz[a];
+      x = d.ux[a];
+      y = d.uy[a];
+      z = key[k];
+      d.rx[a] = x;
+      d.ry[a] = y;
+      d.rz[a] = z;
     }
+    d.su[base + k] = x;
+    d.su[nsc + base + k] = y;
+    d.su[2 * nsc + base + k] = z;
   }
   for (int m = threadIdx.x; m < nclu; m += 256) {
     unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0, 0, 0};
@@ -249,15 +298,27 @@ __device__ __forceinline__ bool is_excluded(const SysDev& d, int a, int b) {
 // squared distance of two atoms from their fixed-point positions, rounded
 // exactly as the oracle does (no contraction)
 __device__ __forceinline__ double r2_exact(const SysDev& d, unsigned long long ax, unsigned long long ay,
-                                           unsigned long long az, int b) {
+                                           unsigned long long az, unsigned long long bx, unsigned long long by,
+                                           unsigned long long bz) {
 #pragma clang fp contract(off)
-  const double dx = (double)(long long)(d.ux[b] - ax) * d.sc[0];
-  const double dy = (double)(long long)(d.uy[b] - ay) * d.sc[1];
-  const double dz = (double)(long long)(d.uz[b] - az) * d.sc[2];
+  const double dx = (double)(long long)(bx - ax) * d.sc[0];
+  const double dy = (double)(long long)(by - ay) * d.sc[1];
+  const double dz = (double)(long long)(bz - az) * d.sc[2];
   double r2 = dx * dx + dy * dy;
   r2 = r2 + dz * dz;
   return r2;
 }
+
+// exclusion test with atom a's 64-bit window word already loaded
+__device__ __forceinline__ bool excluded_w(const SysDev& d, unsigned long long wa, int a, int b) {
+  if (!d.use_excl_list) {
+    const int off = b - a;
+    return off >= -32 && off <= 31 && ((wa >> (off + 32)) & 1ull);
+  }
+  return is_excluded(d, a, b);
+}
+
+constexpr int SEARCH_BATCH = 8;  // candidate J clusters whose atoms are loaded together
 
 // One wave per I cluster.  Candidates J >= I whose bounding-box gap is below
 // the pair-list distance are tested atom by atom: lane l (i = l & 7, j = l >> 3)
@@ -272,9 +333,11 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
   const int il = lane & 7, jl = lane >> 3;
   const int I = lb * 4 + (threadIdx.x >> 6);
   if (I >= *d.n_clusters) return;
+  const int nsc = d.n_slots_cap;
   const int ai = d.slot_atom[I * CLUSTER + il];
-  unsigned long long aix = 0, aiy = 0, aiz = 0;
-  if (ai >= 0) { aix = d.ux[ai]; aiy = d.uy[ai]; aiz = d.uz[ai]; }
+  const unsigned long long aix = d.su[I * CLUSTER + il], aiy = d.su[nsc + I * CLUSTER + il],
+                           aiz = d.su[2 * nsc + I * CLUSTER + il];
+  const unsigned long long wa = (ai >= 0 && !d.use_excl_list) ? d.excl_bits[ai] : 0ull;
   unsigned long long* mout = d.search_mtmp + (size_t)I * SEARCH_TMP_CAP;
   unsigned long long bi[6];
 #pragma unroll
@@ -312,21 +375,41 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
         }
         unsigned long long bal = __ballot(hit);
         while (bal) {
-          const int k = __ffsll((long long)bal) - 1;
-          bal &= bal - 1ull;
-          const int Jk = jb + k;
-          const int b = d.slot_atom[Jk * CLUSTER + jl];
-          bool on = ai >= 0 && b >= 0 && (I != Jk || il < jl);
-          if (on) on = !is_excluded(d, ai, b);
-          if (on) on = r2_exact(d, aix, aiy, aiz, b) < d.rl2;
-          const unsigned long long m = __ballot(on);
-          if (m) {
-            if (count < SEARCH_TMP_CAP) {
-              if (lane == 0) { out[count] = Jk; mout[count] = m; }
-            } else {
-              over = true;
+          // take up to SEARCH_BATCH hits (in ascending J) and load all their atoms
+          // before testing any, so the loads overlap
+          int Jk[SEARCH_BATCH], b[SEARCH_BATCH];
+          unsigned long long bx[SEARCH_BATCH], by[SEARCH_BATCH], bz[SEARCH_BATCH];
+#pragma unroll
+          for (int u = 0; u < SEARCH_BATCH; u++) {
+            Jk[u] = -1;
+            if (bal) {
+              Jk[u] = jb + __ffsll((long long)bal) - 1;
+              bal &= bal - 1ull;
             }
-            count++;
+          }
+#pragma unroll
+          for (int u = 0; u < SEARCH_BATCH; u++) {
+            const int sl = (Jk[u] >= 0 ? Jk[u] : I) * CLUSTER + jl;  // I: an in-range stand-in
+            b[u] = d.slot_atom[sl];
+            bx[u] = d.su[sl];
+            by[u] = d.su[nsc + sl];
+            bz[u] = d.su[2 * nsc + sl];
+          }
+#pragma unroll
+          for (int u = 0; u < SEARCH_BATCH; u++) {
+            if (Jk[u] < 0) break;
+            bool on = ai >= 0 && b[u] >= 0 && (I != Jk[u] || il < jl);
+            if (on) on = !excluded_w(d, wa, ai, b[u]);
+            if (on) on = r2_exact(d, aix, aiy, aiz, bx[u], by[u], bz[u]) < d.rl2;
+            const unsigned long long m = __ballot(on);
+            if (m) {
+              if (count < SEARCH_TMP_CAP) {
+                if (lane == 0) { out[count] = Jk[u]; mout[count] = m; }
+              } else {
+                over = true;
+              }
+              count++;
+            }
           }
         }
       }

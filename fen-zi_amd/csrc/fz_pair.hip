@@ -44,6 +44,8 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict_
       v.w = __float_as_int((float)d.q[a]);
     }
     ((int4*)d.xs)[sl] = v;
+    const int t = a >= 0 ? d.type[a] : 0;
+    d.lp[sl] = d.lj_sep ? d.lj_ah[t] : make_float2(__int_as_float(t), 0.0f);
   } else {
     Slot64 v = {0ull, 0ull, 0ull, 0.0};
     if (a >= 0) v = Slot64{d.ux[a], d.uy[a], d.uz[a], d.q[a]};
@@ -103,6 +105,38 @@ __device__ __forceinline__ void store_j(Acc* F, int cap, int J, unsigned long lo
     else unsafeAtomicAdd((double*)p, (double)v);
   }
 }
+// Global-address-space views of the buffers: loads and atomics through them
+// compile to global_* instructions (SGPR base + VGPR offset) instead of flat_*,
+// which also keeps them out of the LDS counter the DPP/permute traffic waits on.
+#define FZ_GLOBAL __attribute__((address_space(1)))
+typedef FZ_GLOBAL int GI;
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef FZ_GLOBAL v4i GI4;
+typedef FZ_GLOBAL v2f GF2;
+typedef FZ_GLOBAL unsigned long long GU64;
+
+__device__ __forceinline__ void atomic_add_g(GU64* p, unsigned long long v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// store_j for the mixed form's fixed-point accumulators
+__device__ __forceinline__ void store_j_g(GU64* F, int cap, int J, unsigned long long mask, int lane, float jx,
+                                          float jy, float jz) {
+  const int c = lane >> 3, jj = lane & 7;
+  const int src = jj << 3;
+  const float gx = __shfl(jx, src, 64), gy = __shfl(jy, src, 64), gz = __shfl(jz, src, 64);
+  const float v = c == 0 ? gx : (c == 1 ? gy : gz);
+  if (lane < 24 && ((mask >> (8 * jj)) & 0xffull))
+    atomic_add_g(F + (size_t)c * cap + J * CLUSTER + jj, to_fixed_acc((double)v));
+}
+
+__device__ __forceinline__ void add3_g(GU64* F, int cap, int slot, double fx, double fy, double fz) {
+  atomic_add_g(F + slot, to_fixed_acc(fx));
+  atomic_add_g(F + cap + slot, to_fixed_acc(fy));
+  atomic_add_g(F + 2 * cap + slot, to_fixed_acc(fz));
+}
+
 template <class T>
 __device__ __forceinline__ T red_stride8(T v) {  // sum over lanes sharing lane & 7
   v += __shfl_xor(v, 8, 64);
@@ -197,41 +231,64 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
     double cev[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const long long np = *d.n_pairs;
     const long long e0 = (long long)(w - chunk_pre[s]) * PAIR_CHUNK;
-    const long long e1 = e0 + PAIR_CHUNK < np ? e0 + PAIR_CHUNK : np;
-    const int4* __restrict__ xs = (const int4*)d.xs;
-    const int* __restrict__ ts = d.ts;
-    const float2* __restrict__ lj = d.lj32;
+    const int ne = rfl((int)(np - e0 < PAIR_CHUNK ? np - e0 : PAIR_CHUNK));  // uniform: scalar loop control
+    const GI4* xs = (const GI4*)d.xs;
+    const GF2* lp = (const GF2*)d.lp;
+    const GF2* lj = (const GF2*)d.lj32;
+    const int sep = d.lj_sep;
     const int T = d.n_types;
-    unsigned long long* F = (unsigned long long*)d.Fs;
+    GU64* F = (GU64*)d.Fs;
     const int cap = d.n_slots_cap;
     const float sx = d.sc32[0], sy = d.sc32[1], sz = d.sc32[2];
     const float rc2 = (float)d.rc2, ron2 = (float)d.ron2, beta = (float)d.beta, swi = (float)d.sw_inv;
     const float cc = (float)CC;
     const float ccb = (float)(TWO_OVER_SQRT_PI * d.beta);
     const float nb2l2e = -(float)(d.beta * d.beta * 1.4426950408889634);
+    // the chunk's entries, one per lane, read once and broadcast per entry
+    int vI = 0, vJ = 0;
+    unsigned vMlo = 0u, vMhi = 0u;
+    if (lane < ne) {
+      vI = ((const GI*)d.pl_i)[e0 + lane];
+      vJ = ((const GI*)d.pl_j)[e0 + lane];
+      const unsigned long long m = ((const GU64*)d.pl_mask)[e0 + lane];
+      vMlo = (unsigned)m;
+      vMhi = (unsigned)(m >> 32);
+    }
+    // j cluster data run one entry ahead of the arithmetic
+    int Jn = __builtin_amdgcn_readlane(vJ, 0);
+    v4i xjn = xs[Jn * CLUSTER + jl];
+    v2f pjn = lp[Jn * CLUSTER + jl];
     int curI = -1;
     int xix = 0, xiy = 0, xiz = 0, ti = 0;
     float qi = 0.f, qi_raw = 0.f;
+    v2f pi = {0.f, 0.f};
     float fix = 0.f, fiy = 0.f, fiz = 0.f;
-    for (long long e = e0; e < e1; e++) {
-      const int I = rfl(d.pl_i[e]);
-      const int J = rfl(d.pl_j[e]);
-      const unsigned long long mask = rfl64(d.pl_mask[e]);
+    for (int k = 0; k < ne; k++) {
+      const int I = __builtin_amdgcn_readlane(vI, k);
+      const int J = Jn;
+      const unsigned long long mask = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)vMhi, k) << 32) |
+                                      (unsigned)__builtin_amdgcn_readlane((int)vMlo, k);
+      const v4i xj = xjn;
+      const v2f pj = pjn;
+      if (k + 1 < ne) {
+        Jn = __builtin_amdgcn_readlane(vJ, k + 1);
+        xjn = xs[Jn * CLUSTER + jl];
+        pjn = lp[Jn * CLUSTER + jl];
+      }
       if (I != curI) {
         if (curI >= 0) {
           float rx = red_stride8(fix), ry = red_stride8(fiy), rz = red_stride8(fiz);
-          if (jl == 0) add3(F, cap, curI * CLUSTER + il, (double)rx, (double)ry, (double)rz);
+          if (jl == 0) add3_g(F, cap, curI * CLUSTER + il, (double)rx, (double)ry, (double)rz);
         }
         curI = I;
-        const int4 v = xs[I * CLUSTER + il];
+        const v4i v = xs[I * CLUSTER + il];
         xix = v.x; xiy = v.y; xiz = v.z;
         qi_raw = __int_as_float(v.w);
         qi = qi_raw * cc;
-        ti = ts[I * CLUSTER + il] * T;
+        pi = lp[I * CLUSTER + il];
+        ti = __float_as_int(pi.x) * T;
         fix = fiy = fiz = 0.f;
       }
-      const int4 xj = xs[J * CLUSTER + jl];
-      const int tj = ts[J * CLUSTER + jl];
       const float dx = (float)(int)((unsigned)xj.x - (unsigned)xix) * sx;
       const float dy = (float)(int)((unsigned)xj.y - (unsigned)xiy) * sy;
       const float dz = (float)(int)((unsigned)xj.z - (unsigned)xiz) * sz;
@@ -242,7 +299,13 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
       const float rinv2 = rinv * rinv;
       const float r = r2 * rinv;
       // Lennard-Jones with CHARMM switch (energy switched, exact gradient)
-      const float2 p = lj[ti + tj];
+      v2f p;
+      if (sep) {
+        const float h = pi.y + pj.y;
+        p = v2f{pi.x * pj.x, h * h};
+      } else {
+        p = lj[ti + __float_as_int(pj.x)];
+      }
       const float s2 = p.y * rinv2;
       const float s6 = s2 * s2 * s2;
       const float a6 = p.x * s6, a12 = a6 * s6;
@@ -265,7 +328,7 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
       const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
       fix -= fx; fiy -= fy; fiz -= fz;
       const float jx = red8(fx), jy = red8(fy), jz = red8(fz);
-      store_j(F, cap, J, mask, lane, jx, jy, jz);
+      store_j_g(F, cap, J, mask, lane, jx, jy, jz);
       if (EV) {
         if (act) {
           // energies of an evaluation step in FP64 from the same FP32 distance, so
@@ -288,7 +351,7 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
     }
     if (curI >= 0) {
       float rx = red_stride8(fix), ry = red_stride8(fiy), rz = red_stride8(fiz);
-      if (jl == 0) add3(F, cap, curI * CLUSTER + il, (double)rx, (double)ry, (double)rz);
+      if (jl == 0) add3_g(F, cap, curI * CLUSTER + il, (double)rx, (double)ry, (double)rz);
     }
     if (EV) ev.add_chunk(cev);
   }

@@ -74,6 +74,7 @@ struct SysDev {
   int *bin_tmp;          // [n_slots_cap] column-scattered atoms
   int *clu_col;          // [n_clusters_cap]
   unsigned long long *bb; // [6 * n_clusters_cap] lo/hi per axis
+  unsigned long long *su; // [3 * n_slots_cap] slot-ordered positions at the last build (x, y, z blocks)
   int *n_clusters;       // [0] clusters of the last build, [1] spread slot multiplier
   int *search_cnt;       // [n_clusters_cap + 1]
   int *search_tmp;       // [n_clusters_cap * SEARCH_TMP_CAP] J of each kept entry
@@ -94,6 +95,13 @@ struct SysDev {
   int2 *p14_ij;                           // 1-4 pairs
   // ---- LJ tables --------------------------------------------------------------
   float2 *lj32;   double2 *lj64;          // (eps, sigma^2) per type pair
+  // Mixed form: when every pair entry follows the combination rule
+  // eps_ij = sqrt(eps_ii eps_jj), sigma_ij = (sigma_ii + sigma_jj) / 2 (no NBFIX),
+  // lj_sep = 1 and lj_ah[t] = (sqrt(eps_tt), sigma_tt / 2); the pair kernel then
+  // forms the pair parameters from per-slot values instead of a table lookup.
+  int lj_sep;
+  float2 *lj_ah;                          // [n_types]
+  float2 *lp;                             // [n_slots_cap] per slot: lj_ah[type] (lj_sep) or (type bits, 0)
   double2 *lj14;                          // 1-4 (eps, sigma^2)
   // ---- PME --------------------------------------------------------------------
   void *Qacc;      // K0*K1*K2 accumulators (double or fixed int64)
