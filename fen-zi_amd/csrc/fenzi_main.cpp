@@ -48,6 +48,13 @@ int main(int argc, char** argv) {
   fz_system* sys = nullptr;
   if (fz_system_create(ctx, fz_deck_topology(deck), fz_deck_params(deck), prec, &sys)) return die("system setup");
   if (fz_set_state(sys, fz_deck_coordinates(deck), nullptr)) return die("initial state");
+  if (rs.minimizesteps > 0) {  // the reference minimises before dynamics (src/fenzi.cu:397-420)
+    fz_min_params mp;
+    mp.method = 1;
+    mp.max_step = 0.05;
+    mp.cgfac = rs.cgfac; mp.sdfac = rs.sdfac; mp.cgrate = rs.cgrate; mp.sdrate = rs.sdrate;
+    if (fz_minimize_ex(sys, (int)rs.minimizesteps, &mp)) return die("minimization");
+  }
   uint64_t counter = 0;
   if (fz_reassign_velocities(sys, rs.temperature, rs.seed, counter++)) return die("velocities");
   FILE* out = stdout;

@@ -27,13 +27,60 @@ __device__ __forceinline__ void cross3(const double* a, const double* b, double*
   c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+// Forces of a block's terms are first summed in LDS over the block's atom
+// window (consecutive terms share atoms: chain neighbours, the members of one
+// water), then added to the slot accumulators once per atom and component.
+// A block whose terms span more than BOND_WIN atoms adds directly.  In the
+// mixed form the LDS sums are fixed-point integers, so the result is the same
+// bit for bit either way.
+constexpr int BOND_WIN = 1024;
+
 template <class P>
-__device__ __forceinline__ void addf(const SysDev& d, int atom, const double* f) {
+struct Sink {
+  bool lds;
+  int w0;
+  typename P::acc_t* lacc;  // [3][BOND_WIN]
+};
+
+__device__ __forceinline__ void lds_add(unsigned long long* p, double v) { atomicAdd(p, to_fixed_acc(v)); }
+__device__ __forceinline__ void lds_add(double* p, double v) { atomicAdd(p, v); }
+
+template <class P>
+__device__ __forceinline__ void addf(const Sink<P>& sk, const SysDev& d, int atom, const double* f) {
+  if (sk.lds) {
+    const int k = atom - sk.w0;
+    lds_add(sk.lacc + k, f[0]);
+    lds_add(sk.lacc + BOND_WIN + k, f[1]);
+    lds_add(sk.lacc + 2 * BOND_WIN + k, f[2]);
+    return;
+  }
   typename P::acc_t* F = (typename P::acc_t*)d.Fs;
   const int sl = d.atom_slot[atom], cap = d.n_slots_cap;
   P::add(F + sl, f[0]);
   P::add(F + cap + sl, f[1]);
   P::add(F + 2 * cap + sl, f[2]);
+}
+
+__device__ __forceinline__ void add_acc(unsigned long long* p, unsigned long long v) { atomicAdd(p, v); }
+__device__ __forceinline__ void add_acc(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// smallest and largest atom of flat term t (hi = -1 if t is past the last term)
+__device__ __forceinline__ void term_span(const SysDev& d, int t, int& lo, int& hi) {
+  int4 v = make_int4(0, 0, 0, 0);
+  int na = 0;
+  if (t < d.n_bonds) {
+    const int2 b = d.bond_ij[t]; v = make_int4(b.x, b.y, b.y, b.y); na = 2;
+  } else if ((t -= d.n_bonds) < d.n_angles) {
+    v = d.angle_ijk[t]; v.w = v.z; na = 3;
+  } else if ((t -= d.n_angles) < d.n_diheds + d.n_imprs) {
+    v = t < d.n_diheds ? d.dihe_ijkl[t] : d.impr_ijkl[t - d.n_diheds]; na = 4;
+  } else if ((t -= d.n_diheds + d.n_imprs) < d.n_ew) {
+    const int2 b = d.ew_ij[t]; v = make_int4(b.x, b.y, b.y, b.y); na = 2;
+  } else if ((t -= d.n_ew) < d.n_p14) {
+    const int2 b = d.p14_ij[t]; v = make_int4(b.x, b.y, b.y, b.y); na = 2;
+  }
+  lo = na ? min(min(v.x, v.y), min(v.z, v.w)) : 0x7fffffff;
+  hi = na ? max(max(v.x, v.y), max(v.z, v.w)) : -1;
 }
 
 __device__ __forceinline__ void vir_outer(double* vir, const double* r, const double* f) {
@@ -73,6 +120,30 @@ __global__ __launch_bounds__(256) void k_bonded(const SysDev* __restrict__ S, in
   const int s_ = locate(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   int t = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  // atom window of the block
+  __shared__ typename P::acc_t lacc[3 * BOND_WIN];
+  __shared__ int wlo, whi;
+  if (threadIdx.x == 0) { wlo = 0x7fffffff; whi = -1; }
+  __syncthreads();
+  {
+    int lo, hi;
+    term_span(d, t, lo, hi);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      lo = min(lo, __shfl_xor(lo, o, 64));
+      hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0 && hi >= 0) { atomicMin(&wlo, lo); atomicMax(&whi, hi); }
+  }
+  __syncthreads();
+  Sink<P> sk{whi >= 0 && whi - wlo < BOND_WIN, wlo, lacc};
+  const int span = sk.lds ? whi - wlo + 1 : 0;
+  for (int k = threadIdx.x; k < span; k += 256) {
+    lacc[k] = 0;
+    lacc[BOND_WIN + k] = 0;
+    lacc[2 * BOND_WIN + k] = 0;
+  }
+  __syncthreads();
   double e[BE_N + 9];
 #pragma unroll
   for (int k = 0; k < BE_N + 9; k++) e[k] = 0.0;
@@ -87,8 +158,8 @@ __global__ __launch_bounds__(256) void k_bonded(const SysDev* __restrict__ S, in
       e[BE_BOND] += p.x * dr * dr;
       const double fr = -2.0 * p.x * dr / r;
       double fb[3] = {fr * v[0], fr * v[1], fr * v[2]}, fa[3] = {-fb[0], -fb[1], -fb[2]};
-      addf<P>(d, ab.y, fb);
-      addf<P>(d, ab.x, fa);
+      addf<P>(sk, d, ab.y, fb);
+      addf<P>(sk, d, ab.x, fa);
       if (EV) vir_outer(vir, v, fb);
       break;
     }
@@ -112,9 +183,9 @@ __global__ __launch_bounds__(256) void k_bonded(const SysDev* __restrict__ S, in
         fc[k] = kf * (v1[k] * q - cs * v2[k] / l2);
         fb[k] = -fa[k] - fc[k];
       }
-      addf<P>(d, at.x, fa);
-      addf<P>(d, at.z, fc);
-      addf<P>(d, at.y, fb);
+      addf<P>(sk, d, at.x, fa);
+      addf<P>(sk, d, at.z, fc);
+      addf<P>(sk, d, at.y, fb);
       if (EV) { vir_outer(vir, v1, fa); vir_outer(vir, v2, fc); }
       if (p.z != 0.0) {
         double w[3];
@@ -123,8 +194,8 @@ __global__ __launch_bounds__(256) void k_bonded(const SysDev* __restrict__ S, in
         e[BE_UB] += p.z * dr * dr;
         const double fr = -2.0 * p.z * dr / r;
         double f2[3] = {fr * w[0], fr * w[1], fr * w[2]}, f1[3] = {-f2[0], -f2[1], -f2[2]};
-        addf<P>(d, at.z, f2);
-        addf<P>(d, at.x, f1);
+        addf<P>(sk, d, at.z, f2);
+        addf<P>(sk, d, at.x, f1);
         if (EV) vir_outer(vir, w, f2);
       }
       break;
@@ -156,10 +227,10 @@ __global__ __launch_bounds__(256) void k_bonded(const SysDev* __restrict__ S, in
       for (int m = 0; m < 4; m++)
 #pragma unroll
         for (int k = 0; k < 3; k++) fo[m][k] = -dEdphi * g[m][k];
-      addf<P>(d, at.x, fo[0]);
-      addf<P>(d, at.y, fo[1]);
-      addf<P>(d, at.z, fo[2]);
-      addf<P>(d, at.w, fo[3]);
+      addf<P>(sk, d, at.x, fo[0]);
+      addf<P>(sk, d, at.y, fo[1]);
+      addf<P>(sk, d, at.z, fo[2]);
+      addf<P>(sk, d, at.w, fo[3]);
       if (EV) {
         double ra[3] = {F[0] + G[0], F[1] + G[1], F[2] + G[2]};
         vir_outer(vir, ra, fo[0]);
@@ -179,8 +250,8 @@ __global__ __launch_bounds__(256) void k_bonded(const SysDev* __restrict__ S, in
       e[BE_EWEXCL] += -qq * erfv / r;
       const double fr = qq * (br * TWO_OVER_SQRT_PI * exp(-br * br) - erfv) / (r2 * r);
       double fj[3] = {fr * v[0], fr * v[1], fr * v[2]}, fi[3] = {-fj[0], -fj[1], -fj[2]};
-      addf<P>(d, ij.y, fj);
-      addf<P>(d, ij.x, fi);
+      addf<P>(sk, d, ij.y, fj);
+      addf<P>(sk, d, ij.x, fi);
       if (EV) vir_outer(vir, v, fj);
       break;
     }
@@ -211,12 +282,25 @@ __global__ __launch_bounds__(256) void k_bonded(const SysDev* __restrict__ S, in
       e[BE_ELEC] += ec;
       const double fr = fc + flj;  // F_i = fr (r_i - r_j)
       double fj[3] = {fr * v[0], fr * v[1], fr * v[2]}, fi[3] = {-fj[0], -fj[1], -fj[2]};
-      addf<P>(d, ij.y, fj);
-      addf<P>(d, ij.x, fi);
+      addf<P>(sk, d, ij.y, fj);
+      addf<P>(sk, d, ij.x, fi);
       if (EV) vir_outer(vir, v, fj);
       break;
     }
   } while (false);
+  __syncthreads();
+  if (span) {
+    typename P::acc_t* F = (typename P::acc_t*)d.Fs;
+    const int cap = d.n_slots_cap;
+    for (int k = threadIdx.x; k < span; k += 256) {
+      const typename P::acc_t fx = lacc[k], fy = lacc[BOND_WIN + k], fz = lacc[2 * BOND_WIN + k];
+      if (fx == 0 && fy == 0 && fz == 0) continue;
+      const int sl = d.atom_slot[sk.w0 + k];
+      if (fx != 0) add_acc(F + sl, fx);
+      if (fy != 0) add_acc(F + cap + sl, fy);
+      if (fz != 0) add_acc(F + 2 * cap + sl, fz);
+    }
+  }
   if (EV) {
     const int slots[BE_N + 9] = {A_BOND, A_ANGLE, A_UB, A_DIHED, A_IMPR, A_EWEXCL, A_VDW, A_ELEC,
                                  A_VIR + 0, A_VIR + 1, A_VIR + 2, A_VIR + 3, A_VIR + 4,

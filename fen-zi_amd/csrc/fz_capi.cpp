@@ -286,4 +286,26 @@ int fz_batch_minimize(fz_batch* b, int nsteps, double max_step) {
   FZ_GUARD_END
 }
 
+static bool min_params_ok(const fz_min_params* p) {
+  if (!p) return false;
+  if (p->method == 0) return p->max_step > 0;
+  return (p->method == 1 || p->method == 2) && p->cgfac >= 0 && p->sdfac >= 0 && p->cgrate >= 0 && p->sdrate >= 0;
+}
+
+int fz_minimize_ex(fz_system* s, int nsteps, const fz_min_params* p) {
+  FZ_GUARD_BEGIN
+  if (!s || nsteps < 0 || !min_params_ok(p)) return set_error(FZ_ERR_ARG, "invalid argument");
+  FZ_HIP(hipSetDevice(s->ctx->device));
+  return s->eng->minimize_ref(nsteps, *p);
+  FZ_GUARD_END
+}
+
+int fz_batch_minimize_ex(fz_batch* b, int nsteps, const fz_min_params* p) {
+  FZ_GUARD_BEGIN
+  if (!b || nsteps < 0 || !min_params_ok(p)) return set_error(FZ_ERR_ARG, "invalid argument");
+  FZ_HIP(hipSetDevice(b->ctx->device));
+  return b->eng->minimize_ref(nsteps, *p);
+  FZ_GUARD_END
+}
+
 }  // extern "C"

@@ -348,6 +348,7 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
     long dynasteps = 10000, outputTiming = 100, startstep = 0, minimize = 0, tempreset = 0;
     int shake = 0, printstdout = 1;
     double seed = 0;
+    double cgfac = 1e-10, sdfac = 1e-4, cgrate = 0.1, sdrate = 0.01;
     double box[3] = {0, 0, 0};
     int K[3] = {0, 0, 0};
     std::string crd, psf, prm, rtf, outname;
@@ -363,6 +364,10 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
       else if (k == "topology") rtf = val(1);
       else if (k == "temperature") T = atof(val(1).c_str());
       else if (k == "minimizesteps") minimize = (long)atof(val(1).c_str());
+      else if (k == "conjgradfac") cgfac = atof(val(1).c_str());
+      else if (k == "steepdesfac") sdfac = atof(val(1).c_str());
+      else if (k == "conjgradrate") cgrate = atof(val(1).c_str());
+      else if (k == "steepdesrate") sdrate = atof(val(1).c_str());
       else if (k == "startstep") startstep = atol(val(1).c_str());
       else if (k == "timestep") dt = atof(val(1).c_str());
       else if (k == "dynasteps") dynasteps = atol(val(1).c_str());
@@ -529,6 +534,7 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
     s.tempreset_every = tempreset; s.output_every = outputTiming; s.seed = (uint64_t)seed;
     s.shake = shake; s.print_to_stdout = printstdout;
     snprintf(s.output_prefix, sizeof s.output_prefix, "%s", outname.c_str());
+    s.cgfac = cgfac; s.sdfac = sdfac; s.cgrate = cgrate; s.sdrate = sdrate;
     *out = guard.release();
     return FZ_OK;
   } catch (...) {

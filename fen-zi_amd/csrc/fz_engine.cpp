@@ -274,6 +274,8 @@ int System::alloc_device() {
   d.rc2 = par.cutoff * par.cutoff;
   d.ron2 = par.cuton * par.cuton;
   d.rl2 = par.pairlist * par.pairlist;
+  d.rl2f = (float)(par.pairlist * par.pairlist * (1.0 + 1e-6));
+  d.zwin = (unsigned long long)(par.pairlist / d.sc[2] * (1.0 + 1e-9)) + 64ull;
   double half_skin = 0.5 * (par.pairlist - par.cutoff);
   d.skin2 = half_skin * half_skin;
   d.beta = par.beta;
@@ -386,7 +388,9 @@ int System::alloc_device() {
   }
   FZ_TRY(dalloc(&d.acc, A_NSLOT));
   FZ_TRY(dalloc(&d.flags, F_NSLOT));
-  FZ_TRY(dalloc(&d.mstep, 1));
+  FZ_TRY(dalloc(&d.mstep, 2));
+  FZ_TRY(dalloc(&d.mH, (size_t)3 * n));
+  FZ_TRY(dalloc(&d.mF, (size_t)3 * n));
   FZ_TRY(dalloc(&snap, (size_t)9 * n));
   return set_column_grid(par.column_grid[0], par.column_grid[1], 1.0);
 }
@@ -413,6 +417,7 @@ int System::set_column_grid(int ncx, int ncy, double f) {
   d.ky_range = (int)std::ceil(par.pairlist / (par.box[1] / ncy)) + 1;
   d.colw_x = (unsigned long long)(kTwo64 / ncx);
   d.colw_y = (unsigned long long)(kTwo64 / ncy);
+  d.nzb = std::min(256, std::max(4, (int)std::ceil(par.box[2] / 2.0)));  // ~2 A z bins
   const int clusters_cap = n / CLUSTER + d.ncol + 64;
   const double cw = std::sqrt(par.box[0] / ncx * par.box[1] / ncy);
   const double est = 4.0 / 3.0 * kPi * std::pow(par.pairlist + cw + 1.0, 3) * rho / CLUSTER * 0.5 + 2.0;
@@ -427,6 +432,8 @@ int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   SysDev& d = dv;
   nb_alloc0 = allocs.size();
   FZ_TRY(dalloc(&d.col_count, d.ncol)); FZ_TRY(dalloc(&d.col_fill, d.ncol)); FZ_TRY(dalloc(&d.col_clu0, d.ncol + 1));
+  FZ_TRY(dalloc(&d.zt_hi, (size_t)d.ncol * (d.nzb + 1)));
+  FZ_TRY(dalloc(&d.zt_lo, (size_t)d.ncol * (d.nzb + 1)));
   d.n_clusters_cap = clusters_cap;
   d.n_slots_cap = clusters_cap * CLUSTER;
   d.n_pairs_cap = pairs_cap;
@@ -861,6 +868,71 @@ int Engine::minimize(int nsteps, double max_step) {
   // the adaptive step carries over to the next call, so a minimisation split
   // into several calls takes the same path as one call
   for (int s = 0; s < ns; s++) sys_[s]->min_step = step[s];
+  launch_zero_vel(L_);
+  return forces(true);
+}
+
+void launch_min_ref_move(const Launch& L);
+
+// The reference's schedule (MinimizeStep, src/dynamics.cu:491-620; factor ramp
+// src/fenzi.cu:402-415), per system: move by the current method, evaluate; an
+// energy above the last accepted one (the first move is always accepted) undoes
+// the move and demotes conjugate gradient -> steepest descent -> stop; after
+// every 100 accepted moves the active factor grows by rate * 100.
+int Engine::minimize_ref(int nsteps, const fz_min_params& p) {
+  if (p.method == 0) return minimize(nsteps, p.max_step);
+  if (p.method != 1 && p.method != 2) return set_error(FZ_ERR_ARG, "unknown minimisation method %d", p.method);
+  FZ_TRY(ensure());
+  const int ns = (int)sys_.size();
+  std::vector<int> method(ns, p.method), done(ns, 0), have(ns, 0), reject(ns, 0);
+  std::vector<double> cg(ns, p.cgfac), sd(ns, p.sdfac), e_prev(ns, 0.0);
+  std::vector<double> ctl(2 * ns);
+  int* d_reject = nullptr;
+  FZ_HIP(hipMalloc(&d_reject, ns * sizeof(int)));
+  struct Free { int* p; ~Free() { if (p) (void)hipFree(p); } } guard{d_reject};
+  for (auto* s : sys_) {
+    FZ_HIP(hipMemsetAsync(s->dv.mH, 0, sizeof(double) * 3 * s->topo.n, L_.stream));
+    FZ_HIP(hipMemsetAsync(s->dv.mF, 0, sizeof(double) * 3 * s->topo.n, L_.stream));
+  }
+  FZ_TRY(forces(true));
+  fz_energies e;
+  for (int it = 0; it < nsteps; it++) {
+    bool active = false;
+    for (int s = 0; s < ns; s++) {
+      active = active || method[s] != 0;
+      ctl[2 * s] = method[s] == 1 ? cg[s] : sd[s];
+      ctl[2 * s + 1] = (double)method[s];
+      FZ_HIP(hipMemcpyAsync(sys_[s]->dv.mstep, &ctl[2 * s], 2 * sizeof(double), hipMemcpyHostToDevice, L_.stream));
+    }
+    if (!active) break;
+    launch_min_ref_move(L_);
+    FZ_HIP(hipGetLastError());
+    FZ_TRY(forces(true));
+    bool any = false;
+    for (int s = 0; s < ns; s++) {
+      reject[s] = 0;
+      if (method[s] == 0) continue;
+      sys_[s]->energies(&e);
+      if (have[s] && e.potential > e_prev[s]) {
+        reject[s] = 1;
+        any = true;
+        method[s] = method[s] == 1 ? 2 : 0;
+      } else {
+        e_prev[s] = e.potential;
+        have[s] = 1;
+        const int idx = done[s]++;  // 0-based index of the accepted move
+        if (idx > 0 && idx % 100 == 0) {
+          if (method[s] == 1) cg[s] *= p.cgrate * 100.0;
+          else sd[s] *= p.sdrate * 100.0;
+        }
+      }
+    }
+    if (any) {
+      FZ_HIP(hipMemcpyAsync(d_reject, reject.data(), ns * sizeof(int), hipMemcpyHostToDevice, L_.stream));
+      launch_min_restore(L_, d_reject);
+      FZ_TRY(forces(true));
+    }
+  }
   launch_zero_vel(L_);
   return forces(true);
 }

@@ -90,6 +90,7 @@ class Engine {
   int steps(int n);                    // n MD steps (energies on the last)
   int maxwell(double T, uint64_t seed, uint64_t counter);
   int minimize(int nsteps, double max_step);  // adaptive steepest descent, velocities zeroed
+  int minimize_ref(int nsteps, const fz_min_params& p);  // the reference's CG -> SD schedule
   // nsteps eager (non-graph) steps with HIP events between phases; mean ms per
   // phase per step into out[FZ_NPHASE] (order: see fz_phase_times in fenzi.h)
   int phase_times(int nsteps, double* out);

@@ -359,6 +359,53 @@ __global__ __launch_bounds__(256) void k_min_move(const SysDev* __restrict__ S, 
   d.uz[i] = uz + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, 2, false) * d.isc[2]);
 }
 
+// The reference's minimiser moves (ConjugatedGradient / SteepestDescent,
+// src/mdcuda.cu:126-330): method 1, per-atom Fletcher-Reeves direction
+// H = (F + gamma H_prev) * fac with gamma = |F|^2 / |F_prev|^2 (F_prev taken as F
+// when |F_prev|^2 < 1e-8); method 2, H = F * fac; method 0, no move.  u += H.
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_min_ref_move(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int n = d.n_atoms;
+  if (i >= n) return;
+  const unsigned long long ux = d.ux[i], uy = d.uy[i], uz = d.uz[i];
+  d.ox[i] = ux; d.oy[i] = uy; d.oz[i] = uz;
+  const int method = (int)d.mstep[1];
+  if (method == 0) return;
+  const double fac = d.mstep[0];
+  const int sl = d.atom_slot[i];
+  const double F[3] = {take_force<MIXED>(d, sl, 0, false), take_force<MIXED>(d, sl, 1, false),
+                       take_force<MIXED>(d, sl, 2, false)};
+  double h[3];
+  if (method == 1) {
+    double p[3] = {d.mF[i], d.mF[n + i], d.mF[2 * n + i]};
+    double p2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    const double f2 = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+    if (p2 < 1e-8) p2 = f2;
+    const double g = p2 > 0.0 ? f2 / p2 : 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      h[k] = (F[k] + g * d.mH[k * n + i]) * fac;
+      d.mH[k * n + i] = h[k];
+      d.mF[k * n + i] = F[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) h[k] = F[k] * fac;
+  }
+  d.ux[i] = ux + (unsigned long long)__double2ll_rn(h[0] * d.isc[0]);
+  d.uy[i] = uy + (unsigned long long)__double2ll_rn(h[1] * d.isc[1]);
+  d.uz[i] = uz + (unsigned long long)__double2ll_rn(h[2] * d.isc[2]);
+}
+
+void launch_min_ref_move(const Launch& L) {
+  if (L.mixed) hipLaunchKernelGGL(k_min_ref_move<true>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  else hipLaunchKernelGGL(k_min_ref_move<false>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  launch_shake(L);
+}
+
 __global__ __launch_bounds__(256) void k_min_restore(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
                                                      const int* __restrict__ reject) {
   const int s_ = locate(pre, nsys, blockIdx.x);

@@ -243,6 +243,25 @@ __global__ __launch_bounds__(256) void k_col_sort(const SysDev* __restrict__ S, 
     for (int e = 0; e < 3; e++) { b[2 * e] = lo[e]; b[2 * e + 1] = hi[e]; }
     d.clu_col[clu0 + m] = c;
   }
+  // z-window tables: the clusters' z bounds are non-decreasing along the column
+  // (atoms are sorted by z), so each table entry is a binary search
+  const int nzb = d.nzb;
+  for (int bz = threadIdx.x; bz <= nzb; bz += 256) {
+    int lo = 0, hi = nclu;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      const unsigned long long zu = key[min(mid * CLUSTER + CLUSTER - 1, n - 1)];
+      if ((int)__umul64hi(zu, (unsigned long long)nzb) >= bz) hi = mid; else lo = mid + 1;
+    }
+    d.zt_hi[(size_t)c * (nzb + 1) + bz] = lo;
+    lo = 0; hi = nclu;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      const unsigned long long zl = key[mid * CLUSTER];
+      if ((int)__umul64hi(zl, (unsigned long long)nzb) >= bz) hi = mid; else lo = mid + 1;
+    }
+    d.zt_lo[(size_t)c * (nzb + 1) + bz] = lo;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     d.col_count[c] = 0;
@@ -318,14 +337,33 @@ __device__ __forceinline__ bool excluded_w(const SysDev& d, unsigned long long w
   return is_excluded(d, a, b);
 }
 
-constexpr int SEARCH_BATCH = 8;  // candidate J clusters whose atoms are loaded together
+// FP32 atom-pair distance of the list test: 32-bit top words of the fixed-point
+// positions, products and sums rounded one by one (no contraction), so the
+// oracle restates it bit for bit.  The threshold carries a 1e-6 relative margin
+// over rl^2, so every pair with r < rl is kept whatever the rounding.
+__device__ __forceinline__ float r2_list32(const SysDev& d, unsigned ax, unsigned ay, unsigned az, unsigned bx,
+                                           unsigned by, unsigned bz) {
+#pragma clang fp contract(off)
+  const float dx = (float)(int)(bx - ax) * d.sc32[0];
+  const float dy = (float)(int)(by - ay) * d.sc32[1];
+  const float dz = (float)(int)(bz - az) * d.sc32[2];
+  float r2 = dx * dx;
+  r2 = r2 + dy * dy;
+  r2 = r2 + dz * dz;
+  return r2;
+}
 
-// One wave per I cluster.  Candidates J >= I whose bounding-box gap is below
-// the pair-list distance are tested atom by atom: lane l (i = l & 7, j = l >> 3)
-// keeps its pair when both slots hold atoms, the pair is not excluded, i < j
-// inside a self pair, and r < pair-list distance at build time.  Pairs farther
-// than that cannot reach the cutoff before the next build, so an entry whose
-// 64-bit lane mask is empty is dropped.
+// One wave per I cluster.
+//  1. Candidate columns (column-gap prefilter), one per lane; per column, the
+//     z-window tables give the (at most two) ranges of clusters whose z bounds
+//     can come within the pair-list distance of I's z bounds.
+//  2. Those ranges (J >= I) are packed 64 to a block; each lane tests its J's
+//     bounding box exactly (periodic gap in integers, FP64 squares).
+//  3. Each hit is tested atom by atom: lane l (i = l & 7, j = l >> 3) keeps its
+//     pair when both slots hold atoms, the pair is not excluded, i < j inside a
+//     self pair, and the FP32 list distance is below the threshold.  An entry
+//     whose 64-bit lane mask is empty is dropped.  The next hit's atoms are
+//     loaded while the current one is tested.
 __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
@@ -334,11 +372,13 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
   const int I = lb * 4 + (threadIdx.x >> 6);
   if (I >= *d.n_clusters) return;
   const int nsc = d.n_slots_cap;
+  const unsigned* su32 = (const unsigned*)d.su;  // high words at odd indices
   const int ai = d.slot_atom[I * CLUSTER + il];
-  const unsigned long long aix = d.su[I * CLUSTER + il], aiy = d.su[nsc + I * CLUSTER + il],
-                           aiz = d.su[2 * nsc + I * CLUSTER + il];
+  const unsigned aix = su32[2 * (I * CLUSTER + il) + 1], aiy = su32[2 * (nsc + I * CLUSTER + il) + 1],
+                 aiz = su32[2 * (2 * nsc + I * CLUSTER + il) + 1];
   const unsigned long long wa = (ai >= 0 && !d.use_excl_list) ? d.excl_bits[ai] : 0ull;
   unsigned long long* mout = d.search_mtmp + (size_t)I * SEARCH_TMP_CAP;
+  int* out = d.search_tmp + (size_t)I * SEARCH_TMP_CAP;
   unsigned long long bi[6];
 #pragma unroll
   for (int e = 0; e < 6; e++) bi[e] = d.bb[6 * (size_t)I + e];
@@ -350,71 +390,107 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
   const bool allx = 2 * d.kx_range + 1 >= d.ncx, ally = 2 * d.ky_range + 1 >= d.ncy;
   const int nyv = ally ? d.ncy : 2 * d.ky_range + 1;
   const int nxv = allx ? d.ncx : 2 * d.kx_range + 1;
-  int* out = d.search_tmp + (size_t)I * SEARCH_TMP_CAP;
+  // z window [w0, w1] on the 2^64 circle
+  const int nzb = d.nzb;
+  const unsigned long long R = d.zwin;
+  const unsigned long long binw = ~0ull / (unsigned long long)nzb;
+  const bool zfull = R >= (1ull << 61) || bi[5] - bi[4] >= ~0ull - 2 * R - binw;
+  const unsigned long long w0 = bi[4] - R, w1 = bi[5] + R;
+  const bool zwrap = !zfull && w0 > w1;
+  const int b0 = (int)__umul64hi(w0, (unsigned long long)nzb);
+  const int b1 = (int)__umul64hi(w1, (unsigned long long)nzb);
+
   int count = 0;
   bool over = false;
-  for (int iy = 0; iy < nyv; iy++) {
-    int yy = ally ? iy : (cy + iy - d.ky_range + 2 * d.ncy) % d.ncy;
-    double gy = col_gap(ay0, ay1, yy, wy, d.L[1]);
-    if (gy * gy >= d.rl2) continue;
-    for (int ix = 0; ix < nxv; ix++) {
-      int xx = allx ? ix : (cx + ix - d.kx_range + 2 * d.ncx) % d.ncx;
-      double gx = col_gap(ax0, ax1, xx, wx, d.L[0]);
-      if (gx * gx + gy * gy >= d.rl2) continue;
-      const int C = yy * d.ncx + xx;
-      int j0 = d.col_clu0[C], j1 = d.col_clu0[C + 1];
-      if (j0 < I) j0 = I;
-      for (int jb = j0; jb < j1; jb += 64) {
-        const int J = jb + lane;
-        bool hit = false;
-        if (J < j1) {
-          unsigned long long bj[6];
+  int cand = 0, fill = 0;
+
+  auto test_hits = [&]() {
+    bool hit = false;
+    if (lane < fill) {
+      unsigned long long bj[6];
 #pragma unroll
-          for (int e = 0; e < 6; e++) bj[e] = d.bb[6 * (size_t)J + e];
-          hit = bbox_within(bi, bj, d);
+      for (int e = 0; e < 6; e++) bj[e] = d.bb[6 * (size_t)cand + e];
+      hit = bbox_within(bi, bj, d);
+    }
+    unsigned long long bal = __ballot(hit);
+    if (!bal) return;
+    int Jn = __builtin_amdgcn_readlane(cand, __ffsll((long long)bal) - 1);
+    bal &= bal - 1ull;
+    int sl = Jn * CLUSTER + jl;
+    int bn = d.slot_atom[sl];
+    unsigned bxn = su32[2 * sl + 1], byn = su32[2 * (nsc + sl) + 1], bzn = su32[2 * (2 * nsc + sl) + 1];
+    for (;;) {
+      const int Jk = Jn, b = bn;
+      const unsigned bx = bxn, by = byn, bz = bzn;
+      const bool more = bal != 0ull;
+      if (more) {
+        Jn = __builtin_amdgcn_readlane(cand, __ffsll((long long)bal) - 1);
+        bal &= bal - 1ull;
+        sl = Jn * CLUSTER + jl;
+        bn = d.slot_atom[sl];
+        bxn = su32[2 * sl + 1]; byn = su32[2 * (nsc + sl) + 1]; bzn = su32[2 * (2 * nsc + sl) + 1];
+      }
+      bool on = ai >= 0 && b >= 0 && (I != Jk || il < jl);
+      if (on) on = !excluded_w(d, wa, ai, b);
+      if (on) on = r2_list32(d, aix, aiy, aiz, bx, by, bz) < d.rl2f;
+      const unsigned long long m = __ballot(on);
+      if (m) {
+        if (count < SEARCH_TMP_CAP) {
+          if (lane == 0) { out[count] = Jk; mout[count] = m; }
+        } else {
+          over = true;
         }
-        unsigned long long bal = __ballot(hit);
-        while (bal) {
-          // take up to SEARCH_BATCH hits (in ascending J) and load all their atoms
-          // before testing any, so the loads overlap
-          int Jk[SEARCH_BATCH], b[SEARCH_BATCH];
-          unsigned long long bx[SEARCH_BATCH], by[SEARCH_BATCH], bz[SEARCH_BATCH];
-#pragma unroll
-          for (int u = 0; u < SEARCH_BATCH; u++) {
-            Jk[u] = -1;
-            if (bal) {
-              Jk[u] = jb + __ffsll((long long)bal) - 1;
-              bal &= bal - 1ull;
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < SEARCH_BATCH; u++) {
-            const int sl = (Jk[u] >= 0 ? Jk[u] : I) * CLUSTER + jl;  // I: an in-range stand-in
-            b[u] = d.slot_atom[sl];
-            bx[u] = d.su[sl];
-            by[u] = d.su[nsc + sl];
-            bz[u] = d.su[2 * nsc + sl];
-          }
-#pragma unroll
-          for (int u = 0; u < SEARCH_BATCH; u++) {
-            if (Jk[u] < 0) break;
-            bool on = ai >= 0 && b[u] >= 0 && (I != Jk[u] || il < jl);
-            if (on) on = !excluded_w(d, wa, ai, b[u]);
-            if (on) on = r2_exact(d, aix, aiy, aiz, bx[u], by[u], bz[u]) < d.rl2;
-            const unsigned long long m = __ballot(on);
-            if (m) {
-              if (count < SEARCH_TMP_CAP) {
-                if (lane == 0) { out[count] = Jk[u]; mout[count] = m; }
-              } else {
-                over = true;
-              }
-              count++;
-            }
-          }
+        count++;
+      }
+      if (!more) break;
+    }
+  };
+  auto append = [&](int a, int e) {  // J in [a, e), uniform
+    while (a < e) {
+      const int m = min(e - a, 64 - fill);
+      if (lane >= fill && lane < fill + m) cand = a + (lane - fill);
+      fill += m;
+      a += m;
+      if (fill == 64) { test_hits(); fill = 0; }
+    }
+  };
+
+  const int nq = nyv * nxv;
+  for (int g = 0; g < nq; g += 64) {
+    // per lane: one candidate column and its (up to two) cluster ranges
+    int r0a = 0, r0e = 0, r1a = 0, r1e = 0;
+    const int q = g + lane;
+    if (q < nq) {
+      const int iy = q / nxv, ix = q - iy * nxv;
+      const int yy = ally ? iy : (cy + iy - d.ky_range + 2 * d.ncy) % d.ncy;
+      const int xx = allx ? ix : (cx + ix - d.kx_range + 2 * d.ncx) % d.ncx;
+      const double gy = col_gap(ay0, ay1, yy, wy, d.L[1]);
+      const double gx = col_gap(ax0, ax1, xx, wx, d.L[0]);
+      if (gx * gx + gy * gy < d.rl2) {
+        const int C = yy * d.ncx + xx;
+        const int j0 = d.col_clu0[C], nc = d.col_clu0[C + 1] - j0;
+        const int* th = d.zt_hi + (size_t)C * (nzb + 1);
+        const int* tl = d.zt_lo + (size_t)C * (nzb + 1);
+        if (zfull) {
+          r0a = 0; r0e = nc;
+        } else if (!zwrap) {
+          r0a = th[b0]; r0e = tl[b1 + 1];
+        } else {
+          const int e1 = tl[b1 + 1], s0 = th[b0];
+          if (s0 <= e1) { r0a = 0; r0e = nc; }
+          else { r0a = 0; r0e = e1; r1a = s0; r1e = nc; }
         }
+        r0a += j0; r0e += j0; r1a += j0; r1e += j0;
+        r0a = max(r0a, I); r1a = max(r1a, I);
       }
     }
+    const int nl = min(64, nq - g);
+    for (int k = 0; k < nl; k++) {
+      append(__builtin_amdgcn_readlane(r0a, k), __builtin_amdgcn_readlane(r0e, k));
+      append(__builtin_amdgcn_readlane(r1a, k), __builtin_amdgcn_readlane(r1e, k));
+    }
   }
+  if (fill) test_hits();
   if (lane == 0) {
     d.search_cnt[I] = count < SEARCH_TMP_CAP ? count : SEARCH_TMP_CAP;
     if (over) atomicOr(d.flags + F_OVERFLOW, OV_SEARCH);

@@ -37,6 +37,12 @@ struct SysDev {
   int ncx, ncy, ncol;    // column grid
   int kx_range, ky_range; // column search half-widths
   unsigned long long colw_x, colw_y; // floor(2^64 / ncx), floor(2^64 / ncy)
+  // z windows of the pair search: z is cut into nzb bins (bin(z) = mulhi(u_z, nzb));
+  // per column, zt_hi[c * (nzb + 1) + b] = first cluster k (within the column) whose
+  // upper z bound has bin >= b, zt_lo likewise for the lower bound (b = nzb: count)
+  int nzb;
+  unsigned long long zwin;  // pair-list distance in z fixed-point units, rounded up
+  float rl2f;               // FP32 atom-pair list threshold: rl^2 (1 + 1e-6)
   int K[3];              // PME grid
   int Kxh;               // K[0]/2 + 1
   int use_excl_list;     // 1: exclusions by sorted list, 0: 64-bit window bit vectors
@@ -71,6 +77,7 @@ struct SysDev {
   // ---- binning / search --------------------------------------------------
   int *atom_col;         // [n_atoms]
   int *col_count, *col_fill, *col_clu0; // [ncol], [ncol], [ncol + 1]
+  int *zt_hi, *zt_lo;                   // [ncol * (nzb + 1)] z-window tables
   int *bin_tmp;          // [n_slots_cap] column-scattered atoms
   int *clu_col;          // [n_clusters_cap]
   unsigned long long *bb; // [6 * n_clusters_cap] lo/hi per axis
@@ -118,7 +125,9 @@ struct SysDev {
   // ---- accumulators / flags ---------------------------------------------
   unsigned long long *acc;  // [A_NSLOT] int64 fixed point (A_FMAX: bits of max |F|^2)
   int *flags;               // [F_NSLOT]
-  double *mstep;            // minimiser: largest atom displacement of the next move (A)
+  double *mstep;            // minimiser: [0] largest atom displacement of the next move (A) or the
+                            // conjugate-gradient / steepest-descent factor, [1] schedule method
+  double *mH, *mF;          // [3 * n_atoms] conjugate-gradient direction and previous force (x, y, z blocks)
 };
 
 }  // namespace fz

@@ -62,7 +62,8 @@ class RunSettings(C.Structure):
     _fields_ = [("temperature", C.c_double), ("dynasteps", C.c_int64), ("minimizesteps", C.c_int64),
                 ("startstep", C.c_int64), ("tempreset_every", C.c_int64), ("output_every", C.c_int64),
                 ("seed", C.c_uint64), ("shake", C.c_int32), ("print_to_stdout", C.c_int32),
-                ("output_prefix", C.c_char * 256)]
+                ("output_prefix", C.c_char * 256), ("cgfac", C.c_double), ("sdfac", C.c_double),
+                ("cgrate", C.c_double), ("sdrate", C.c_double)]
 
 
 # every symbol include/fenzi.h declares
@@ -74,7 +75,20 @@ EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create
            "fz_deck_topology", "fz_deck_params", "fz_deck_coordinates", "fz_deck_settings",
            "fz_deck_psf_counts", "fz_format_energy_header", "fz_format_energy_row",
            "fz_server_create", "fz_server_submit", "fz_server_poll", "fz_server_job_info", "fz_server_job_result",
-           "fz_server_release", "fz_server_estimate_bytes", "fz_server_destroy"]
+           "fz_server_release", "fz_server_estimate_bytes", "fz_server_destroy",
+           "fz_minimize_ex", "fz_batch_minimize_ex"]
+
+MIN_ADAPTIVE_SD, MIN_REF_CG, MIN_REF_SD = 0, 1, 2
+
+
+class MinParams(C.Structure):
+    _fields_ = [("method", C.c_int32), ("max_step", C.c_double), ("cgfac", C.c_double), ("sdfac", C.c_double),
+                ("cgrate", C.c_double), ("sdrate", C.c_double)]
+
+
+def min_params(method=MIN_REF_CG, max_step=0.05, cgfac=1e-10, sdfac=1e-4, cgrate=0.1, sdrate=0.01):
+    """fz_min_params; defaults are the reference's (src/globals.cu:487-497)."""
+    return MinParams(int(method), float(max_step), float(cgfac), float(sdfac), float(cgrate), float(sdrate))
 
 JOB_QUEUED, JOB_MINIMIZING, JOB_DYNAMICS, JOB_DONE, JOB_FAILED = 0, 1, 2, 3, -1
 
@@ -131,6 +145,8 @@ def lib():
         L.fz_batch_create.argtypes = [vp, C.POINTER(vp), C.c_int, C.POINTER(vp)]
         L.fz_batch_step.argtypes = [vp, C.c_int]
         L.fz_batch_minimize.argtypes = [vp, C.c_int, C.c_double]
+        L.fz_minimize_ex.argtypes = [vp, C.c_int, C.POINTER(MinParams)]
+        L.fz_batch_minimize_ex.argtypes = [vp, C.c_int, C.POINTER(MinParams)]
         L.fz_batch_destroy.argtypes = [vp]
         L.fz_deck_load.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.fz_deck_destroy.argtypes = [vp]
@@ -292,6 +308,9 @@ class System:
     def minimize(self, n, max_step=0.05):
         _check(lib().fz_minimize(self.h, int(n), float(max_step)), "fz_minimize")
 
+    def minimize_ex(self, n, params):
+        _check(lib().fz_minimize_ex(self.h, int(n), C.byref(params)), "fz_minimize_ex")
+
     def energies(self):
         e = Energies()
         _check(lib().fz_get_energies(self.h, C.byref(e)), "fz_get_energies")
@@ -348,6 +367,9 @@ class Batch:
 
     def minimize(self, n, max_step=0.05):
         _check(lib().fz_batch_minimize(self.h, int(n), float(max_step)), "fz_batch_minimize")
+
+    def minimize_ex(self, n, params):
+        _check(lib().fz_batch_minimize_ex(self.h, int(n), C.byref(params)), "fz_batch_minimize_ex")
 
     def autotune(self, nsteps=10):
         out = np.zeros(4)

@@ -134,9 +134,29 @@ int fz_compute_forces(fz_system *sys, double *forces, fz_energies *e);
 int fz_step(fz_system *sys, int nsteps);
 /* Energies of the last evaluated step (temperature from current velocities). */
 int fz_get_energies(fz_system *sys, fz_energies *e);
-/* Steepest-descent / conjugate-gradient minimisation, nsteps evaluations.
- * Replaces MinimizeStep (src/dynamics.cu:491). */
+/* Adaptive steepest descent, nsteps iterations: the atom with the largest force
+ * moves max_step (A), an energy increase is undone and halves the step, a
+ * decrease grows it by 1.2 (capped); SHAKE after every move; velocities zeroed.
+ * Replaces MinimizeStep (src/dynamics.cu:491) with a step rule of this build. */
 int fz_minimize(fz_system *sys, int nsteps, double max_step);
+
+/* The reference's minimisation schedule (MinimizeStep, src/dynamics.cu:491-620;
+ * ConjugatedGradient / SteepestDescent, src/mdcuda.cu:126-330; factor ramp,
+ * src/fenzi.cu:402-415): per-atom Fletcher-Reeves conjugate gradient
+ * r += (F + gamma H_prev) cgfac, gamma = |F|^2/|F_prev|^2; on an energy
+ * increase the move is undone and conjugate gradient switches to steepest
+ * descent (r += F sdfac), and an increase under steepest descent ends the
+ * minimisation; after every 100 accepted moves the active factor is multiplied
+ * by rate * 100.  The .in keywords conjgradfac, steepdesfac, conjgradrate,
+ * steepdesrate (src/prmread.cu:547-561).  This build also applies SHAKE after
+ * each move.  method: 0 adaptive steepest descent (fz_minimize, max_step),
+ * 1 the reference schedule from conjugate gradient, 2 from steepest descent. */
+typedef struct fz_min_params {
+  int32_t method;
+  double max_step;
+  double cgfac, sdfac, cgrate, sdrate; /* reference defaults 1e-10, 1e-4, 0.1, 0.01 */
+} fz_min_params;
+int fz_minimize_ex(fz_system *sys, int nsteps, const fz_min_params *p);
 int fz_system_info(fz_system *sys, int64_t *n_atoms, int64_t *device_bytes, int32_t *precision);
 /* Per-phase device time: runs nsteps steps without graph capture, with HIP
  * events between phases on the system's stream; mean ms per step per phase in
@@ -178,6 +198,8 @@ int fz_batch_create(fz_ctx *ctx, fz_system *const *systems, int n, fz_batch **ou
 int fz_batch_step(fz_batch *b, int nsteps);
 /* Steepest descent on every member (per-system step control), one launch per phase. */
 int fz_batch_minimize(fz_batch *b, int nsteps, double max_step);
+/* fz_minimize_ex on every member (each member keeps its own schedule state). */
+int fz_batch_minimize_ex(fz_batch *b, int nsteps, const fz_min_params *p);
 /* fz_autotune over the whole batch (one setting for all members). */
 int fz_batch_autotune(fz_batch *b, int nsteps, double *out);
 void fz_batch_destroy(fz_batch *b);
@@ -242,6 +264,7 @@ typedef struct fz_run_settings {
   uint64_t seed;
   int32_t shake, print_to_stdout;
   char output_prefix[256];
+  double cgfac, sdfac, cgrate, sdrate; /* conjgradfac, steepdesfac, conjgradrate, steepdesrate */
 } fz_run_settings;
 int fz_deck_settings(const fz_deck *deck, fz_run_settings *out);
 /* Section counts of the PSF as read: bonds, angles, dihedrals, impropers. */

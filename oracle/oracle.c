@@ -854,8 +854,13 @@ int64_t or_cluster_pairs(or_sys *s, const uint64_t *u, int32_t nx, int32_t ny,
   double rl2 = s->p.pairlist * s->p.pairlist;
   /* Entry (I, J >= I) exists iff the bounding-box gap is below the pair-list
    * distance and at least one lane pair qualifies: both slots hold atoms, not
-   * excluded, i < j inside a self pair, r < pair-list distance.  The lane mask
-   * holds exactly the qualifying pairs.  Per I: (J, mask) in ascending J. */
+   * excluded, i < j inside a self pair, and the FP32 list distance (32-bit top
+   * words of the positions, each product and sum rounded to float) is below
+   * rl^2 (1 + 1e-6) rounded to float.  The lane mask holds exactly the
+   * qualifying pairs.  Per I: (J, mask) in ascending J. */
+  const float sc32[3] = {(float)(s->p.box[0] / 4294967296.0), (float)(s->p.box[1] / 4294967296.0),
+                         (float)(s->p.box[2] / 4294967296.0)};
+  const float rl2f = (float)(s->p.pairlist * s->p.pairlist * (1.0 + 1e-6));
   int64_t *cntI = (int64_t *)calloc((size_t)ncl + 1, sizeof(int64_t));
   int32_t **lj = (int32_t **)calloc((size_t)ncl + 1, sizeof(int32_t *));
   uint64_t **lm = (uint64_t **)calloc((size_t)ncl + 1, sizeof(uint64_t *));
@@ -878,12 +883,13 @@ int64_t or_cluster_pairs(or_sys *s, const uint64_t *u, int32_t nx, int32_t ny,
         if (a < 0 || b < 0) continue;
         if (I == J && il >= jl) continue;
         if (excluded(s, a, b)) continue;
-        double dx = (double)(int64_t)(u[3 * b] - u[3 * a]) * sc[0];
-        double dy = (double)(int64_t)(u[3 * b + 1] - u[3 * a + 1]) * sc[1];
-        double dz = (double)(int64_t)(u[3 * b + 2] - u[3 * a + 2]) * sc[2];
-        double r2 = dx * dx + dy * dy;
+        float dx = (float)(int32_t)((uint32_t)(u[3 * b] >> 32) - (uint32_t)(u[3 * a] >> 32)) * sc32[0];
+        float dy = (float)(int32_t)((uint32_t)(u[3 * b + 1] >> 32) - (uint32_t)(u[3 * a + 1] >> 32)) * sc32[1];
+        float dz = (float)(int32_t)((uint32_t)(u[3 * b + 2] >> 32) - (uint32_t)(u[3 * a + 2] >> 32)) * sc32[2];
+        float r2 = dx * dx;
+        r2 = r2 + dy * dy;
         r2 = r2 + dz * dz;
-        if (!(r2 < rl2)) continue;
+        if (!(r2 < rl2f)) continue;
         mask |= 1ull << l;
       }
       if (!mask) continue;
@@ -1085,6 +1091,21 @@ int or_step(or_sys *s, uint64_t *u, double *v, int nsteps) {
     for (int c = 0; c < s->n_clusters; c++) rattle_cluster(s, &s->clusters[c], u, sc, v);
   }
   free(f); free(uold);
+  return rc;
+}
+
+/* SHAKE of every constraint cluster of positions u, with uold as the
+ * reference (the minimiser's constraint step after each move). */
+int or_shake_positions(or_sys *s, const uint64_t *uold, uint64_t *u) {
+  double sc[3] = {s->p.box[0] * TWO_POW_M64, s->p.box[1] * TWO_POW_M64, s->p.box[2] * TWO_POW_M64};
+  int rc = 0;
+  for (int c = 0; c < s->n_clusters; c++) {
+    const or_cluster *cl = &s->clusters[c];
+    double dr[8][3];
+    if (shake_cluster(s, cl, uold, u, sc, dr) < 0) rc = -1;
+    for (int k = 0; k < cl->natom; k++)
+      for (int d = 0; d < 3; d++) u[3 * cl->atom[k] + d] += (uint64_t)(int64_t)llrint(dr[k][d] / sc[d]);
+  }
   return rc;
 }
 

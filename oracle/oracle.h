@@ -88,6 +88,8 @@ int64_t or_cluster_pairs(or_sys *s, const uint64_t *u, int32_t nx, int32_t ny,
 
 /* Velocity Verlet with SHAKE/RATTLE: nsteps steps; u, v updated in place. */
 int or_step(or_sys *s, uint64_t *u, double *v, int nsteps);
+/* SHAKE of positions u against reference positions uold (minimiser moves). */
+int or_shake_positions(or_sys *s, const uint64_t *uold, uint64_t *u);
 /* Maxwell velocities (Philox4x32-10 + Box-Muller), momentum removal, RATTLE projection. */
 void or_maxwell(or_sys *s, const uint64_t *u, double T, uint64_t seed, uint64_t counter, double *v);
 double or_kinetic(or_sys *s, const double *v);

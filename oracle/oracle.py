@@ -74,6 +74,8 @@ def lib():
         L.or_cluster_pairs.restype = C.c_int64
         L.or_step.argtypes = [C.c_void_p, _up, _dp, C.c_int]
         L.or_step.restype = C.c_int
+        L.or_shake_positions.argtypes = [C.c_void_p, _up, _up]
+        L.or_shake_positions.restype = C.c_int
         L.or_maxwell.argtypes = [C.c_void_p, _up, C.c_double, C.c_uint64, C.c_uint64, _dp]
         L.or_kinetic.argtypes = [C.c_void_p, _dp]
         L.or_kinetic.restype = C.c_double
@@ -183,6 +185,62 @@ class Oracle:
         v = np.ascontiguousarray(v, dtype=np.float64).copy()
         rc = lib().or_step(self.h, _p(u, C.c_uint64), _p(v, C.c_double), int(nsteps))
         return u, v, rc
+
+    def shake_positions(self, uold, u):
+        u = np.ascontiguousarray(u, dtype=np.uint64).copy()
+        uold = np.ascontiguousarray(uold, dtype=np.uint64)
+        rc = lib().or_shake_positions(self.h, _p(uold, C.c_uint64), _p(u, C.c_uint64))
+        return u, rc
+
+    def minimize_ref(self, u, nsteps, method=1, cgfac=1e-10, sdfac=1e-4, cgrate=0.1, sdrate=0.01):
+        """The reference's minimisation schedule (MinimizeStep, src/dynamics.cu:491-620;
+        ConjugatedGradient / SteepestDescent, src/mdcuda.cu:126-330; factor ramp,
+        src/fenzi.cu:402-415) in FP64, with this build's SHAKE after every move:
+        method 1 moves r += H, H = (F + gamma H_prev) cgfac, gamma = |F|^2/|F_prev|^2
+        per atom (F_prev taken as F when |F_prev|^2 < 1e-8); method 2 moves
+        r += F sdfac.  An energy above the last accepted one (the first move is
+        always accepted) undoes the move and demotes 1 -> 2 -> 0 (stop).  After
+        the accepted moves with 0-based index 100, 200, ... the active factor is
+        multiplied by rate * 100.  Returns (u, method, accepted moves)."""
+        sc = np.array(self.box) / 2.0 ** 64
+        u = np.ascontiguousarray(u, dtype=np.uint64).copy()
+        H = np.zeros((self.n, 3))
+        Fp = np.zeros((self.n, 3))
+        f, _, _ = self.forces(u)
+        e_prev, have, done = 0.0, False, 0
+        for _ in range(nsteps):
+            if method == 0:
+                break
+            uold = u.copy()
+            if method == 1:
+                f2 = (f * f).sum(1)
+                p2 = (Fp * Fp).sum(1)
+                p2 = np.where(p2 < 1e-8, f2, p2)
+                g = np.where(p2 > 0, f2 / np.where(p2 > 0, p2, 1.0), 0.0)
+                H = (f + g[:, None] * H) * cgfac
+                Fp = f.copy()
+                h = H
+            else:
+                h = f * sdfac
+            u = u + np.rint(h / sc).astype(np.int64).view(np.uint64)
+            u, _ = self.shake_positions(uold, u)
+            fn, e, _ = self.forces(u)
+            pot = sum(e.values())
+            if have and pot > e_prev:
+                u = uold
+                method = 2 if method == 1 else 0
+                f, _, _ = self.forces(u)
+                continue
+            f = fn
+            e_prev, have = pot, True
+            idx = done
+            done += 1
+            if idx > 0 and idx % 100 == 0:
+                if method == 1:
+                    cgfac *= cgrate * 100.0
+                else:
+                    sdfac *= sdrate * 100.0
+        return u, method, done
 
     def maxwell(self, u, T, seed, counter=0):
         u = np.ascontiguousarray(u, dtype=np.uint64)

@@ -189,6 +189,35 @@ def test_minimize_lowers_energy_and_keeps_constraints(ctx, prec):
     assert systems.rel_rms(f1, f_ref) < (1e-10 if prec == fenzi.FP64 else 1e-5)
 
 
+@pytest.mark.parametrize("prec,tol", [(fenzi.FP64, 1e-7), (fenzi.MIXED, 1e-4)])
+@pytest.mark.parametrize("stages", [
+    [(1, 2e-6, 1e-6, 40)],
+    [(2, 1e-10, 3e-6, 40)],
+    # a second call whose conjugate-gradient factor overshoots: one move is undone
+    # and the schedule continues by steepest descent
+    [(1, 5e-5, 2e-6, 30), (1, 1.5e-3, 2e-4, 40)],
+])
+def test_reference_minimiser_schedule_vs_oracle(ctx, prec, tol, stages):
+    """fz_minimize_ex (conjugate gradient -> steepest descent schedule) follows
+    the oracle's restatement move for move: same final positions."""
+    topo, xyz, params = systems.slab(n_chains=3, n_waters=250, L=22.0, seed=12, K=(24, 24, 24))
+    orc = oracle.Oracle(topo, params)
+    u_ref = orc.to_fixed(xyz)
+    s = fenzi.System(ctx, topo, params, prec)
+    s.set_fixed(u_ref)
+    _, e0 = s.forces()
+    for method, cgfac, sdfac, n in stages:
+        s.minimize_ex(n, fenzi.min_params(method, cgfac=cgfac, sdfac=sdfac, cgrate=0.1, sdrate=0.01))
+        u_ref, _, done = orc.minimize_ref(u_ref, n, method, cgfac, sdfac, 0.1, 0.01)
+        assert done > 0
+        d = (s.get_fixed() - u_ref).view(np.int64).astype(np.float64) * (params["box"][0] / 2.0 ** 64)
+        assert np.abs(d).max() < tol
+    _, e1 = s.forces()
+    assert e1["potential"] < e0["potential"]
+    assert np.abs(s.get_state()[1]).max() == 0.0
+    s.close()
+
+
 @pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
 def test_reference_deck_forces_vs_oracle(ctx, prec, tmp_path):
     """The reference's own dmpc_small input (CHARMM lipid + SPC/Fw water, with

@@ -239,7 +239,20 @@ def test_cluster_pairs_cover_exactly_the_pairs_within_list_distance():
                 key = (min(a, b), max(a, b))
                 assert key not in covered
                 covered.add(key)
-    assert len(covered) == o.count_pairs_within(u, params["pairlist"])
+    # brute force in FP64 (water only: a pair is excluded iff both atoms are in one
+    # molecule): every pair within the list distance is covered, and nothing beyond
+    # the FP32 test's 1e-6 relative margin is
+    n = topo["n_atoms"]
+    ia, ib = np.triu_indices(n, 1)
+    sc = np.array(params["box"]) / 2.0 ** 64
+    dd = ((u[ib] - u[ia]).view(np.int64)).astype(np.float64) * sc
+    r2 = (dd ** 2).sum(1)
+    keep = (ia // 3) != (ib // 3)
+    rl2 = params["pairlist"] ** 2
+    inner = set(zip(ia[keep & (r2 < rl2)].tolist(), ib[keep & (r2 < rl2)].tolist()))
+    outer = set(zip(ia[keep & (r2 < rl2 * (1 + 4e-6))].tolist(), ib[keep & (r2 < rl2 * (1 + 4e-6))].tolist()))
+    assert inner <= covered <= outer
+    assert len(inner) == o.count_pairs_within(u, params["pairlist"])
     # every atom sits in exactly one slot, in the column its x, y fractions name
     real = slots[slots >= 0]
     assert sorted(real.tolist()) == list(range(topo["n_atoms"]))
@@ -263,6 +276,76 @@ def test_shake_keeps_rigid_water_geometry():
         d2 = x[w + 2] - x[w + 1]; d2 -= box * np.round(d2 / box)
         assert abs(np.linalg.norm(d1) - fz_synth.R_OH) < 1e-8
         assert abs(np.linalg.norm(d2) - fz_synth.R_HH) < 1e-8
+
+
+def _two_atom_bond(k, r0, m=12.0):
+    return dict(n_atoms=2, n_types=1, charge=np.zeros(2), mass=np.full(2, m), type=np.zeros(2, np.int32),
+                lj_eps=np.zeros(1), lj_sig2=np.ones(1), lj14_eps=np.zeros(1), lj14_sig2=np.ones(1),
+                n_bonds=1, bond_atoms=np.array([0, 1], np.int32), bond_k=np.array([k]), bond_r0=np.array([r0]),
+                n_angles=0, angle_atoms=np.zeros(0, np.int32), angle_k=np.zeros(0), angle_theta0=np.zeros(0),
+                ub_k=np.zeros(0), ub_s0=np.zeros(0), n_dihedrals=0, dihedral_atoms=np.zeros(0, np.int32),
+                dihedral_k=np.zeros(0), dihedral_n=np.zeros(0, np.int32), dihedral_delta=np.zeros(0),
+                n_impropers=0, improper_atoms=np.zeros(0, np.int32), improper_k=np.zeros(0),
+                improper_psi0=np.zeros(0), n_constraints=0, constraint_atoms=np.zeros(0, np.int32),
+                constraint_length=np.zeros(0))
+
+
+def _bond_length_after_schedule(k, x0, nsteps, method, cgfac, sdfac, cgrate, sdrate):
+    """The minimiser schedule on one harmonic bond, as a scalar recurrence on the
+    stretch x = d - r0 (E = k x^2; the force on the far atom is f = -2 k x along
+    the bond, the near atom gets -f, so each move changes d by twice the far
+    atom's displacement)."""
+    x, hprev, fprev = x0, 0.0, 0.0
+    e_prev, have, done = 0.0, False, 0
+    for _ in range(nsteps):
+        if method == 0:
+            break
+        f = -2.0 * k * x
+        if method == 1:
+            p2 = fprev * fprev
+            if p2 < 1e-8:
+                p2 = f * f
+            g = f * f / p2 if p2 > 0 else 0.0
+            h = (f + g * hprev) * cgfac
+            hprev, fprev = h, f
+        else:
+            h = f * sdfac
+        xn = x + 2.0 * h
+        e = k * xn * xn
+        if have and e > e_prev:
+            method = 2 if method == 1 else 0
+            continue
+        x, e_prev, have = xn, e, True
+        idx = done
+        done += 1
+        if idx > 0 and idx % 100 == 0:
+            if method == 1:
+                cgfac *= cgrate * 100.0
+            else:
+                sdfac *= sdrate * 100.0
+    return x, method, done
+
+
+@pytest.mark.parametrize("method,cgfac,sdfac,nsteps", [
+    (2, 1e-10, 1e-3, 30),      # steepest descent: x_k = x_0 (1 - 4 k sdfac)^k
+    (2, 1e-10, 2e-5, 250),     # with two factor ramps (x10 after moves 100 and 200)
+    (1, 6e-3, 1e-3, 40),       # conjugate gradient overshoots, switches to steepest descent
+    (1, 1e-4, 1e-3, 30),       # conjugate gradient with gamma = 1 on a single bond
+])
+def test_reference_minimiser_schedule_on_one_bond(method, cgfac, sdfac, nsteps):
+    k, r0, x0 = 100.0, 1.5, 0.2
+    o = oracle.Oracle(_two_atom_bond(k, r0), dict(box=(30.0,) * 3, cutoff=9.0, cuton=8.0, pairlist=10.0,
+                                                  beta=0.3, K=(16, 16, 16), dt_ps=0.001))
+    u = o.to_fixed(np.array([[0.0, 0.0, 0.0], [r0 + x0, 0.0, 0.0]]))
+    u2, m2, done = o.minimize_ref(u, nsteps, method, cgfac, sdfac, 0.1, 0.01)
+    x2 = o.from_fixed(u2)
+    x_ref, m_ref, done_ref = _bond_length_after_schedule(k, x0, nsteps, method, cgfac, sdfac, 0.1, 0.01)
+    assert m2 == m_ref and done == done_ref
+    assert abs((x2[1, 0] - x2[0, 0] - r0) - x_ref) < 1e-9
+    if method == 2 and nsteps == 30:
+        assert abs(x_ref - x0 * (1 - 4 * k * sdfac) ** nsteps) < 1e-12
+    if method == 1 and cgfac == 4e-4:
+        assert m_ref == 2  # the overshoot did switch methods
 
 
 def test_harmonic_bond_oscillates_at_the_closed_form_frequency():
