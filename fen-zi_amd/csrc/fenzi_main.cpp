@@ -6,6 +6,7 @@
 // every outputTiming steps, then report the wall-clock rate (the reference
 // times with clock(), i.e. CPU time, src/fenzi.cu:441,557).
 // Extra flag: -p fp64|mixed selects the precision form (default fp64).
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -47,7 +48,23 @@ int main(int argc, char** argv) {
   if (fz_ctx_create(device, &ctx)) return die("device");
   fz_system* sys = nullptr;
   if (fz_system_create(ctx, fz_deck_topology(deck), fz_deck_params(deck), prec, &sys)) return die("system setup");
-  if (fz_set_state(sys, fz_deck_coordinates(deck), nullptr)) return die("initial state");
+  const fz_params* par = fz_deck_params(deck);
+  const int64_t n_atoms = fz_deck_topology(deck)->n_atoms;
+  // restart from <prefix>.rst when it exists (src/fenzi.cu:161-181)
+  const std::string rst = std::string(rs.file_prefix) + ".rst";
+  bool restart = false;
+  std::vector<double> rx((size_t)3 * n_atoms), rv((size_t)3 * n_atoms);
+  if (FILE* probe = fopen(rst.c_str(), "rb")) {
+    fclose(probe);
+    int64_t rstep = 0;
+    double rseed = 0, rbox[3];
+    if (fz_restart_read(rst.c_str(), rs.checkpoint_binary, n_atoms, &rstep, &rseed, rbox, rx.data(), rv.data()))
+      return die("reading the restart file");
+    rs.startstep = rstep;
+    restart = true;
+  }
+  if (fz_set_state(sys, restart ? rx.data() : fz_deck_coordinates(deck), restart ? rv.data() : nullptr))
+    return die("initial state");
   if (rs.minimizesteps > 0) {  // the reference minimises before dynamics (src/fenzi.cu:397-420)
     fz_min_params mp;
     mp.method = 1;
@@ -55,16 +72,25 @@ int main(int argc, char** argv) {
     mp.cgfac = rs.cgfac; mp.sdfac = rs.sdfac; mp.cgrate = rs.cgrate; mp.sdrate = rs.sdrate;
     if (fz_minimize_ex(sys, (int)rs.minimizesteps, &mp)) return die("minimization");
   }
-  uint64_t counter = 0;
-  if (fz_reassign_velocities(sys, rs.temperature, rs.seed, counter++)) return die("velocities");
+  // velocity draws are counted from the run's first step, so a restarted run
+  // continues the same sequence
+  uint64_t counter = rs.tempreset_every > 0 ? (uint64_t)(rs.startstep / rs.tempreset_every) + 1 : 1;
+  if (!restart && fz_reassign_velocities(sys, rs.temperature, rs.seed, 0)) return die("velocities");
   FILE* out = stdout;
   std::string outpath;
   if (!rs.print_to_stdout) {
-    outpath = std::string(rs.output_prefix[0] ? rs.output_prefix : "fenzi") + "_dev" + std::to_string(device) + ".out";
-    out = fopen(outpath.c_str(), "w");
+    outpath = std::string(rs.file_prefix) + "_dev" + std::to_string(device) + ".out";
+    out = fopen(outpath.c_str(), restart ? "a" : "w");
     if (!out) { fprintf(stderr, "fenzi: cannot open %s\n", outpath.c_str()); return 1; }
   }
-  const double dt = fz_deck_params(deck)->dt_ps;
+  const double dt = par->dt_ps;
+  fz_dcd* dcd = nullptr;
+  if (rs.print_level > 1) {  // trajectory (src/fenzi.cu:280-369)
+    const std::string path = std::string(rs.file_prefix) + "_dev" + std::to_string(device) + ".dcd";
+    if (fz_dcd_open(path.c_str(), n_atoms, rs.dynasteps / rs.traj_every, rs.traj_every, rs.traj_every, rs.dynasteps,
+                    dt, &dcd))
+      return die("trajectory");
+  }
   char buf[1024];
   fz_format_energy_header(buf, sizeof buf);
   fputs(buf, out);
@@ -75,15 +101,13 @@ int main(int argc, char** argv) {
   fputs(buf, out);
   const long every = rs.output_every > 0 ? rs.output_every : 100;
   long step = rs.startstep;
-  const long end = rs.startstep + rs.dynasteps;
+  // dynasteps is the last step number, also after a restart (src/fenzi.cu:442)
+  const long end = rs.dynasteps > rs.startstep ? rs.dynasteps : rs.startstep;
   auto t0 = std::chrono::steady_clock::now();
+  auto until = [&](long period) { return period > 0 ? period - step % period : end - step; };
   while (step < end) {
-    long chunk = every - step % every;
-    if (rs.tempreset_every > 0) {
-      long r = rs.tempreset_every - step % rs.tempreset_every;
-      if (r < chunk) chunk = r;
-    }
-    if (step + chunk > end) chunk = end - step;
+    long chunk = std::min({until(every), until(rs.tempreset_every), until(rs.checkpoint_every),
+                           dcd ? until(rs.traj_every) : end - step, end - step});
     if (fz_step(sys, (int)chunk)) return die("dynamics");
     step += chunk;
     if (step % every == 0 || step == end) {
@@ -93,9 +117,19 @@ int main(int argc, char** argv) {
       fputs(buf, out);
       fflush(out);
     }
+    const bool ckpt = rs.print_level > 0 && rs.checkpoint_every > 0 && step % rs.checkpoint_every == 0;
+    const bool frame = dcd && step % rs.traj_every == 0;
+    if (ckpt || frame) {
+      if (fz_get_state(sys, rx.data(), rv.data())) return die("state");
+      if (ckpt && fz_restart_write(rst.c_str(), rs.checkpoint_binary, n_atoms, step, (double)rs.seed, par->box,
+                                   rx.data(), rv.data()))
+        return die("checkpoint");
+      if (frame && fz_dcd_write_frame(dcd, rx.data(), par->box)) return die("trajectory");
+    }
     if (rs.tempreset_every > 0 && step % rs.tempreset_every == 0 && step < end)
       if (fz_reassign_velocities(sys, rs.temperature, rs.seed, counter++)) return die("velocities");
   }
+  if (dcd) fz_dcd_close(dcd);
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   long nsteps = end - rs.startstep;
   if (nsteps > 0) {

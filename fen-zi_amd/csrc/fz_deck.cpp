@@ -349,6 +349,8 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
     int shake = 0, printstdout = 1;
     double seed = 0;
     double cgfac = 1e-10, sdfac = 1e-4, cgrate = 0.1, sdrate = 0.01;
+    long ckpt_every = 100000, traj_every = -1;
+    int printlevel = 0, ckpt_binary = 1;
     double box[3] = {0, 0, 0};
     int K[3] = {0, 0, 0};
     std::string crd, psf, prm, rtf, outname;
@@ -389,11 +391,18 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
       } else if (k == "seed") seed = atof(val(1).c_str());
       else if (k == "printtostdout") printstdout = atoi(val(1).c_str());
       else if (k == "outputtrajectory" || k == "outputname") { if (outname.empty() || k == "outputtrajectory") outname = val(1); }
+      else if (k == "checkpointtiming") ckpt_every = atol(val(1).c_str());
+      else if (k == "trajfrequencytiming") traj_every = atol(val(1).c_str());
+      else if (k == "printlevel") printlevel = atoi(val(1).c_str());
+      else if (k == "checkpointformat") {
+        const char c = val(1).empty() ? 'b' : val(1)[0];
+        if (c == 'a' || c == 'A' || c == '1') ckpt_binary = 0;
+        else if (c == 'b' || c == 'B' || c == '2') ckpt_binary = 1;
+        else return set_error(FZ_ERR_PARSE, "checkpoint format \"%s\" is undefined (Ascii or Binary)", val(1).c_str());
+      }
       // accepted and not used by the force-and-integrate path
-      else if (k == "thermostat" || k == "conjgradfac" || k == "steepdesfac" || k == "conjgradrate" ||
-               k == "steepdesrate" || k == "inbfrq" || k == "checkpointtiming" || k == "gpublocksize" ||
-               k == "trajfrequencytiming" || k == "hfac" || k == "printlevel" || k == "keepcheckpoints" ||
-               k == "checkpointformat" || k == "restartfilename" || k == "pmass" || k == "pref" ||
+      else if (k == "thermostat" || k == "inbfrq" || k == "gpublocksize" || k == "hfac" ||
+               k == "keepcheckpoints" || k == "restartfilename" || k == "pmass" || k == "pref" ||
                k == "consharm" || k == "consharmdist" || k == "consharmfc" || k == "consfix") {
       } else {
         fprintf(stderr, "Ignoring unrecognized token, \"%s\"\n", t[0].c_str());
@@ -535,6 +544,14 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
     s.shake = shake; s.print_to_stdout = printstdout;
     snprintf(s.output_prefix, sizeof s.output_prefix, "%s", outname.c_str());
     s.cgfac = cgfac; s.sdfac = sdfac; s.cgrate = cgrate; s.sdrate = sdrate;
+    s.checkpoint_every = ckpt_every; s.traj_every = traj_every > 0 ? traj_every : outputTiming;
+    s.print_level = printlevel; s.checkpoint_binary = ckpt_binary;
+    {  // <outputtrajectory> without the extension after its last '/' (src/fenzi.cu:150-158)
+      std::string pfx = outname.empty() ? std::string("fenzi") : outname;
+      const size_t dot = pfx.rfind('.'), sl = pfx.rfind('/');
+      if (dot != std::string::npos && (sl == std::string::npos || dot > sl)) pfx.resize(dot);
+      snprintf(s.file_prefix, sizeof s.file_prefix, "%s", pfx.c_str());
+    }
     *out = guard.release();
     return FZ_OK;
   } catch (...) {

@@ -67,6 +67,16 @@ __device__ __forceinline__ R bspline4_k(R f, int k) {
 // cache lines) instead of 64 unrelated lines.  Slots are visited with a
 // multiplier coprime to their count (near 0.618 ns, set at each list build; a
 // bijection of [0, ns)), so that concurrent waves work on distant atoms.
+//
+// The wave's 64 atoms are read and placed on the grid in parallel first (lane k:
+// the k-th atom), so the loop over atoms carries no memory loads: per atom it
+// broadcasts the grid corner, the spline fractions and the charge, each lane
+// forms its point's weight and issues its atomic.
+__device__ __forceinline__ float bcast(float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); }
+__device__ __forceinline__ double bcast(double v, int k) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), k), __builtin_amdgcn_readlane(__double2loint(v), k));
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_spread(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using R = typename std::conditional<MIXED, float, double>::type;
@@ -78,22 +88,33 @@ __global__ __launch_bounds__(256) void k_spread(const SysDev* __restrict__ S, in
   const unsigned long long mult = (unsigned long long)d.n_clusters[1];
   const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
   const int ca = lane & 3, cb = (lane >> 2) & 3, cc = lane >> 4;
-  for (int k = 0; k < 64; k++) {
-    const int t = t0 + k;
-    if (t >= ns) break;
-    const int sl = (int)(((unsigned long long)t * mult) % (unsigned long long)ns);
+  // lane k: atom of visit t0 + k
+  R q = R(0), fx = R(0), fy = R(0), fz = R(0);
+  int gx0 = 0, gy0 = 0, gz0 = 0;
+  bool has = false;
+  if (t0 + lane < ns) {
+    const int sl = (int)(((unsigned long long)(t0 + lane) * mult) % (unsigned long long)ns);
     const int a = d.slot_atom[sl];
-    if (a < 0) continue;
-    const double qd = d.q[a];
-    if (qd == 0.0) continue;
-    R fx, fy, fz;
-    int gx = spline_pos<R>(d.ux[a], K0, fx) + ca;
-    int gy = spline_pos<R>(d.uy[a], K1, fy) + cb;
-    int gz = spline_pos<R>(d.uz[a], K2, fz) + cc;
+    if (a >= 0) {
+      q = (R)d.q[a];
+      has = q != R(0);
+      gx0 = spline_pos<R>(d.ux[a], K0, fx);
+      gy0 = spline_pos<R>(d.uy[a], K1, fy);
+      gz0 = spline_pos<R>(d.uz[a], K2, fz);
+    }
+  }
+  unsigned long long bal = __ballot(has);
+  while (bal) {
+    const int k = __ffsll((long long)bal) - 1;
+    bal &= bal - 1ull;
+    int gx = __builtin_amdgcn_readlane(gx0, k) + ca;
+    int gy = __builtin_amdgcn_readlane(gy0, k) + cb;
+    int gz = __builtin_amdgcn_readlane(gz0, k) + cc;
     gx -= gx >= K0 ? K0 : 0;
     gy -= gy >= K1 ? K1 : 0;
     gz -= gz >= K2 ? K2 : 0;
-    const R v = (R)qd * bspline4_k<R>(fx, ca) * bspline4_k<R>(fy, cb) * bspline4_k<R>(fz, cc);
+    const R v = bcast(q, k) * bspline4_k<R>(bcast(fx, k), ca) * bspline4_k<R>(bcast(fy, k), cb) *
+                bspline4_k<R>(bcast(fz, k), cc);
     const size_t g = ((size_t)gz * K1 + gy) * K0 + gx;
     if (MIXED) atomicAdd((unsigned long long*)d.Qacc + g, (unsigned long long)__float2ll_rn((float)v * 4294967296.0f));
     else unsafeAtomicAdd((double*)d.Qacc + g, (double)v);

@@ -63,7 +63,9 @@ class RunSettings(C.Structure):
                 ("startstep", C.c_int64), ("tempreset_every", C.c_int64), ("output_every", C.c_int64),
                 ("seed", C.c_uint64), ("shake", C.c_int32), ("print_to_stdout", C.c_int32),
                 ("output_prefix", C.c_char * 256), ("cgfac", C.c_double), ("sdfac", C.c_double),
-                ("cgrate", C.c_double), ("sdrate", C.c_double)]
+                ("cgrate", C.c_double), ("sdrate", C.c_double), ("checkpoint_every", C.c_int64),
+                ("traj_every", C.c_int64), ("print_level", C.c_int32), ("checkpoint_binary", C.c_int32),
+                ("file_prefix", C.c_char * 256)]
 
 
 # every symbol include/fenzi.h declares
@@ -76,7 +78,8 @@ EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create
            "fz_deck_psf_counts", "fz_format_energy_header", "fz_format_energy_row",
            "fz_server_create", "fz_server_submit", "fz_server_poll", "fz_server_job_info", "fz_server_job_result",
            "fz_server_release", "fz_server_estimate_bytes", "fz_server_destroy",
-           "fz_minimize_ex", "fz_batch_minimize_ex"]
+           "fz_minimize_ex", "fz_batch_minimize_ex", "fz_restart_write", "fz_restart_read", "fz_dcd_open",
+           "fz_dcd_write_frame", "fz_dcd_close"]
 
 MIN_ADAPTIVE_SD, MIN_REF_CG, MIN_REF_SD = 0, 1, 2
 
@@ -146,6 +149,13 @@ def lib():
         L.fz_batch_step.argtypes = [vp, C.c_int]
         L.fz_batch_minimize.argtypes = [vp, C.c_int, C.c_double]
         L.fz_minimize_ex.argtypes = [vp, C.c_int, C.POINTER(MinParams)]
+        L.fz_restart_write.argtypes = [C.c_char_p, C.c_int32, C.c_int64, C.c_int64, C.c_double, _dp, _dp, _dp]
+        L.fz_restart_read.argtypes = [C.c_char_p, C.c_int32, C.c_int64, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_double), _dp, _dp, _dp]
+        L.fz_dcd_open.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_double,
+                                  C.POINTER(vp)]
+        L.fz_dcd_write_frame.argtypes = [vp, _dp, _dp]
+        L.fz_dcd_close.argtypes = [vp]
         L.fz_batch_minimize_ex.argtypes = [vp, C.c_int, C.POINTER(MinParams)]
         L.fz_batch_destroy.argtypes = [vp]
         L.fz_deck_load.argtypes = [C.c_char_p, C.POINTER(vp)]
@@ -386,6 +396,44 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def restart_write(path, step, seed, box, xyz, vel, binary=True):
+    x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    _check(lib().fz_restart_write(path.encode(), int(binary), len(x), int(step), float(seed), _p(b, C.c_double),
+                                  _p(x, C.c_double), _p(v, C.c_double)), "fz_restart_write")
+
+
+def restart_read(path, n_atoms, binary=True):
+    x = np.empty((n_atoms, 3))
+    v = np.empty((n_atoms, 3))
+    b = np.empty(3)
+    st = C.c_int64()
+    sd = C.c_double()
+    _check(lib().fz_restart_read(path.encode(), int(binary), int(n_atoms), C.byref(st), C.byref(sd),
+                                 _p(b, C.c_double), _p(x, C.c_double), _p(v, C.c_double)), "fz_restart_read")
+    return dict(step=st.value, seed=sd.value, box=b, xyz=x, vel=v)
+
+
+class Dcd:
+    """CHARMM DCD trajectory writer (fz_dcd_* in include/fenzi.h)."""
+
+    def __init__(self, path, n_atoms, nframes, first_step, every, nsteps, dt_ps):
+        self.h = C.c_void_p()
+        _check(lib().fz_dcd_open(path.encode(), int(n_atoms), int(nframes), int(first_step), int(every), int(nsteps),
+                                 float(dt_ps), C.byref(self.h)), "fz_dcd_open")
+
+    def write(self, xyz, box):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        _check(lib().fz_dcd_write_frame(self.h, _p(x, C.c_double), _p(b, C.c_double)), "fz_dcd_write_frame")
+
+    def close(self):
+        if self.h:
+            _check(lib().fz_dcd_close(self.h), "fz_dcd_close")
+            self.h = C.c_void_p()
 
 
 def estimate_bytes(topo, params, precision=MIXED):

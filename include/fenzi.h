@@ -248,6 +248,24 @@ int fz_server_release(fz_server *srv, int64_t job_id);
 int64_t fz_server_estimate_bytes(const fz_topology *topo, const fz_params *par, int32_t precision);
 void fz_server_destroy(fz_server *srv);
 
+/* ---- checkpoint and trajectory files ------------------------------------------------ */
+/* Restart file in the reference's layouts (SaveCheckpointBinary/Ascii and
+ * LoadCheckpointBinary/Ascii, src/prmwrite.cu:31-252): atom count, step, seed,
+ * box, zero box velocities, then per atom position and velocity, single
+ * precision.  binary: 1 binary, 0 text. */
+int fz_restart_write(const char *path, int32_t binary, int64_t n_atoms, int64_t step, double seed,
+                     const double *box, const double *xyz, const double *vel);
+/* Fails (FZ_ERR_PARSE) on a short file or a different atom count.  Outputs nullable. */
+int fz_restart_read(const char *path, int32_t binary, int64_t n_atoms, int64_t *step, double *seed,
+                    double *box, double *xyz, double *vel);
+/* CHARMM DCD trajectory (charmm_trj_write_header/titles/natoms/frame,
+ * src/io/charmmio.c:50-175, Fortran records as ffwrite, src/io/ffile.c:155-203). */
+typedef struct fz_dcd fz_dcd;
+int fz_dcd_open(const char *path, int64_t n_atoms, int64_t nframes, int64_t first_step, int64_t every,
+                int64_t nsteps, double dt_ps, fz_dcd **out);
+int fz_dcd_write_frame(fz_dcd *f, const double *xyz, const double *box);
+int fz_dcd_close(fz_dcd *f);
+
 /* ---- input deck (.in / .psf / .prm / .rtf / .crd) --------------------------------- */
 /* Replaces ReadInput, ReadCrd/ReadPdb/ReadXYZ, ReadPsf_CHARMM, ReadPrm_CHARMM
  * (src/prmread.cu:491,331,410,285,1442,2181). */
@@ -265,6 +283,11 @@ typedef struct fz_run_settings {
   int32_t shake, print_to_stdout;
   char output_prefix[256];
   double cgfac, sdfac, cgrate, sdrate; /* conjgradfac, steepdesfac, conjgradrate, steepdesrate */
+  int64_t checkpoint_every;            /* checkpointTiming (default 100000) */
+  int64_t traj_every;                  /* trajfrequencyTiming (default outputTiming) */
+  int32_t print_level;                 /* printlevel: 0 none, 1 energies + checkpoints, 2 + trajectory */
+  int32_t checkpoint_binary;           /* checkpointformat: 1 binary (default), 0 ascii */
+  char file_prefix[256];               /* outputtrajectory without its extension: <prefix>.rst, <prefix>_dev<d>.dcd */
 } fz_run_settings;
 int fz_deck_settings(const fz_deck *deck, fz_run_settings *out);
 /* Section counts of the PSF as read: bonds, angles, dihedrals, impropers. */

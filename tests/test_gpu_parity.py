@@ -266,3 +266,30 @@ def test_fenzi_driver_runs_a_deck(tmp_path):
     assert [int(r[0]) for r in rows] == [0, 20, 40]
     assert all(len(r) == 16 for r in rows)
     assert "Ns per day" in out.stdout
+
+
+def test_fenzi_driver_checkpoints_restarts_and_writes_a_trajectory(tmp_path):
+    import subprocess
+    topo, xyz, params = systems.water_box(seed=21)
+    path = fz_synth.write_deck(topo, xyz, params, str(tmp_path), "ck", steps=40, output_every=20, reassign_every=0)
+    prefix = str(tmp_path / "ck_run")
+    with open(path, "a") as f:
+        f.write("outputtrajectory %s.dcd\nprintlevel 2\ncheckpointTiming 20\ntrajfrequencyTiming 20\n"
+                "checkpointformat binary\n" % prefix)
+    exe = os.path.join(os.path.dirname(fenzi.LIB_PATH), "fenzi")
+    out = subprocess.run([exe, "-d", "0", "-f", path, "-p", "mixed"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    n = topo["n_atoms"]
+    r40 = fenzi.restart_read(prefix + ".rst", n, binary=True)
+    assert r40["step"] == 40
+    raw = open(prefix + "_dev0.dcd", "rb").read()
+    assert len(raw) == (8 + 84) + (8 + 84) + (8 + 4) + 2 * ((8 + 48) + 3 * (8 + 4 * n))
+    # a second run with a later end step resumes from the checkpoint
+    txt = open(path).read().replace("dynasteps 40", "dynasteps 60")
+    open(path, "w").write(txt)
+    out = subprocess.run([exe, "-d", "0", "-f", path, "-p", "mixed"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [l.split() for l in out.stdout.splitlines()[1:] if l and l[0].isdigit()]
+    assert [int(r[0]) for r in rows] == [40, 60]
+    r60 = fenzi.restart_read(prefix + ".rst", n, binary=True)
+    assert r60["step"] == 60 and not np.array_equal(r60["xyz"], r40["xyz"])

@@ -7,6 +7,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 import fenzi
 import fz_synth
@@ -94,3 +95,87 @@ def test_invalid_inputs_are_rejected_without_a_device(tmp_path):
         assert "not found" in str(e)
     else:
         raise AssertionError("missing files accepted")
+
+
+# ---- checkpoint (.rst) and trajectory (.dcd) files -------------------------------------
+def _state(n=37, seed=3):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-20, 20, (n, 3)), rng.normal(0, 0.02, (n, 3)), np.array([40.5, 41.25, 60.0])
+
+
+def test_binary_restart_layout_and_round_trip(tmp_path):
+    import struct
+    x, v, box = _state()
+    p = str(tmp_path / "a.rst")
+    fenzi.restart_write(p, 1200, 2345.0, box, x, v, binary=True)
+    raw = open(p, "rb").read()
+    # SaveCheckpointBinary: int natom, int step, double seed, float3 box, float3 zero, 6 floats per atom
+    assert len(raw) == 4 + 4 + 8 + 12 + 12 + 24 * len(x)
+    n, step, seed = struct.unpack_from("<iid", raw, 0)
+    assert (n, step, seed) == (len(x), 1200, 2345.0)
+    assert np.allclose(struct.unpack_from("<3f", raw, 16), box.astype(np.float32), rtol=0, atol=0)
+    assert struct.unpack_from("<3f", raw, 28) == (0.0, 0.0, 0.0)
+    rec = np.frombuffer(raw, dtype="<f4", offset=40).reshape(-1, 6)
+    assert np.array_equal(rec[:, :3], x.astype(np.float32)) and np.array_equal(rec[:, 3:], v.astype(np.float32))
+    r = fenzi.restart_read(p, len(x), binary=True)
+    assert r["step"] == 1200 and r["seed"] == 2345.0
+    assert np.array_equal(r["xyz"], x.astype(np.float32).astype(np.float64))
+    assert np.array_equal(r["vel"], v.astype(np.float32).astype(np.float64))
+    with pytest.raises(fenzi.FenziError):
+        fenzi.restart_read(p, len(x) + 1, binary=True)
+    open(p, "wb").write(raw[:-10])
+    with pytest.raises(fenzi.FenziError):
+        fenzi.restart_read(p, len(x), binary=True)
+
+
+def test_ascii_restart_layout_and_round_trip(tmp_path):
+    x, v, box = _state(5, seed=4)
+    p = str(tmp_path / "a.rst")
+    fenzi.restart_write(p, 77, 12.5, box, x, v, binary=False)
+    lines = open(p).read().split("\n")
+    # SaveCheckpointAscii: "%12d \n", "%12d \n", "%le \n", box + zeros, "%10f" x 6 per atom
+    assert lines[0] == "%12d " % 5 and lines[1] == "%12d " % 77 and lines[2] == "%e " % 12.5
+    assert lines[3] == "%12f %12f %12f %12f %12f %12f" % (40.5, 41.25, 60.0, 0, 0, 0)
+    a = [float(t) for t in lines[4].split()]
+    assert np.allclose(a, np.concatenate([x[0], v[0]]), atol=6e-7)
+    r = fenzi.restart_read(p, 5, binary=False)
+    assert r["step"] == 77 and r["seed"] == 12.5
+    assert np.allclose(r["xyz"], x, atol=6e-7) and np.allclose(r["vel"], v, atol=6e-7)
+
+
+def _fortran_records(raw):
+    import struct
+    out, k = [], 0
+    while k < len(raw):
+        (n,) = struct.unpack_from("<I", raw, k)
+        body = raw[k + 4:k + 4 + n]
+        (m,) = struct.unpack_from("<I", raw, k + 4 + n)
+        assert m == n
+        out.append(body)
+        k += 8 + n
+    return out
+
+
+def test_dcd_layout(tmp_path):
+    import struct
+    x, _, box = _state(11, seed=5)
+    p = str(tmp_path / "t.dcd")
+    d = fenzi.Dcd(p, len(x), nframes=2, first_step=50, every=50, nsteps=100, dt_ps=0.002)
+    d.write(x, box)
+    d.write(x + 1.0, box)
+    d.close()
+    recs = _fortran_records(open(p, "rb").read())
+    assert len(recs) == 3 + 2 * 4
+    hdr = recs[0]
+    assert hdr[:4] == b"CORD" and len(hdr) == 84
+    ints = struct.unpack_from("<20i", hdr, 4)
+    assert ints[0:4] == (2, 50, 50, 100) and ints[7] == 1 and ints[8] == 0 and ints[10] == 1 and ints[19] == 34
+    assert abs(struct.unpack_from("<f", hdr, 4 + 9 * 4)[0] - 0.002 / 4.888821e-2) < 1e-7
+    assert struct.unpack_from("<i", recs[1], 0)[0] == 1 and len(recs[1]) == 84
+    assert struct.unpack_from("<i", recs[2], 0)[0] == 11
+    for f, shift in ((0, 0.0), (1, 1.0)):
+        cell = struct.unpack("<6d", recs[3 + 4 * f])
+        assert cell == (40.5, 0.0, 41.25, 0.0, 0.0, 60.0)
+        for k in range(3):
+            col = np.frombuffer(recs[4 + 4 * f + k], dtype="<f4")
+            assert np.array_equal(col, (x[:, k] + shift).astype(np.float32))
