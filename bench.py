@@ -54,16 +54,26 @@ def ns_day(ms, dt_ps):
     return 86400.0 / (ms * 1e-3) * dt_ps * 1e-3
 
 
-def prepare(ctx, cid, min_steps):
+def prepare(ctx, cid, min_steps, equil_steps=600):
+    """Generate, minimise (FP64 steepest descent), then equilibrate at 300 K with
+    velocities redrawn every 100 steps (mixed form), all untimed: the generated
+    structures release a lot of potential energy, and without this the timed
+    steps would run hot (more frequent list rebuilds than a 300 K system)."""
     t0 = time.perf_counter()
     topo, xyz, p = fz_synth.config(cid)
     s = fenzi.System(ctx, topo, p, fenzi.FP64, xyz=xyz)
     s.minimize(min_steps, 0.05)
     x, _ = s.get_state()
+    s.close()
+    s = fenzi.System(ctx, topo, p, fenzi.MIXED, xyz=x)
+    for k in range(0, equil_steps, 100):
+        s.reassign_velocities(p.get("temperature", 300.0), 99, k // 100)
+        s.step(min(100, equil_steps - k))
+    x, _ = s.get_state()
     e = s.energies()
     s.close()
-    log("workload %d: %d atoms generated+relaxed in %.1fs (Epot %.1f)" % (cid, topo["n_atoms"], time.perf_counter() - t0,
-                                                                        e["potential"]))
+    log("workload %d: %d atoms generated, relaxed and equilibrated in %.1fs (T %.0f K before the last redraw)" % (
+        cid, topo["n_atoms"], time.perf_counter() - t0, e["temperature"]))
     return topo, x, p
 
 
@@ -156,6 +166,19 @@ def roofline(run, prec):
             "flop_per_launch": flops, "pairs_per_launch": run["pairs_within_cutoff"],
             "launch_ms": run["phases_ms"]["pair"],
             "timing": "mean of 10 eagerly launched steps, HIP events on the launch stream around each phase"}
+
+
+def pme_hbm(run, n_atoms, K):
+    """Achieved bandwidth of the PME phases against their algorithmic bytes (FP32
+    form): spread 16N + 4K^3, FFT passes with the fused convolution 16K^3 (R2C
+    8K^3 + C2R 8K^3), gather 4K^3 + 28N (positions 16N, forces 12N)."""
+    K3 = float(K[0] * K[1] * K[2])
+    parts = {"pme_spread": 16 * n_atoms + 4 * K3, "pme_fft_conv": 16 * K3, "pme_gather": 4 * K3 + 28 * n_atoms}
+    out = {}
+    for k, b in parts.items():
+        t = run["phases_ms"][k] * 1e-3
+        out[k] = {"bytes": b, "ms": run["phases_ms"][k], "GB_per_s": b / t / 1e9, "frac_of_8TBps": b / t / 8e12}
+    return out
 
 
 def cpu_baseline(topo, x, p, budget_s=20.0):
@@ -344,7 +367,8 @@ def main():
     log("17K fp64 : %.3f ms/step %.1f ns/day" % (r1d["ms_per_step"], r1d["ns_per_day"]))
     err1 = compare_forms(ctx, topo1, x1, p1)
     extra = {"slab17k_mixed": r1m, "slab17k_fp64": r1d, "slab17k_mixed_vs_fp64": err1,
-             "slab17k_mixed_over_fp64": r1d["ms_per_step"] / r1m["ms_per_step"]}
+             "slab17k_mixed_over_fp64": r1d["ms_per_step"] / r1m["ms_per_step"],
+             "slab17k_mixed_pme_hbm": pme_hbm(r1m, topo1["n_atoms"], p1["K"])}
     if not a.no_large:
         topo2, x2, p2 = prepare(ctx, 2, a.min_steps)
         s2 = max(50, a.steps // 3)
@@ -359,7 +383,8 @@ def main():
         log("230K fp64 : %.3f ms/step %.1f ns/day" % (r2d["ms_per_step"], r2d["ns_per_day"]))
         extra.update({"slab230k_mixed": r2m, "slab230k_fp64": r2d,
                       "slab230k_mixed_vs_fp64": compare_forms(ctx, topo2, x2, p2),
-                      "slab230k_mixed_over_fp64": r2d["ms_per_step"] / r2m["ms_per_step"]})
+                      "slab230k_mixed_over_fp64": r2d["ms_per_step"] / r2m["ms_per_step"],
+                      "slab230k_mixed_pme_hbm": pme_hbm(r2m, topo2["n_atoms"], p2["K"])})
     if not a.no_water:
         topo0, x0, p0 = prepare(ctx, 0, 100)
         r0, s = md(ctx, topo0, x0, p0, fenzi.FP64, 1000, 50)

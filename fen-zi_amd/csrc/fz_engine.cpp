@@ -649,10 +649,10 @@ int Engine::ensure() {
   return stale ? prepare() : FZ_OK;
 }
 
-void Engine::record_forces(bool ev) {
+void Engine::record_forces(bool ev, bool lite) {
   launch_nblist(L_, true);
   launch_gather_sorted(L_);
-  launch_pair(L_, ev);
+  launch_pair(L_, ev ? (lite ? 1 : 2) : 0);
   launch_bonded(L_, ev);
   launch_pme(L_, ev, 0);
 }
@@ -714,7 +714,7 @@ int Engine::grow_and_reset(int si, int bits) {
   return FZ_OK;
 }
 
-int Engine::forces(bool ev) {
+int Engine::forces(bool ev, bool lite) {
   FZ_TRY(ensure());
   for (int attempt = 0; attempt < 4; attempt++) {
     for (auto* s : sys_) {
@@ -726,7 +726,7 @@ int Engine::forces(bool ev) {
     }
     launch_zero_acc(L_);
     launch_zero_forces(L_);
-    record_forces(ev);
+    record_forces(ev, lite);
     launch_kinetic(L_);
     FZ_HIP(hipGetLastError());
     FZ_TRY(read_status());
@@ -843,7 +843,7 @@ int Engine::minimize(int nsteps, double max_step) {
   int* d_reject = nullptr;
   FZ_HIP(hipMalloc(&d_reject, ns * sizeof(int)));
   struct Free { int* p; ~Free() { if (p) (void)hipFree(p); } } guard{d_reject};
-  FZ_TRY(forces(true));
+  FZ_TRY(forces(true, true));
   fz_energies e;
   for (int s = 0; s < ns; s++) { sys_[s]->energies(&e); e_old[s] = e.potential; }
   for (int it = 0; it < nsteps; it++) {
@@ -851,7 +851,7 @@ int Engine::minimize(int nsteps, double max_step) {
       FZ_HIP(hipMemcpyAsync(sys_[s]->dv.mstep, &step[s], sizeof(double), hipMemcpyHostToDevice, L_.stream));
     launch_min_move(L_);
     FZ_HIP(hipGetLastError());
-    FZ_TRY(forces(true));
+    FZ_TRY(forces(true, true));
     bool any = false;
     for (int s = 0; s < ns; s++) {
       sys_[s]->energies(&e);
@@ -862,7 +862,7 @@ int Engine::minimize(int nsteps, double max_step) {
     if (any) {
       FZ_HIP(hipMemcpyAsync(d_reject, reject.data(), ns * sizeof(int), hipMemcpyHostToDevice, L_.stream));
       launch_min_restore(L_, d_reject);
-      FZ_TRY(forces(true));
+      FZ_TRY(forces(true, true));
     }
   }
   // the adaptive step carries over to the next call, so a minimisation split
@@ -894,7 +894,7 @@ int Engine::minimize_ref(int nsteps, const fz_min_params& p) {
     FZ_HIP(hipMemsetAsync(s->dv.mH, 0, sizeof(double) * 3 * s->topo.n, L_.stream));
     FZ_HIP(hipMemsetAsync(s->dv.mF, 0, sizeof(double) * 3 * s->topo.n, L_.stream));
   }
-  FZ_TRY(forces(true));
+  FZ_TRY(forces(true, true));
   fz_energies e;
   for (int it = 0; it < nsteps; it++) {
     bool active = false;
@@ -907,7 +907,7 @@ int Engine::minimize_ref(int nsteps, const fz_min_params& p) {
     if (!active) break;
     launch_min_ref_move(L_);
     FZ_HIP(hipGetLastError());
-    FZ_TRY(forces(true));
+    FZ_TRY(forces(true, true));
     bool any = false;
     for (int s = 0; s < ns; s++) {
       reject[s] = 0;
@@ -930,7 +930,7 @@ int Engine::minimize_ref(int nsteps, const fz_min_params& p) {
     if (any) {
       FZ_HIP(hipMemcpyAsync(d_reject, reject.data(), ns * sizeof(int), hipMemcpyHostToDevice, L_.stream));
       launch_min_restore(L_, d_reject);
-      FZ_TRY(forces(true));
+      FZ_TRY(forces(true, true));
     }
   }
   launch_zero_vel(L_);
@@ -955,7 +955,7 @@ int Engine::phase_times(int nsteps, double* out) {
     hipEventRecord(ev[2], L_.stream);
     launch_gather_sorted(L_);
     hipEventRecord(ev[3], L_.stream);
-    launch_pair(L_, false);
+    launch_pair(L_, 0);
     hipEventRecord(ev[4], L_.stream);
     launch_bonded(L_, false);
     hipEventRecord(ev[5], L_.stream);

@@ -86,7 +86,9 @@ class Engine {
   Engine(int device, std::vector<System*> sys);
   ~Engine();
   int prepare();                       // work tables, descriptors, streams
-  int forces(bool ev);                 // stand-alone evaluation at current positions
+  // stand-alone evaluation at current positions; lite: pair energies summed in FP32
+  // (the minimisers' accept/reject tests), otherwise in FP64
+  int forces(bool ev, bool lite = false);
   int steps(int n);                    // n MD steps (energies on the last)
   int maxwell(double T, uint64_t seed, uint64_t counter);
   int minimize(int nsteps, double max_step);  // adaptive steepest descent, velocities zeroed
@@ -120,7 +122,7 @@ class Engine {
   int make_work(Work& w, const std::vector<int>& blocks);
   int capture(bool ev, hipGraphExec_t* out);
   void record_step(bool ev);
-  void record_forces(bool ev);
+  void record_forces(bool ev, bool lite = false);
   int grow_and_reset(int s, int bits);
   void drop_graphs();
   int upload_desc();

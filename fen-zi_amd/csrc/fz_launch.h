@@ -34,7 +34,7 @@ struct Launch {
 
 void launch_nblist(const Launch& L, bool always_rebuild_check);
 void launch_gather_sorted(const Launch& L);
-void launch_pair(const Launch& L, bool ev);
+void launch_pair(const Launch& L, int ev);  // 0 forces, 1 + FP32 energies, 2 + FP64 energies
 void launch_bonded(const Launch& L, bool ev);
 void launch_pme(const Launch& L, bool ev, int conv_mode);
 void launch_pme_fft_only(const Launch& L, int conv_mode);

@@ -211,7 +211,10 @@ struct EvAcc {
 };
 
 // ---------------------------------------------------------------------------
-template <bool EV>
+// EV: 0 forces only; 1 also energies and virial summed from the FP32 pair terms
+// (the minimisers' accept/reject tests); 2 energies recomputed in FP64 from the
+// FP32 distance (reported energies)
+template <int EV>
 __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S, int nsys, const int* __restrict__ chunk_pre) {
   const int lane = threadIdx.x & 63, il = lane & 7, jl = lane >> 3;
   const int total = chunk_pre[nsys];
@@ -329,7 +332,15 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
       fix -= fx; fiy -= fy; fiz -= fz;
       const float jx = red8(fx), jy = red8(fy), jz = red8(fz);
       store_j_g(F, cap, J, mask, lane, jx, jy, jz);
-      if (EV) {
+      if (EV == 1) {
+        if (act) {
+          cev[0] += (double)e_lj;
+          cev[1] += (double)e_c;
+          ev.npr++;
+          cev[2] += (double)(fx * dx); cev[3] += (double)(fy * dy); cev[4] += (double)(fz * dz);
+          cev[5] += (double)(fx * dy); cev[6] += (double)(fx * dz); cev[7] += (double)(fy * dz);
+        }
+      } else if (EV == 2) {
         if (act) {
           // energies of an evaluation step in FP64 from the same FP32 distance, so
           // that no FP32 bias of the erfc/exp approximations sums over millions of pairs
@@ -459,11 +470,12 @@ __global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, 
   if (EV && evs >= 0) ev.flush(S[evs], lane);
 }
 
-void launch_pair(const Launch& L, bool ev) {
+void launch_pair(const Launch& L, int ev) {
   dim3 g(L.pair_grid), b(256);
   if (L.mixed) {
-    if (ev) hipLaunchKernelGGL(k_pair_mixed<true>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
-    else hipLaunchKernelGGL(k_pair_mixed<false>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+    if (ev == 2) hipLaunchKernelGGL(k_pair_mixed<2>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+    else if (ev == 1) hipLaunchKernelGGL(k_pair_mixed<1>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+    else hipLaunchKernelGGL(k_pair_mixed<0>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
   } else {
     if (ev) hipLaunchKernelGGL(k_pair_f64<true>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
     else hipLaunchKernelGGL(k_pair_f64<false>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
