@@ -610,9 +610,9 @@ int Engine::prepare() {
   for (auto* s : sys_) kmax = std::max({kmax, s->dv.K[0], s->dv.K[1], s->dv.K[2]});
   const int csz = L_.mixed ? 8 : 16;
   int NL = tuned_nl_ > 0 ? tuned_nl_ : 16;
-  while (NL > 1 && (size_t)NL * 2 * kmax * csz > 49152) NL >>= 1;
+  while (NL > 1 && (size_t)(NL * 2 + 1) * kmax * csz > 49152) NL >>= 1;
   L_.fft_nl = NL;
-  L_.fft_lds = NL * 2 * kmax * csz;
+  L_.fft_lds = (NL * 2 + 1) * kmax * csz;  // two line buffers + the axis' twiddles
   for (auto* s : sys_) {
     const SysDev& d = s->dv;
     atoms.push_back((d.n_atoms + 255) / 256);

@@ -167,18 +167,25 @@ __device__ __forceinline__ void dft_small(C* v, int Rdx, R dir) {
 
 // nl lines of length K stored line-major in src; returns the buffer holding the
 // (unnormalised) transform.  dir = -1 forward, +1 inverse.  tw[k] = e^{-2 pi i k/K}.
+// floor(t / d) for t < 2^16 and 2 <= d <= 4096 as mulhi(t, floor((2^32 - 1) / d) + 1)
+__device__ __forceinline__ int fdiv(int t, int d, unsigned m) { return d == 1 ? t : (int)__umulhi((unsigned)t, m); }
+__device__ __forceinline__ unsigned fdiv_magic(int d) { return d == 1 ? 0u : 0xFFFFFFFFu / (unsigned)d + 1u; }
+
+// tw: the axis' twiddles, staged in LDS by the caller
 template <class C, class R>
-__device__ C* fft_lines(C* src, C* dst, int K, int nl, const C* __restrict__ tw, R dir) {
+__device__ C* fft_lines(C* src, C* dst, int K, int nl, const C* tw, R dir) {
   int Ns = 1, rem = K;
   while (rem > 1) {
     const int Rdx = (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : 5;
     const int nb = K / Rdx;
     const int step = K / (Ns * Rdx);
+    const unsigned m_nb = fdiv_magic(nb), m_ns = fdiv_magic(Ns);
     for (int t = threadIdx.x; t < nl * nb; t += blockDim.x) {
-      const int line = t / nb, j = t - line * nb;
+      const int line = fdiv(t, nb, m_nb), j = t - line * nb;
       const C* in = src + line * K;
       C* out = dst + line * K;
-      const int k = j % Ns;
+      const int jq = fdiv(j, Ns, m_ns);
+      const int k = j - jq * Ns;
       C v[5];
 #pragma unroll
       for (int r = 0; r < 5; r++)
@@ -191,7 +198,7 @@ __device__ C* fft_lines(C* src, C* dst, int K, int nl, const C* __restrict__ tw,
           v[r] = cmul(v[r], w);
         }
       dft_small<C, R>(v, Rdx, dir);
-      const int base = (j / Ns) * Ns * Rdx + k;
+      const int base = jq * Ns * Rdx + k;
 #pragma unroll
       for (int r = 0; r < 5; r++)
         if (r < Rdx) out[base + r * Ns] = v[r];
@@ -202,6 +209,13 @@ __device__ C* fft_lines(C* src, C* dst, int K, int nl, const C* __restrict__ tw,
     rem /= Rdx;
   }
   return src;
+}
+
+// the axis' twiddles (K entries) into LDS after the two line buffers
+template <class C>
+__device__ __forceinline__ const C* stage_tw(C* lds, const C* __restrict__ tw, int K) {
+  for (int k = threadIdx.x; k < K; k += blockDim.x) lds[k] = tw[k];
+  return lds;
 }
 
 // x pass, forward: real accumulators -> half spectrum; clears the accumulators
@@ -234,8 +248,9 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const SysDev* __restrict__ S,
     }
     b0[e] = v;
   }
+  const C* twl = stage_tw(b1 + NL * K0, (const C*)d.tw, K0);
   __syncthreads();
-  C* res = fft_lines<C, R>(b0, b1, K0, NL, (const C*)d.tw, R(-1));
+  C* res = fft_lines<C, R>(b0, b1, K0, NL, twl, R(-1));
   C* Qc = (C*)d.Qc;
   for (int e = threadIdx.x; e < NL * Kxh; e += 256) {
     const int line = e / Kxh, kx = e - line * Kxh;
@@ -264,8 +279,9 @@ __global__ __launch_bounds__(256) void k_fft_y(const SysDev* __restrict__ S, int
     if (kx0 + l < Kxh) v = Qc[((size_t)z * K1 + y) * Kxh + kx0 + l];
     b0[l * K1 + y] = v;
   }
+  const C* twl = stage_tw(b1 + NL * K1, (const C*)d.tw + d.K[0], K1);
   __syncthreads();
-  C* res = fft_lines<C, R>(b0, b1, K1, NL, (const C*)d.tw + d.K[0], R(dirv));
+  C* res = fft_lines<C, R>(b0, b1, K1, NL, twl, R(dirv));
   for (int e = threadIdx.x; e < NL * K1; e += 256) {
     const int y = e / NL, l = e - y * NL;
     if (kx0 + l < Kxh) Qc[((size_t)z * K1 + y) * Kxh + kx0 + l] = res[l * K1 + y];
@@ -295,8 +311,8 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(const SysDev* __restrict__ S
     if (kx0 + l < Kxh) v = Qc[(size_t)z * zs + (size_t)y * Kxh + kx0 + l];
     b0[l * K2 + z] = v;
   }
+  const C* twz = stage_tw(b1 + NL * K2, (const C*)d.tw + K0 + K1, K2);
   __syncthreads();
-  const C* twz = (const C*)d.tw + K0 + K1;
   C* res = fft_lines<C, R>(b0, b1, K2, NL, twz, R(-1));
   double ev[10];
 #pragma unroll
@@ -369,8 +385,9 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const SysDev* __restrict__ S,
     }
     b0[e] = v;
   }
+  const C* twl = stage_tw(b1 + NL * K0, (const C*)d.tw, K0);
   __syncthreads();
-  C* res = fft_lines<C, R>(b0, b1, K0, NL, (const C*)d.tw, R(1));
+  C* res = fft_lines<C, R>(b0, b1, K0, NL, twl, R(1));
   R* phi = (R*)d.phi;
   for (int e = threadIdx.x; e < NL * K0; e += 256) {
     const int line = e / K0;
