@@ -605,6 +605,8 @@ int Engine::prepare() {
   L_.mixed = sys_[0]->mixed;
   for (auto* s : sys_)
     if (s->mixed != L_.mixed) return set_error(FZ_ERR_ARG, "a batch must use one precision");
+  L_.lj_sep = 1;
+  for (auto* s : sys_) L_.lj_sep &= s->dv.lj_sep;
   std::vector<int> atoms, slots, cols, clus, terms, ccl, fx_, fy_, fz_;
   int kmax = 4;
   for (auto* s : sys_) kmax = std::max({kmax, s->dv.K[0], s->dv.K[1], s->dv.K[2]});

@@ -30,6 +30,7 @@ struct Launch {
   int pair_grid = 0;         // persistent pair-kernel blocks
   int fft_lds = 0;           // dynamic LDS bytes for the FFT kernels
   int fft_nl = 16;           // lines per FFT workgroup
+  int lj_sep = 0;            // every system follows the LJ combination rule
 };
 
 void launch_nblist(const Launch& L, bool always_rebuild_check);

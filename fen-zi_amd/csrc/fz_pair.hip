@@ -120,21 +120,45 @@ __device__ __forceinline__ void atomic_add_g(GU64* p, unsigned long long v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// store_j for the mixed form's fixed-point accumulators
-__device__ __forceinline__ void store_j_g(GU64* F, int cap, int J, unsigned long long mask, int lane, float jx,
-                                          float jy, float jz) {
-  const int c = lane >> 3, jj = lane & 7;
-  const int src = jj << 3;
-  const float gx = __shfl(jx, src, 64), gy = __shfl(jy, src, 64), gz = __shfl(jz, src, 64);
-  const float v = c == 0 ? gx : (c == 1 ? gy : gz);
-  if (lane < 24 && ((mask >> (8 * jj)) & 0xffull))
-    atomic_add_g(F + (size_t)c * cap + J * CLUSTER + jj, to_fixed_acc((double)v));
+// Raw buffer views of the slot arrays: a load is (SGPR descriptor, SGPR byte
+// offset of the cluster, VGPR byte offset of the lane), with no 64-bit address
+// arithmetic on the vector ALU.  gfx9 descriptor word 3 (raw, 32-bit format).
+// On gfx9 the scalar offset is not part of the range check, so num_records
+// only covers the lane offsets; the caller's indices are in range.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ v4i buf_ld4(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ v2f buf_ld2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(v2f, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
 
-__device__ __forceinline__ void add3_g(GU64* F, int cap, int slot, double fx, double fy, double fz) {
-  atomic_add_g(F + slot, to_fixed_acc(fx));
-  atomic_add_g(F + cap + slot, to_fixed_acc(fy));
-  atomic_add_g(F + 2 * cap + slot, to_fixed_acc(fz));
+// Fixed point (2^32) of an FP32 value: round-to-nearest-even of v * 2^32 as an
+// int64, the same value to_fixed_acc((double)v) gives.  For |v| < 2^19 the
+// FP64 sum v * 2^32 + 1.5 * 2^52 is exact before its single rounding to an
+// integer, which it keeps in the low mantissa bits: its bit pattern minus that
+// of 1.5 * 2^52 is the result.  The same sum scaled by 2^-32, v + 1.5 * 2^20,
+// has the same bits and needs no multiply (one FP64 add, one subtract).  A wave with
+// a larger value (a minimiser's first steps from overlapping atoms) takes the
+// general conversion.
+__device__ __forceinline__ unsigned long long fixed_f32(float v) {
+  if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(__builtin_fabsf(v) < 524288.0f)) == 0ull, 1)) {
+    // 1.5 * 2^20 + v lies in [2^20, 2^21), where the FP64 spacing is 2^-32
+    const double w = (double)v + 1572864.0;
+    return (unsigned long long)__double_as_longlong(w) - 0x4138000000000000ull;
+  }
+  return to_fixed_acc((double)v);
+}
+
+// Lanes 8k + c (c < 3) of a 64-bit lane mask whose byte k is not zero.
+__device__ __forceinline__ unsigned long long byte_any3(unsigned long long a) {
+  a |= a >> 4;
+  a |= a >> 2;
+  a |= a >> 1;
+  a &= 0x0101010101010101ull;
+  return a | (a << 1) | (a << 2);
 }
 
 template <class T>
@@ -143,6 +167,44 @@ __device__ __forceinline__ T red_stride8(T v) {  // sum over lanes sharing lane 
   v += __shfl_xor(v, 16, 64);
   v += __shfl_xor(v, 32, 64);
   return v;
+}
+
+// i forces of cluster I at the end of its run of entries: sums over the lanes
+// sharing lane & 7; lane i + 8c (c < 3) adds component c of slot i
+__device__ __forceinline__ void flush_i(FZ_GLOBAL char* Fb, unsigned cap8, int I, int lane, float fix, float fiy,
+                                        float fiz) {
+  const float rx = red_stride8(fix), ry = red_stride8(fiy), rz = red_stride8(fiz);
+  const int c = lane >> 3;
+  const float v = c == 0 ? rx : (c == 1 ? ry : rz);
+  if (lane < 24)
+    atomic_add_g((GU64*)(Fb + (size_t)I * (CLUSTER * 8) + (unsigned)c * cap8 + (unsigned)(lane & 7) * 8u), fixed_f32(v));
+}
+
+// Sums of three values over the 8 lanes sharing lane >> 3, every lane of the
+// group getting them: nine v_add_f32 with a DPP operand (xor 1, xor 2, mirror
+// within 8).  Written out because the compiler fuses the first add with the
+// product that feeds it (a mov_dpp + fma pair per value) and sinks the last
+// stage into the store's branch (mov_dpp + add).  s_nop 1 gives the two wait
+// states a DPP read needs after the VALU write of its operand; inside the block
+// each read comes two instructions after the write.  Called with all lanes on.
+__device__ __forceinline__ void red8x3(float& x, float& y, float& z) {
+  float a, b, c;
+  __asm__(
+      "s_nop 1\n\t"
+      "v_add_f32_dpp %0, %3, %3 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_add_f32_dpp %1, %4, %4 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_add_f32_dpp %2, %5, %5 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_add_f32_dpp %0, %0, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_add_f32_dpp %1, %1, %1 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_add_f32_dpp %2, %2, %2 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_add_f32_dpp %0, %0, %0 row_half_mirror row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_add_f32_dpp %1, %1, %1 row_half_mirror row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+      "v_add_f32_dpp %2, %2, %2 row_half_mirror row_mask:0xf bank_mask:0xf bound_ctrl:1"
+      : "=&v"(a), "=&v"(b), "=&v"(c)
+      : "v"(x), "v"(y), "v"(z));
+  x = a;
+  y = b;
+  z = c;
 }
 
 __device__ __forceinline__ float erfc_fit(float x, float ex) {
@@ -213,8 +275,9 @@ struct EvAcc {
 // ---------------------------------------------------------------------------
 // EV: 0 forces only; 1 also energies and virial summed from the FP32 pair terms
 // (the minimisers' accept/reject tests); 2 energies recomputed in FP64 from the
-// FP32 distance (reported energies)
-template <int EV>
+// FP32 distance (reported energies).  SEP: every system of the launch follows
+// the combination rule (per-slot LJ parameters, no table lookup).
+template <int EV, bool SEP>
 __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S, int nsys, const int* __restrict__ chunk_pre) {
   const int lane = threadIdx.x & 63, il = lane & 7, jl = lane >> 3;
   const int total = chunk_pre[nsys];
@@ -235,16 +298,27 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
     const long long np = *d.n_pairs;
     const long long e0 = (long long)(w - chunk_pre[s]) * PAIR_CHUNK;
     const int ne = rfl((int)(np - e0 < PAIR_CHUNK ? np - e0 : PAIR_CHUNK));  // uniform: scalar loop control
-    const GI4* xs = (const GI4*)d.xs;
-    const GF2* lp = (const GF2*)d.lp;
+    // slot data of cluster C at (uniform base + C * stride) + per-lane offset:
+    // global loads with an SGPR base and a 32-bit VGPR offset
+    const unsigned cap = (unsigned)d.n_slots_cap;
+    const __amdgpu_buffer_rsrc_t xsr = raw_rsrc(d.xs, cap * 16u);
+    const __amdgpu_buffer_rsrc_t lpr = raw_rsrc(d.lp, cap * 8u);
     const GF2* lj = (const GF2*)d.lj32;
-    const int sep = d.lj_sep;
+    const int sep = SEP || d.lj_sep;
     const int T = d.n_types;
-    GU64* F = (GU64*)d.Fs;
-    const int cap = d.n_slots_cap;
+    FZ_GLOBAL char* Fb = (FZ_GLOBAL char*)d.Fs;
+    const unsigned cap8 = cap * 8u;
+    // force atomics: lane 8k + c (c < 3) adds component c of slot k of a cluster
+    const unsigned foff = (unsigned)(lane & 7) * cap8 + (unsigned)(lane >> 3) * 8u;
     const float sx = d.sc32[0], sy = d.sc32[1], sz = d.sc32[2];
-    const float rc2 = (float)d.rc2, ron2 = (float)d.ron2, beta = (float)d.beta, swi = (float)d.sw_inv;
+    const float rc2 = (float)d.rc2, ron2 = (float)d.ron2;
+    // CHARMM switch S(r2) = (rc2 - r2)^2 (rc2 + 2 r2 - 3 ron2) / (rc2 - ron2)^3 in the
+    // form 6 S = c1^2 (sw_a r2 + sw_b), c1 = rc2 - r2, and -2 dS/dr2 = sw_d c1 (r2 - ron2);
+    // r2 is clamped to ron2 below the switching range, where 6 S = 6 and dS = 0
+    const float sw_a = (float)(12.0 * d.sw_inv), sw_b = (float)(6.0 * d.sw_inv * (d.rc2 - 3.0 * d.ron2));
+    const float sw_d = (float)(12.0 * d.sw_inv);
     const float cc = (float)CC;
+    const float pb = (float)(FZ_ERFC_P * d.beta);
     const float ccb = (float)(TWO_OVER_SQRT_PI * d.beta);
     const float nb2l2e = -(float)(d.beta * d.beta * 1.4426950408889634);
     // the chunk's entries, one per lane, read once and broadcast per entry
@@ -258,14 +332,20 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
       vMhi = (unsigned)(m >> 32);
     }
     // j cluster data run one entry ahead of the arithmetic
+    const unsigned xoff = (unsigned)jl * 16u, poff = (unsigned)jl * 8u;
     int Jn = __builtin_amdgcn_readlane(vJ, 0);
-    v4i xjn = xs[Jn * CLUSTER + jl];
-    v2f pjn = lp[Jn * CLUSTER + jl];
+    v4i xjn = buf_ld4(xsr, xoff, (unsigned)Jn * (CLUSTER * 16));
+    v2f pjn = buf_ld2(lpr, poff, (unsigned)Jn * (CLUSTER * 8));
     int curI = -1;
     int xix = 0, xiy = 0, xiz = 0, ti = 0;
     float qi = 0.f, qi_raw = 0.f;
     v2f pi = {0.f, 0.f};
     float fix = 0.f, fiy = 0.f, fiz = 0.f;
+    // The j-force atomic of a tile is issued one tile late, after the next
+    // prefetch: the wait for prefetched data (vmcnt counts loads and atomics in
+    // issue order) then covers only an atomic issued a whole tile earlier.
+    unsigned long long pend_m = 0ull, pend_v = 0ull;
+    int pend_J = 0;
     for (int k = 0; k < ne; k++) {
       const int I = __builtin_amdgcn_readlane(vI, k);
       const int J = Jn;
@@ -275,33 +355,43 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
       const v2f pj = pjn;
       if (k + 1 < ne) {
         Jn = __builtin_amdgcn_readlane(vJ, k + 1);
-        xjn = xs[Jn * CLUSTER + jl];
-        pjn = lp[Jn * CLUSTER + jl];
+        xjn = buf_ld4(xsr, xoff, (unsigned)Jn * (CLUSTER * 16));
+        pjn = buf_ld2(lpr, poff, (unsigned)Jn * (CLUSTER * 8));
       }
+      if (__builtin_amdgcn_inverse_ballot_w64(pend_m))
+        atomic_add_g((GU64*)(Fb + (size_t)pend_J * (CLUSTER * 8) + foff), pend_v);
       if (I != curI) {
-        if (curI >= 0) {
-          float rx = red_stride8(fix), ry = red_stride8(fiy), rz = red_stride8(fiz);
-          if (jl == 0) add3_g(F, cap, curI * CLUSTER + il, (double)rx, (double)ry, (double)rz);
-        }
+        // new i cluster's data first, then the old one's sums, so that the wait
+        // for the loads below does not include the flush's atomic
+        const v4i v = buf_ld4(xsr, (unsigned)il * 16u, (unsigned)I * (CLUSTER * 16));
+        const v2f pv = buf_ld2(lpr, (unsigned)il * 8u, (unsigned)I * (CLUSTER * 8));
+        if (curI >= 0) flush_i(Fb, cap8, curI, lane, fix, fiy, fiz);
         curI = I;
-        const v4i v = xs[I * CLUSTER + il];
         xix = v.x; xiy = v.y; xiz = v.z;
         qi_raw = __int_as_float(v.w);
         qi = qi_raw * cc;
-        pi = lp[I * CLUSTER + il];
+        pi = pv;
         ti = __float_as_int(pi.x) * T;
+        // consume the i loads here: a load still in flight where this branch
+        // rejoins the loop makes the compiler wait for every memory operation
+        // (the last tile's atomics and the next tile's prefetch) on each tile
+        __asm__ volatile("" ::"v"(qi), "v"(xix), "v"(xiy), "v"(xiz), "v"(pi.x), "v"(pi.y));
         fix = fiy = fiz = 0.f;
       }
       const float dx = (float)(int)((unsigned)xj.x - (unsigned)xix) * sx;
       const float dy = (float)(int)((unsigned)xj.y - (unsigned)xiy) * sy;
       const float dz = (float)(int)((unsigned)xj.z - (unsigned)xiz) * sz;
       float r2 = dx * dx + dy * dy + dz * dz;
-      const bool act = ((mask >> lane) & 1ull) && r2 < rc2;
+      // pairs inside the cutoff as a lane mask: lane l's bit of the entry mask is
+      // tested in scalar registers, not shifted out per lane
+      const unsigned long long actm = mask & __builtin_amdgcn_ballot_w64(r2 < rc2);
+      const bool act = __builtin_amdgcn_inverse_ballot_w64(actm);
       r2 = act ? r2 : 1.0f;
       const float rinv = __builtin_amdgcn_rsqf(r2);
       const float rinv2 = rinv * rinv;
       const float r = r2 * rinv;
-      // Lennard-Jones with CHARMM switch (energy switched, exact gradient)
+      // Lennard-Jones with CHARMM switch (energy switched, exact gradient):
+      // e6 = E_LJ, f6 = r2 F_LJ / (6 r) unswitched
       v2f p;
       if (sep) {
         const float h = pi.y + pj.y;
@@ -312,28 +402,42 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
       const float s2 = p.y * rinv2;
       const float s6 = s2 * s2 * s2;
       const float a6 = p.x * s6, a12 = a6 * s6;
-      float e_lj = a12 - a6;
-      float f_lj = (12.0f * a12 - 6.0f * a6) * rinv2;
-      if (r2 > ron2) {
-        const float c1 = rc2 - r2, c2 = rc2 + 2.0f * r2 - 3.0f * ron2;
-        const float sw = c1 * c1 * c2 * swi;
-        const float dsw = 12.0f * c1 * (r2 - ron2) * swi;
-        f_lj = f_lj * sw + e_lj * dsw;
-        e_lj *= sw;
-      }
-      // real-space Ewald
-      const float qq = qi * __int_as_float(xj.w);
-      const float x = beta * r;
+      const float e6 = a12 - a6;
+      const float f6 = 2.0f * a12 - a6;
+      // max(r2, ron2) of two non-negative floats as an integer max of their bits
+      const float r2c = __int_as_float(max(__float_as_int(r2), __float_as_int(ron2)));
+      const float c1 = rc2 - r2c;
+      const float sw6 = c1 * c1 * (sw_a * r2c + sw_b);  // 6 S
+      const float dsw = c1 * (r2c - ron2) * sw_d;       // -2 dS/dr2
+      // real-space Ewald: erfc(beta r) = exp(-beta^2 r2) t P(t), t = 1 / (1 + p beta r)
+      const float t = __builtin_amdgcn_rcpf(pb * r + 1.0f);
       const float ex = __builtin_amdgcn_exp2f(nb2l2e * r2);
-      const float e_c = qq * erfc_fit(x, ex) * rinv;
-      const float f_c = (e_c + qq * ccb * ex) * rinv2;
-      const float fs = act ? f_c + f_lj : 0.0f;
+      float tp = FZ_ERFC_C6;
+      tp = tp * t + FZ_ERFC_C5;
+      tp = tp * t + FZ_ERFC_C4;
+      tp = tp * t + FZ_ERFC_C3;
+      tp = tp * t + FZ_ERFC_C2;
+      tp = tp * t + FZ_ERFC_C1;
+      tp = tp * t + FZ_ERFC_C0;
+      tp *= t;
+      const float qex = qi * __int_as_float(xj.w) * ex;
+      const float fc2 = (tp * rinv + ccb) * qex;  // r2 F_C / r
+      // F / r = (r2 F_C / r + 6 S f6) / r2 + E_LJ (-2 dS/dr2)
+      const float fs = act ? (fc2 + f6 * sw6) * rinv2 + e6 * dsw : 0.0f;
       const float fx = fs * dx, fy = fs * dy, fz = fs * dz;
       fix -= fx; fiy -= fy; fiz -= fz;
-      const float jx = red8(fx), jy = red8(fy), jz = red8(fz);
-      store_j_g(F, cap, J, mask, lane, jx, jy, jz);
+      // j sums: every lane of group j = lane >> 3 holds slot j's; lane 8j + c adds
+      // component c when a pair of slot j is inside the cutoff
+      float jx = fx, jy = fy, jz = fz;
+      red8x3(jx, jy, jz);
+      const int c = lane & 7;
+      pend_v = fixed_f32(c == 0 ? jx : (c == 1 ? jy : jz));
+      pend_m = byte_any3(actm);
+      pend_J = J;
       if (EV == 1) {
         if (act) {
+          const float e_lj = e6 * sw6 * (1.0f / 6.0f);
+          const float e_c = qex * tp * rinv;
           cev[0] += (double)e_lj;
           cev[1] += (double)e_c;
           ev.npr++;
@@ -360,10 +464,9 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
         }
       }
     }
-    if (curI >= 0) {
-      float rx = red_stride8(fix), ry = red_stride8(fiy), rz = red_stride8(fiz);
-      if (jl == 0) add3_g(F, cap, curI * CLUSTER + il, (double)rx, (double)ry, (double)rz);
-    }
+    if (__builtin_amdgcn_inverse_ballot_w64(pend_m))
+      atomic_add_g((GU64*)(Fb + (size_t)pend_J * (CLUSTER * 8) + foff), pend_v);
+    if (curI >= 0) flush_i(Fb, cap8, curI, lane, fix, fiy, fiz);
     if (EV) ev.add_chunk(cev);
   }
   if (EV && evs >= 0) ev.flush(S[evs], lane);
@@ -473,9 +576,15 @@ __global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, 
 void launch_pair(const Launch& L, int ev) {
   dim3 g(L.pair_grid), b(256);
   if (L.mixed) {
-    if (ev == 2) hipLaunchKernelGGL(k_pair_mixed<2>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
-    else if (ev == 1) hipLaunchKernelGGL(k_pair_mixed<1>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
-    else hipLaunchKernelGGL(k_pair_mixed<0>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+    if (L.lj_sep) {
+      if (ev == 2) hipLaunchKernelGGL((k_pair_mixed<2, true>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+      else if (ev == 1) hipLaunchKernelGGL((k_pair_mixed<1, true>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+      else hipLaunchKernelGGL((k_pair_mixed<0, true>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+    } else {
+      if (ev == 2) hipLaunchKernelGGL((k_pair_mixed<2, false>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+      else if (ev == 1) hipLaunchKernelGGL((k_pair_mixed<1, false>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+      else hipLaunchKernelGGL((k_pair_mixed<0, false>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+    }
   } else {
     if (ev) hipLaunchKernelGGL(k_pair_f64<true>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
     else hipLaunchKernelGGL(k_pair_f64<false>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
