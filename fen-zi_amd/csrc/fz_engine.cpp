@@ -360,6 +360,12 @@ int System::alloc_device() {
     FZ_TRY(dalloc((unsigned long long**)&p, K3)); d.Qacc = p;
     FZ_TRY(dalloc((char**)&p, KH * csz)); d.Qc = p;
     FZ_TRY(dalloc((char**)&p, K3 * (mixed ? 4 : 8))); d.phi = p;
+    for (int k = 0; k < 3; k++) d.nbr[k] = (d.K[k] + 7) / 8;
+    d.n_bricks = d.nbr[0] * d.nbr[1] * d.nbr[2];
+    FZ_TRY(dalloc(&d.br_cnt, d.n_bricks));
+    FZ_TRY(dalloc(&d.br_off, d.n_bricks + 1));
+    FZ_TRY(dalloc((char**)&p, (size_t)d.n_atoms * 16 * (mixed ? 4 : 8), false)); d.br_rec = p;
+    FZ_TRY(dalloc((unsigned long long**)&p, (size_t)d.n_bricks * 1331, false)); d.br_part = p;
     const int KS = d.K[0] + d.K[1] + d.K[2];
     std::vector<double> bsp(KS), ex(KS), mv(KS);
     std::vector<double2> tw64(KS);
@@ -447,6 +453,7 @@ int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   FZ_TRY(dalloc(&d.clu_col, clusters_cap));
   FZ_TRY(dalloc(&d.bb, (size_t)6 * clusters_cap));
   FZ_TRY(dalloc(&d.su, (size_t)3 * ns));
+  FZ_TRY(dalloc(&d.br_slot, ns, false));
   FZ_TRY(dalloc(&d.search_cnt, clusters_cap + 1));
   FZ_TRY(dalloc(&d.search_tmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));
   FZ_TRY(dalloc(&d.search_mtmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));
@@ -607,7 +614,7 @@ int Engine::prepare() {
     if (s->mixed != L_.mixed) return set_error(FZ_ERR_ARG, "a batch must use one precision");
   L_.lj_sep = 1;
   for (auto* s : sys_) L_.lj_sep &= s->dv.lj_sep;
-  std::vector<int> atoms, slots, cols, clus, terms, ccl, fx_, fy_, fz_;
+  std::vector<int> atoms, slots, cols, clus, terms, ccl, fx_, fy_, fz_, bricks;
   int kmax = 4;
   for (auto* s : sys_) kmax = std::max({kmax, s->dv.K[0], s->dv.K[1], s->dv.K[2]});
   const int csz = L_.mixed ? 8 : 16;
@@ -628,6 +635,7 @@ int Engine::prepare() {
     fx_.push_back((d.K[1] * d.K[2] + NL - 1) / NL);
     fy_.push_back(d.K[2] * ng);
     fz_.push_back(d.K[1] * ng);
+    bricks.push_back(d.n_bricks);
   }
   FZ_TRY(make_work(L_.w_atoms, atoms));
   FZ_TRY(make_work(L_.w_slots, slots));
@@ -638,6 +646,7 @@ int Engine::prepare() {
   FZ_TRY(make_work(L_.w_fftx, fx_));
   FZ_TRY(make_work(L_.w_ffty, fy_));
   FZ_TRY(make_work(L_.w_fftz, fz_));
+  FZ_TRY(make_work(L_.w_bricks, bricks));
   L_.pair_grid = tuned_grid_ > 0 ? tuned_grid_ : 2048;
   prepared_ = true;
   gens_.clear();

@@ -26,6 +26,7 @@ struct Launch {
   Work w_fftx;    // x-line groups
   Work w_ffty;    // y-line groups
   Work w_fftz;    // z-line groups
+  Work w_bricks;  // one block per PME brick
   int* chunk_pre = nullptr;  // device: pair-kernel chunks per system (prefix)
   int pair_grid = 0;         // persistent pair-kernel blocks
   int fft_lds = 0;           // dynamic LDS bytes for the FFT kernels

@@ -135,23 +135,6 @@ __device__ __forceinline__ v2f buf_ld2(__amdgpu_buffer_rsrc_t r, unsigned voff, 
   return __builtin_bit_cast(v2f, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
 
-// Fixed point (2^32) of an FP32 value: round-to-nearest-even of v * 2^32 as an
-// int64, the same value to_fixed_acc((double)v) gives.  For |v| < 2^19 the
-// FP64 sum v * 2^32 + 1.5 * 2^52 is exact before its single rounding to an
-// integer, which it keeps in the low mantissa bits: its bit pattern minus that
-// of 1.5 * 2^52 is the result.  The same sum scaled by 2^-32, v + 1.5 * 2^20,
-// has the same bits and needs no multiply (one FP64 add, one subtract).  A wave with
-// a larger value (a minimiser's first steps from overlapping atoms) takes the
-// general conversion.
-__device__ __forceinline__ unsigned long long fixed_f32(float v) {
-  if (__builtin_expect(__builtin_amdgcn_ballot_w64(!(__builtin_fabsf(v) < 524288.0f)) == 0ull, 1)) {
-    // 1.5 * 2^20 + v lies in [2^20, 2^21), where the FP64 spacing is 2^-32
-    const double w = (double)v + 1572864.0;
-    return (unsigned long long)__double_as_longlong(w) - 0x4138000000000000ull;
-  }
-  return to_fixed_acc((double)v);
-}
-
 // Lanes 8k + c (c < 3) of a 64-bit lane mask whose byte k is not zero.
 __device__ __forceinline__ unsigned long long byte_any3(unsigned long long a) {
   a |= a >> 4;

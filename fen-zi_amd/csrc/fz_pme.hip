@@ -62,62 +62,211 @@ __device__ __forceinline__ R bspline4_k(R f, int k) {
   return w * R(1.0 / 6.0);
 }
 
-// One atom per wave iteration, one of its 4x4x4 grid points per lane: the 64
-// atomics of an instruction fall on 16 rows of 4 consecutive x points (a few
-// cache lines) instead of 64 unrelated lines.  Slots are visited with a
-// multiplier coprime to their count (near 0.618 ns, set at each list build; a
-// bijection of [0, ns)), so that concurrent waves work on distant atoms.
-//
-// The wave's 64 atoms are read and placed on the grid in parallel first (lane k:
-// the k-th atom), so the loop over atoms carries no memory loads: per atom it
-// broadcasts the grid corner, the spline fractions and the charge, each lane
-// forms its point's weight and issues its atomic.
-__device__ __forceinline__ float bcast(float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); }
-__device__ __forceinline__ double bcast(double v, int k) {
-  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), k), __builtin_amdgcn_readlane(__double2loint(v), k));
-}
+// ---------------------------------------------------------------------------
+// Charge spreading by bricks.  The grid is cut into bricks of 8^3 points; an
+// atom belongs to the brick holding the first point of its 4^3 stencil, so its
+// stencil lies inside the brick's (8+3)^3 block.  Per step:
+//   count    atoms per brick (rank of each atom inside its brick),
+//   scan     brick offsets (and the counts cleared for the next step),
+//   scatter  per atom a record: stencil weights along x, y, z, charge, and the
+//            stencil's first point inside the block,
+//   spread   one workgroup per brick sums its atoms' 64 weights each into the
+//            block in LDS (int64 fixed point at 2^32 per charge unit in the
+//            mixed form, FP64 in the FP64 form) and stores the block,
+//   reduce   every grid point adds the blocks that overlap it (at most 2 per
+//            axis) into Qacc, which it overwrites.
+// No global atomics: the grid is an integer sum in the mixed form, the same
+// for any order of the atoms, and bitwise equal to adding the 64 weights of
+// every atom into the grid one by one.
 
+constexpr int BRK = 8;          // brick edge (grid points)
+constexpr int BRH = BRK + 3;    // block edge: brick + the stencil's reach
+constexpr int BRV = BRH * BRH * BRH;
+
+__device__ __forceinline__ int n_slots_used(const SysDev& d) { return *d.n_clusters * CLUSTER; }
+
+// Slots are spatially sorted, so the lanes of a wave fall into a few bricks:
+// one atomic per distinct brick of the wave, ranks in lane order after it.
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_spread(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+__global__ __launch_bounds__(256) void k_brick_count(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using R = typename std::conditional<MIXED, float, double>::type;
   const int s_ = locate(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int lane = threadIdx.x & 63;
-  const int t0 = ((blockIdx.x - pre[s_]) * 4 + (threadIdx.x >> 6)) * 64;
-  const int ns = *d.n_clusters * CLUSTER;
-  const unsigned long long mult = (unsigned long long)d.n_clusters[1];
-  const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
-  const int ca = lane & 3, cb = (lane >> 2) & 3, cc = lane >> 4;
-  // lane k: atom of visit t0 + k
-  R q = R(0), fx = R(0), fy = R(0), fz = R(0);
-  int gx0 = 0, gy0 = 0, gz0 = 0;
-  bool has = false;
-  if (t0 + lane < ns) {
-    const int sl = (int)(((unsigned long long)(t0 + lane) * mult) % (unsigned long long)ns);
+  const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const bool inside = sl < n_slots_used(d);
+  int h = -1;
+  if (inside) {
     const int a = d.slot_atom[sl];
-    if (a >= 0) {
-      q = (R)d.q[a];
-      has = q != R(0);
-      gx0 = spline_pos<R>(d.ux[a], K0, fx);
-      gy0 = spline_pos<R>(d.uy[a], K1, fy);
-      gz0 = spline_pos<R>(d.uz[a], K2, fz);
+    if (a >= 0 && (R)d.q[a] != R(0)) {
+      R f;
+      const int gx = spline_pos<R>(d.ux[a], d.K[0], f);
+      const int gy = spline_pos<R>(d.uy[a], d.K[1], f);
+      const int gz = spline_pos<R>(d.uz[a], d.K[2], f);
+      h = (gx >> 3) + d.nbr[0] * ((gy >> 3) + d.nbr[1] * (gz >> 3));
     }
   }
-  unsigned long long bal = __ballot(has);
-  while (bal) {
-    const int k = __ffsll((long long)bal) - 1;
-    bal &= bal - 1ull;
-    int gx = __builtin_amdgcn_readlane(gx0, k) + ca;
-    int gy = __builtin_amdgcn_readlane(gy0, k) + cb;
-    int gz = __builtin_amdgcn_readlane(gz0, k) + cc;
-    gx -= gx >= K0 ? K0 : 0;
-    gy -= gy >= K1 ? K1 : 0;
-    gz -= gz >= K2 ? K2 : 0;
-    const R v = bcast(q, k) * bspline4_k<R>(bcast(fx, k), ca) * bspline4_k<R>(bcast(fy, k), cb) *
-                bspline4_k<R>(bcast(fz, k), cc);
-    const size_t g = ((size_t)gz * K1 + gy) * K0 + gx;
-    if (MIXED) atomicAdd((unsigned long long*)d.Qacc + g, (unsigned long long)__float2ll_rn((float)v * 4294967296.0f));
-    else unsafeAtomicAdd((double*)d.Qacc + g, (double)v);
+  const int lane = threadIdx.x & 63;
+  int rank = 0;
+  unsigned long long todo = __ballot(h >= 0);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int hl = __builtin_amdgcn_readlane(h, leader);
+    const unsigned long long m = __ballot(h == hl);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(d.br_cnt + hl, __popcll(m));
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (h == hl)
+      rank = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    todo &= ~m;
+  }
+  if (inside) d.br_slot[sl] = make_int2(h, rank);
+}
+
+// one block per system
+__global__ __launch_bounds__(1024) void k_brick_scan(const SysDev* __restrict__ S, int nsys) {
+  const SysDev& d = S[blockIdx.x];
+  __shared__ int part[1024];
+  const int n = d.n_bricks;
+  const int per = (n + 1023) / 1024;
+  const int c0 = threadIdx.x * per, c1 = min(n, c0 + per);
+  int sum = 0;
+  for (int c = c0; c < c1; c++) sum += d.br_cnt[c];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - sum;
+  for (int c = c0; c < c1; c++) {
+    d.br_off[c] = run;
+    run += d.br_cnt[c];
+    d.br_cnt[c] = 0;
+  }
+  if (threadIdx.x == 1023) d.br_off[n] = part[1023];
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_brick_scatter(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  if (sl >= n_slots_used(d)) return;
+  const int2 r = d.br_slot[sl];
+  if (r.x < 0) return;
+  const int a = d.slot_atom[sl];
+  R fx, fy, fz;
+  const int gx = spline_pos<R>(d.ux[a], d.K[0], fx);
+  const int gy = spline_pos<R>(d.uy[a], d.K[1], fy);
+  const int gz = spline_pos<R>(d.uz[a], d.K[2], fz);
+  R* rec = (R*)d.br_rec + (size_t)(d.br_off[r.x] + r.y) * 16;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    rec[k] = bspline4_k<R>(fx, k);
+    rec[4 + k] = bspline4_k<R>(fy, k);
+    rec[8 + k] = bspline4_k<R>(fz, k);
+  }
+  rec[12] = (R)d.q[a];
+  rec[13] = (R)((gx & 7) + BRH * ((gy & 7) + BRH * (gz & 7)));  // exact integer
+}
+
+// One workgroup per brick; a wave adds one atom's stencil per iteration, lane
+// l holding point (l & 3, (l >> 2) & 3, l >> 4) of it.
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_brick_spread(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using A = typename std::conditional<MIXED, unsigned long long, double>::type;
+  __shared__ A blk[BRV];
+  __shared__ R rec[64 * 16];
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int b = blockIdx.x - pre[s_];
+  const int a0 = d.br_off[b], a1 = d.br_off[b + 1];
+  if (a0 == a1) return;  // no block: the reduce skips empty bricks
+  for (int i = threadIdx.x; i < BRV; i += 256) blk[i] = A(0);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ca = lane & 3, cb = (lane >> 2) & 3, cc = lane >> 4;
+  const int lofs = ca + BRH * (cb + BRH * cc);
+  const R* src = (const R*)d.br_rec;
+  for (int base = a0; base < a1; base += 64) {
+    const int n = min(64, a1 - base);
+    __syncthreads();  // block cleared / previous records consumed
+    for (int i = threadIdx.x; i < n * 16; i += 256) rec[i] = src[(size_t)base * 16 + i];
+    __syncthreads();
+    for (int k = wv; k < n; k += 4) {
+      const R* r = rec + k * 16;
+      const R v = r[12] * r[ca] * r[4 + cb] * r[8 + cc];
+      const int loc = (int)r[13] + lofs;
+      if (MIXED) atomicAdd((unsigned long long*)&blk[loc], fixed_f32((float)v));
+      else atomicAdd((double*)&blk[loc], (double)v);
+    }
+  }
+  __syncthreads();
+  A* out = (A*)d.br_part + (size_t)b * BRV;
+  for (int i = threadIdx.x; i < BRV; i += 256) out[i] = blk[i];
+}
+
+// Block points over grid index x along one axis.  Blocks span the unwrapped
+// coordinates [0, 8 nb + 3); every X = x + m K in that range is point X & 7 of
+// brick X >> 3 (when that brick exists) and, when X & 7 < 3, point (X & 7) + 8
+// of the brick before.  At most 4 entries for any K >= 4.
+__device__ __forceinline__ int brick_cover(int x, int K, int nb, int* br, int* loc) {
+  int n = 0;
+  for (int X = x; X < BRK * nb + 3; X += K) {
+    const int h = X >> 3, l = X & 7;
+    if (h < nb) { br[n] = h; loc[n] = l; n++; }
+    if (l < 3 && h >= 1) { br[n] = h - 1; loc[n] = l + BRK; n++; }
+  }
+  return n;
+}
+
+// One workgroup per brick, two of its 8^3 points per thread (x fastest).
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_brick_reduce(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  using A = typename std::conditional<MIXED, unsigned long long, double>::type;
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
+  const int nb0 = d.nbr[0], nb1 = d.nbr[1];
+  const int b = blockIdx.x - pre[s_];
+  const int bx = b % nb0, byz = b / nb0, by = byz % nb1, bz = byz / nb1;
+  const A* part = (const A*)d.br_part;
+  for (int t = threadIdx.x; t < BRK * BRK * BRK; t += 256) {
+    const int x = BRK * bx + (t & 7), y = BRK * by + ((t >> 3) & 7), z = BRK * bz + (t >> 6);
+    if (x >= K0 || y >= K1 || z >= K2) continue;
+    int cbx[4], clx[4], cby[4], cly[4], cbz[4], clz[4];
+    const int nx = brick_cover(x, K0, nb0, cbx, clx);
+    const int ny = brick_cover(y, K1, nb1, cby, cly);
+    const int nz = brick_cover(z, K2, d.nbr[2], cbz, clz);
+    A sum = A(0);
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          const int c = cbx[i] + nb0 * (cby[j] + nb1 * cbz[k]);
+          if (d.br_off[c + 1] == d.br_off[c]) continue;
+          sum += part[(size_t)c * BRV + clx[i] + BRH * (cly[j] + BRH * clz[k])];
+        }
+    ((A*)d.Qacc)[((size_t)z * K1 + y) * K0 + x] = sum;
+  }
+}
+
+void launch_spread(const Launch& L) {
+  dim3 b(256);
+  if (L.mixed) {
+    hipLaunchKernelGGL(k_brick_count<true>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    hipLaunchKernelGGL(k_brick_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.d_sys, L.nsys);
+    hipLaunchKernelGGL(k_brick_scatter<true>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    hipLaunchKernelGGL(k_brick_spread<true>, dim3(L.w_bricks.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_bricks.pre);
+    hipLaunchKernelGGL(k_brick_reduce<true>, dim3(L.w_bricks.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_bricks.pre);
+  } else {
+    hipLaunchKernelGGL(k_brick_count<false>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    hipLaunchKernelGGL(k_brick_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.d_sys, L.nsys);
+    hipLaunchKernelGGL(k_brick_scatter<false>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    hipLaunchKernelGGL(k_brick_spread<false>, dim3(L.w_bricks.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_bricks.pre);
+    hipLaunchKernelGGL(k_brick_reduce<false>, dim3(L.w_bricks.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_bricks.pre);
   }
 }
 
@@ -218,7 +367,7 @@ __device__ __forceinline__ const C* stage_tw(C* lds, const C* __restrict__ tw, i
   return lds;
 }
 
-// x pass, forward: real accumulators -> half spectrum; clears the accumulators
+// x pass, forward: real grid -> half spectrum
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_fft_x_r2c(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre, int NL) {
   using R = typename std::conditional<MIXED, float, double>::type;
@@ -236,15 +385,8 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const SysDev* __restrict__ S,
     C v; v.x = R(0); v.y = R(0);
     if (l0 + line < (size_t)nlines) {
       const size_t g = (l0 + line) * K0 + x;
-      if (MIXED) {
-        unsigned long long* Q = (unsigned long long*)d.Qacc;
-        v.x = (R)from_fixed_acc(Q[g]);
-        Q[g] = 0ull;
-      } else {
-        double* Q = (double*)d.Qacc;
-        v.x = (R)Q[g];
-        Q[g] = 0.0;
-      }
+      if (MIXED) v.x = (R)from_fixed_acc(((const unsigned long long*)d.Qacc)[g]);
+      else v.x = (R)((const double*)d.Qacc)[g];
     }
     b0[e] = v;
   }
@@ -470,11 +612,11 @@ static void launch_fft_passes(const Launch& L, bool ev, int mode) {
 void launch_pme(const Launch& L, bool ev, int conv_mode) {
   if (L.w_fftx.total == 0) return;
   if (L.mixed) {
-    hipLaunchKernelGGL(k_spread<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    launch_spread(L);
     launch_fft_passes<true>(L, ev, conv_mode);
     hipLaunchKernelGGL(k_pme_gather<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
   } else {
-    hipLaunchKernelGGL(k_spread<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    launch_spread(L);
     launch_fft_passes<false>(L, ev, conv_mode);
     hipLaunchKernelGGL(k_pme_gather<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
   }
@@ -483,8 +625,7 @@ void launch_pme(const Launch& L, bool ev, int conv_mode) {
 void launch_pme_part(const Launch& L, int part, bool ev) {
   if (L.w_fftx.total == 0) return;
   if (part == 0) {
-    if (L.mixed) hipLaunchKernelGGL(k_spread<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
-    else hipLaunchKernelGGL(k_spread<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    launch_spread(L);
   } else if (part == 1) {
     if (L.mixed) launch_fft_passes<true>(L, ev, 0);
     else launch_fft_passes<false>(L, ev, 0);

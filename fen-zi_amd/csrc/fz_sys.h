@@ -118,6 +118,17 @@ struct SysDev {
   double *pme_ex;  // per-axis exp(-pi^2 m_d^2 / beta^2), [K0 + K1 + K2]
   double *pme_m;   // per-axis signed m_d (1/A), [K0 + K1 + K2]
   void *tw;        // twiddles e^{-2 pi i k / K_d}, [K0 + K1 + K2] complex
+  // Charge spreading by grid bricks of 8^3 points: atoms are counted and
+  // ordered by the brick holding the first point of their 4^3 stencil, each
+  // brick's stencils are summed in LDS over its (8+3)^3 block, and the blocks
+  // overlapping a grid point are added into Qacc.
+  int nbr[3];      // bricks per axis, ceil(K_d / 8)
+  int n_bricks;
+  int *br_cnt;     // [n_bricks] atoms per brick (counted each step, cleared by the scan)
+  int *br_off;     // [n_bricks + 1] first record of each brick
+  int2 *br_slot;   // [n_slots_cap] (brick, rank in brick) of each slot's atom, brick -1: none
+  void *br_rec;    // [n_atoms] 16 reals per atom: stencil weights x, y, z (4 each), q, block offset
+  void *br_part;   // [n_bricks * 1331] brick blocks (int64 fixed point / double)
   // ---- constraints --------------------------------------------------------
   int4 *ccl_atoms;    // up to 4 atoms (-1 = none)
   int4 *ccl_cons;     // x,y,z: constraint c as (local a | local b << 8); w: count
