@@ -568,6 +568,11 @@ Engine::~Engine() {
   for (int* p : d_pre_) (void)hipFree(p);
   if (d_sys_) (void)hipFree(d_sys_);
   if (L_.chunk_pre) (void)hipFree(L_.chunk_pre);
+  for (int k = 0; k < 2; k++) {
+    if (ev_join_[k]) (void)hipEventDestroy(ev_join_[k]);
+    if (side_[k]) (void)hipStreamDestroy(side_[k]);
+  }
+  if (ev_fork_) (void)hipEventDestroy(ev_fork_);
   if (L_.stream) (void)hipStreamDestroy(L_.stream);
 }
 
@@ -604,6 +609,11 @@ int Engine::prepare() {
   const int ns = (int)sys_.size();
   if (ns == 0) return set_error(FZ_ERR_ARG, "empty system set");
   if (!L_.stream) FZ_HIP(hipStreamCreateWithFlags(&L_.stream, hipStreamNonBlocking));
+  for (int k = 0; k < 2; k++) {
+    if (!side_[k]) FZ_HIP(hipStreamCreateWithFlags(&side_[k], hipStreamNonBlocking));
+    if (!ev_join_[k]) FZ_HIP(hipEventCreateWithFlags(&ev_join_[k], hipEventDisableTiming));
+  }
+  if (!ev_fork_) FZ_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
   if (!d_sys_) FZ_HIP(hipMalloc(&d_sys_, ns * sizeof(SysDev)));
   if (!L_.chunk_pre) FZ_HIP(hipMalloc(&L_.chunk_pre, (ns + 1) * sizeof(int)));
   FZ_TRY(upload_desc());
@@ -663,9 +673,21 @@ int Engine::ensure() {
 void Engine::record_forces(bool ev, bool lite) {
   launch_nblist(L_, true);
   launch_gather_sorted(L_);
+  // fork: bonded and PME beside the pair kernel (errors of these calls surface
+  // at the stream's next synchronisation or at capture end)
+  (void)hipEventRecord(ev_fork_, L_.stream);
+  Launch side[2] = {L_, L_};
+  for (int k = 0; k < 2; k++) {
+    side[k].stream = side_[k];
+    (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
+  }
+  launch_bonded(side[0], ev);
+  launch_pme(side[1], ev, 0);
   launch_pair(L_, ev ? (lite ? 1 : 2) : 0);
-  launch_bonded(L_, ev);
-  launch_pme(L_, ev, 0);
+  for (int k = 0; k < 2; k++) {
+    (void)hipEventRecord(ev_join_[k], side_[k]);
+    (void)hipStreamWaitEvent(L_.stream, ev_join_[k], 0);
+  }
 }
 
 void Engine::record_step(bool ev) {
