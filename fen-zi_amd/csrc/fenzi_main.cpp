@@ -65,6 +65,17 @@ int main(int argc, char** argv) {
   }
   if (fz_set_state(sys, restart ? rx.data() : fz_deck_coordinates(deck), restart ? rv.data() : nullptr))
     return die("initial state");
+  {  // consfix segments and the consharm restraint (src/prmread.cu:674-722)
+    std::vector<uint8_t> fixed((size_t)n_atoms);
+    const int nf = fz_deck_fixed_atoms(deck, fixed.data());
+    if (nf < 0) return die("fixed atoms");
+    if (nf > 0 && fz_set_fixed_atoms(sys, fixed.data())) return die("fixed atoms");
+    std::vector<int32_t> group((size_t)n_atoms);
+    double dist[3], fc = 0.0;
+    const int on = fz_deck_com_restraint(deck, group.data(), dist, &fc);
+    if (on < 0) return die("restraint");
+    if (on > 0 && fz_set_com_restraint(sys, group.data(), dist, fc)) return die("restraint");
+  }
   if (rs.minimizesteps > 0) {  // the reference minimises before dynamics (src/fenzi.cu:397-420)
     fz_min_params mp;
     mp.method = 1;

@@ -118,6 +118,22 @@ int fz_reassign_velocities(fz_system* s, double T, uint64_t seed, uint64_t count
   FZ_GUARD_END
 }
 
+int fz_set_fixed_atoms(fz_system* s, const uint8_t* fixed) {
+  FZ_GUARD_BEGIN
+  if (!s) return set_error(FZ_ERR_ARG, "null argument");
+  FZ_HIP(hipSetDevice(s->ctx->device));
+  return s->sys.set_fixed(fixed);
+  FZ_GUARD_END
+}
+
+int fz_set_com_restraint(fz_system* s, const int32_t* group, const double dist[3], double fc) {
+  FZ_GUARD_BEGIN
+  if (!s || (group && !dist)) return set_error(FZ_ERR_ARG, "null argument");
+  FZ_HIP(hipSetDevice(s->ctx->device));
+  return s->sys.set_com_restraint(group, dist, fc);
+  FZ_GUARD_END
+}
+
 int fz_compute_forces(fz_system* s, double* f, fz_energies* e) {
   FZ_GUARD_BEGIN
   if (!s) return set_error(FZ_ERR_ARG, "null argument");

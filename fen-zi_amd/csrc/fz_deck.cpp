@@ -92,6 +92,14 @@ struct fz_deck {
       improper_k, improper_psi0, constraint_length;
   std::vector<std::string> type_names;
   int64_t psf_counts[4] = {0, 0, 0, 0};
+  // segment names: CRD column after the coordinates (the consharm groups,
+  // src/prmread.cu:366-378) and PSF column 2 (the consfix segments, :1549)
+  std::vector<std::string> crd_seg, psf_seg;
+  bool consharm = false;
+  std::string consharm_seg[2];
+  double consharm_dist[3] = {10.0, 10.0, 20.0};  // src/globals.cu:231-232
+  double consharm_fc = 0.01;
+  std::vector<std::string> consfix_seg;
 };
 
 namespace {
@@ -188,7 +196,7 @@ int read_rtf_types(const std::string& path, std::map<int, std::string>& types) {
   return FZ_OK;
 }
 
-int read_crd(const std::string& path, std::vector<double>& xyz) {
+int read_crd(const std::string& path, std::vector<double>& xyz, std::vector<std::string>* seg = nullptr) {
   std::ifstream f(path);
   if (!f) return set_error(FZ_ERR_PARSE, "coordinate file %s not found", path.c_str());
   std::string line;
@@ -207,6 +215,7 @@ int read_crd(const std::string& path, std::vector<double>& xyz) {
     auto t = split(line);
     if (t.size() < 7) return set_error(FZ_ERR_PARSE, "%s: malformed line %ld", path.c_str(), i + 1);
     for (int k = 0; k < 3; k++) xyz[3 * i + k] = atof(t[4 + k].c_str());
+    if (seg) seg->push_back(t.size() > 7 ? t[7] : std::string());
   }
   return FZ_OK;
 }
@@ -301,6 +310,7 @@ int read_psf(const std::string& path, const std::map<int, std::string>* rtf, fz_
       ty = it->second;
     }
     atom_type_name[i] = ty;
+    D->psf_seg.push_back(t[1]);
     D->charge[i] = atof(t[6].c_str());
     D->mass[i] = atof(t[7].c_str());
   }
@@ -400,10 +410,22 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
         else if (c == 'b' || c == 'B' || c == '2') ckpt_binary = 1;
         else return set_error(FZ_ERR_PARSE, "checkpoint format \"%s\" is undefined (Ascii or Binary)", val(1).c_str());
       }
+      // restraint and fixed segments (src/prmread.cu:674-690, 699-722)
+      else if (k == "consharm") {
+        D->consharm = true;
+        D->consharm_seg[0] = val(1);
+        D->consharm_seg[1] = val(2);
+      } else if (k == "consharmdist") {
+        for (int c = 0; c < 3; c++) D->consharm_dist[c] = atof(val(1 + c).c_str());
+      } else if (k == "consharmfc") {
+        D->consharm_fc = atof(val(1).c_str());
+      } else if (k == "consfix") {
+        const int ns = atoi(val(1).c_str());
+        for (int j = 0; j < ns; j++) D->consfix_seg.push_back(val(2 + j));
+      }
       // accepted and not used by the force-and-integrate path
       else if (k == "thermostat" || k == "inbfrq" || k == "gpublocksize" || k == "hfac" ||
-               k == "keepcheckpoints" || k == "restartfilename" || k == "pmass" || k == "pref" ||
-               k == "consharm" || k == "consharmdist" || k == "consharmfc" || k == "consfix") {
+               k == "keepcheckpoints" || k == "restartfilename" || k == "pmass" || k == "pref") {
       } else {
         fprintf(stderr, "Ignoring unrecognized token, \"%s\"\n", t[0].c_str());
       }
@@ -414,7 +436,7 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
     // coordinates
     std::string cpath = resolve(base, crd);
     std::string ext = lower(cpath.substr(cpath.find_last_of('.') + 1));
-    int r = ext == "pdb" ? read_pdb(cpath, D->xyz) : ext == "xyz" ? read_xyz(cpath, D->xyz) : read_crd(cpath, D->xyz);
+    int r = ext == "pdb" ? read_pdb(cpath, D->xyz) : ext == "xyz" ? read_xyz(cpath, D->xyz) : read_crd(cpath, D->xyz, &D->crd_seg);
     if (r != FZ_OK) return r;
     // topology type names
     std::map<int, std::string> rtf_types;
@@ -572,6 +594,39 @@ int fz_deck_psf_counts(const fz_deck* d, int64_t* counts) {
   if (!d || !counts) return set_error(FZ_ERR_ARG, "null argument");
   for (int k = 0; k < 4; k++) counts[k] = d->psf_counts[k];
   return FZ_OK;
+}
+
+// consharm groups by CRD segment name, compared case-insensitively
+// (cicompare, src/prmread.cu:366-378)
+int fz_deck_com_restraint(const fz_deck* d, int32_t* group, double* dist, double* fc) {
+  if (!d || !group || !dist || !fc) return set_error(FZ_ERR_ARG, "null argument");
+  if (!d->consharm) return 0;
+  const std::string s0 = lower(d->consharm_seg[0]), s1 = lower(d->consharm_seg[1]);
+  const size_t n = d->charge.size();
+  for (size_t i = 0; i < n; i++) {
+    const std::string seg = i < d->crd_seg.size() ? lower(d->crd_seg[i]) : std::string();
+    group[i] = -1;
+    if (!seg.empty() && seg == s0) group[i] = 0;
+    if (!seg.empty() && seg == s1) group[i] = 1;
+  }
+  for (int k = 0; k < 3; k++) dist[k] = d->consharm_dist[k];
+  *fc = d->consharm_fc;
+  return 1;
+}
+
+// consfix segments by PSF segment name (src/prmread.cu:1675-1697)
+int fz_deck_fixed_atoms(const fz_deck* d, uint8_t* fixed) {
+  if (!d || !fixed) return set_error(FZ_ERR_ARG, "null argument");
+  const size_t n = d->charge.size();
+  int nf = 0;
+  for (size_t i = 0; i < n; i++) {
+    fixed[i] = 0;
+    const std::string seg = lower(d->psf_seg[i]);
+    for (const auto& f : d->consfix_seg)
+      if (seg == lower(f)) fixed[i] = 1;
+    nf += fixed[i];
+  }
+  return nf;
 }
 
 // Energy table (src/evalprops.cu:328-343 header, :248-267 row; UREY_BRADLEY,

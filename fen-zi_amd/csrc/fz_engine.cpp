@@ -58,6 +58,60 @@ static inline uint64_t pkey(int a, int b, int n) {
   return (uint64_t)lo * (uint64_t)n + (uint64_t)hi;
 }
 
+// Constraint clusters (connected components of the constraint graph, at most
+// 4 atoms and 3 constraints each; src/prmread.cu:1192-1303).  Constraints whose
+// atoms are all fixed are left out: they cannot move and would make the
+// cluster's SHAKE matrix singular.
+static int build_ccl(HostTopo& h) {
+  const int n = h.n;
+  h.ccl_atoms.clear();
+  h.ccl_cons.clear();
+  h.ccl_d2.clear();
+  h.n_cons_active = 0;
+  std::vector<int> act;
+  for (int c = 0; c < (int)h.cons_ab.size(); c++)
+    if (!(h.fixed[h.cons_ab[c].x] && h.fixed[h.cons_ab[c].y])) act.push_back(c);
+  h.n_cons_active = (int)act.size();
+  if (act.empty()) return FZ_OK;
+  std::vector<int> par(n);
+  std::iota(par.begin(), par.end(), 0);
+  for (int c : act) {
+    int ra = uf_find(par, h.cons_ab[c].x), rb = uf_find(par, h.cons_ab[c].y);
+    if (ra != rb) par[ra] = rb;
+  }
+  std::vector<int> cid(n, -1);
+  struct Cl { int at[4]; int na; int c1[3], c2[3]; double d2[3]; int nc; };
+  std::vector<Cl> cls;
+  for (int c : act) {
+    const int a = h.cons_ab[c].x, b = h.cons_ab[c].y;
+    int r = uf_find(par, a);
+    if (cid[r] < 0) {
+      cid[r] = (int)cls.size();
+      Cl z;
+      memset(&z, 0, sizeof z);
+      for (int k = 0; k < 4; k++) z.at[k] = -1;
+      cls.push_back(z);
+    }
+    Cl& cl = cls[cid[r]];
+    int la = -1, lb = -1;
+    for (int k = 0; k < cl.na; k++) { if (cl.at[k] == a) la = k; if (cl.at[k] == b) lb = k; }
+    if (la < 0) { if (cl.na >= 4) return set_error(FZ_ERR_ARG, "constraint cluster with more than 4 atoms"); la = cl.na; cl.at[cl.na++] = a; }
+    if (lb < 0) { if (cl.na >= 4) return set_error(FZ_ERR_ARG, "constraint cluster with more than 4 atoms"); lb = cl.na; cl.at[cl.na++] = b; }
+    if (cl.nc >= 3) return set_error(FZ_ERR_ARG, "constraint cluster with more than 3 constraints");
+    cl.c1[cl.nc] = la; cl.c2[cl.nc] = lb;
+    cl.d2[cl.nc] = h.cons_len[c] * h.cons_len[c];
+    cl.nc++;
+  }
+  for (auto& cl : cls) {
+    h.ccl_atoms.push_back(make_int4(cl.at[0], cl.at[1], cl.at[2], cl.at[3]));
+    int p[3] = {0, 0, 0};
+    for (int k = 0; k < cl.nc; k++) p[k] = cl.c1[k] | (cl.c2[k] << 8);
+    h.ccl_cons.push_back(make_int4(p[0], p[1], p[2], cl.nc));
+    h.ccl_d2.push_back(make_double4(cl.d2[0], cl.nc > 1 ? cl.d2[1] : 1.0, cl.nc > 2 ? cl.d2[2] : 1.0, 0.0));
+  }
+  return FZ_OK;
+}
+
 static int process_topology(const fz_topology* t, HostTopo& h) {
   const int n = t->n_atoms;
   if (n <= 0) return set_error(FZ_ERR_ARG, "n_atoms must be positive");
@@ -152,46 +206,17 @@ static int process_topology(const fz_topology* t, HostTopo& h) {
   if (h.excl_list.empty()) h.excl_list.push_back(-1);
   // constraint clusters (src/prmread.cu:1192-1303)
   h.n_constraints = t->n_constraints;
-  if (t->n_constraints > 0) {
-    std::vector<int> par(n);
-    std::iota(par.begin(), par.end(), 0);
-    for (int c = 0; c < t->n_constraints; c++) {
-      int a = t->constraint_atoms[2 * c], b = t->constraint_atoms[2 * c + 1];
-      if (!chk(a) || !chk(b) || a == b) return set_error(FZ_ERR_ARG, "constraint %d atoms invalid", c);
-      int ra = uf_find(par, a), rb = uf_find(par, b);
-      if (ra != rb) par[ra] = rb;
-    }
-    std::vector<int> cid(n, -1);
-    struct Cl { int at[4]; int na; int c1[3], c2[3]; double d2[3]; int nc; };
-    std::vector<Cl> cls;
-    for (int c = 0; c < t->n_constraints; c++) {
-      int a = t->constraint_atoms[2 * c], b = t->constraint_atoms[2 * c + 1];
-      int r = uf_find(par, a);
-      if (cid[r] < 0) {
-        cid[r] = (int)cls.size();
-        Cl z;
-        memset(&z, 0, sizeof z);
-        for (int k = 0; k < 4; k++) z.at[k] = -1;
-        cls.push_back(z);
-      }
-      Cl& cl = cls[cid[r]];
-      int la = -1, lb = -1;
-      for (int k = 0; k < cl.na; k++) { if (cl.at[k] == a) la = k; if (cl.at[k] == b) lb = k; }
-      if (la < 0) { if (cl.na >= 4) return set_error(FZ_ERR_ARG, "constraint cluster with more than 4 atoms"); la = cl.na; cl.at[cl.na++] = a; }
-      if (lb < 0) { if (cl.na >= 4) return set_error(FZ_ERR_ARG, "constraint cluster with more than 4 atoms"); lb = cl.na; cl.at[cl.na++] = b; }
-      if (cl.nc >= 3) return set_error(FZ_ERR_ARG, "constraint cluster with more than 3 constraints");
-      cl.c1[cl.nc] = la; cl.c2[cl.nc] = lb;
-      cl.d2[cl.nc] = t->constraint_length[c] * t->constraint_length[c];
-      cl.nc++;
-    }
-    for (auto& cl : cls) {
-      h.ccl_atoms.push_back(make_int4(cl.at[0], cl.at[1], cl.at[2], cl.at[3]));
-      int p[3] = {0, 0, 0};
-      for (int k = 0; k < cl.nc; k++) p[k] = cl.c1[k] | (cl.c2[k] << 8);
-      h.ccl_cons.push_back(make_int4(p[0], p[1], p[2], cl.nc));
-      h.ccl_d2.push_back(make_double4(cl.d2[0], cl.nc > 1 ? cl.d2[1] : 1.0, cl.nc > 2 ? cl.d2[2] : 1.0, 0.0));
-    }
+  h.cons_ab.clear();
+  h.cons_len.clear();
+  for (int c = 0; c < t->n_constraints; c++) {
+    int a = t->constraint_atoms[2 * c], b = t->constraint_atoms[2 * c + 1];
+    if (!chk(a) || !chk(b) || a == b) return set_error(FZ_ERR_ARG, "constraint %d atoms invalid", c);
+    h.cons_ab.push_back(make_int2(a, b));
+    h.cons_len.push_back(t->constraint_length[c]);
   }
+  h.fixed.assign(n, 0);
+  h.n_fixed = 0;
+  FZ_TRY(build_ccl(h));
   return FZ_OK;
 }
 
@@ -299,6 +324,9 @@ int System::alloc_device() {
   FZ_TRY(dalloc(&d.vx, n)); FZ_TRY(dalloc(&d.vy, n)); FZ_TRY(dalloc(&d.vz, n));
   FZ_TRY(dalloc(&d.q, n)); FZ_TRY(dalloc(&d.minv, n)); FZ_TRY(dalloc(&d.type, n));
   FZ_TRY(up(d.q, h.q)); FZ_TRY(up(d.minv, h.minv)); FZ_TRY(up(d.type, h.type));
+  FZ_TRY(dalloc(&d.mdyn, n));
+  FZ_TRY(upload_mdyn());
+  FZ_TRY(dalloc(&d.com_grp, n));
   FZ_TRY(dalloc(&d.atom_slot, n)); FZ_TRY(dalloc(&d.atom_col, n));
   FZ_TRY(dalloc(&d.n_clusters, 2)); FZ_TRY(dalloc(&d.n_pairs, 1));
   FZ_TRY(dalloc(&d.excl_bits, n)); FZ_TRY(up(d.excl_bits, h.excl_bits));
@@ -464,6 +492,72 @@ int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   return FZ_OK;
 }
 
+// inverse masses of the integrator: 0 for fixed atoms
+int System::upload_mdyn() {
+  std::vector<double> m(topo.minv);
+  for (int i = 0; i < topo.n; i++)
+    if (topo.fixed[i]) m[i] = 0.0;
+  dv.n_mobile = topo.n - topo.n_fixed;
+  return up(dv.mdyn, m);
+}
+
+// Fixed atoms (consfix, src/prmread.cu:699-722,1665-1728; mobility v.w in the
+// half kick, src/mdcuda.cu:70-80).  Velocities of fixed atoms are zeroed; the
+// constraint clusters are rebuilt without constraints between fixed atoms.
+int System::set_fixed(const uint8_t* fixed) {
+  const int n = topo.n;
+  int nf = 0;
+  for (int i = 0; i < n; i++) nf += fixed && fixed[i] ? 1 : 0;
+  if (nf == n) return set_error(FZ_ERR_ARG, "every atom is fixed");
+  for (int i = 0; i < n; i++) topo.fixed[i] = fixed && fixed[i] ? 1 : 0;
+  topo.n_fixed = nf;
+  FZ_TRY(build_ccl(topo));
+  SysDev& d = dv;
+  d.n_ccl = (int)topo.ccl_atoms.size();
+  if (d.n_ccl > 0) {
+    // new (smaller or equal) arrays; the previous ones are released with the system
+    FZ_TRY(dalloc(&d.ccl_atoms, topo.ccl_atoms.size())); FZ_TRY(up(d.ccl_atoms, topo.ccl_atoms));
+    FZ_TRY(dalloc(&d.ccl_cons, topo.ccl_cons.size())); FZ_TRY(up(d.ccl_cons, topo.ccl_cons));
+    FZ_TRY(dalloc(&d.ccl_d2, topo.ccl_d2.size())); FZ_TRY(up(d.ccl_d2, topo.ccl_d2));
+  }
+  FZ_TRY(upload_mdyn());
+  std::vector<double> v((size_t)3 * n);
+  FZ_TRY(download_state(nullptr, v.data(), nullptr));
+  for (int i = 0; i < n; i++)
+    if (topo.fixed[i]) v[3 * i] = v[3 * i + 1] = v[3 * i + 2] = 0.0;
+  FZ_TRY(upload_velocities(v.data()));
+  gen++;
+  return FZ_OK;
+}
+
+// Harmonic COM restraint (consharm, src/prmread.cu:674-690; src/mdcuda.cu:481-516).
+int System::set_com_restraint(const int32_t* group, const double dist[3], double fc) {
+  const int n = topo.n;
+  SysDev& d = dv;
+  if (!group || fc == 0.0) {
+    d.com_on = 0;
+    gen++;
+    return FZ_OK;
+  }
+  if (!std::isfinite(fc)) return set_error(FZ_ERR_ARG, "restraint force constant must be finite");
+  double M[2] = {0.0, 0.0};
+  std::vector<int> g(n);
+  for (int i = 0; i < n; i++) {
+    if (group[i] < -1 || group[i] > 1) return set_error(FZ_ERR_ARG, "restraint group of atom %d must be -1, 0 or 1", i);
+    g[i] = group[i];
+    if (g[i] >= 0) M[g[i]] += topo.mass[i];
+  }
+  if (!(M[0] > 0.0 && M[1] > 0.0)) return set_error(FZ_ERR_ARG, "both restraint groups need atoms");
+  FZ_TRY(up(d.com_grp, g));
+  d.com_on = 1;
+  for (int k = 0; k < 3; k++) d.com_d[k] = dist[k];
+  d.com_fc = fc;
+  d.com_im[0] = 1.0 / M[0];
+  d.com_im[1] = 1.0 / M[1];
+  gen++;
+  return FZ_OK;
+}
+
 int System::upload_fixed(const uint64_t* u) {
   const int n = topo.n;
   std::vector<unsigned long long> c(n);
@@ -550,7 +644,8 @@ int System::energies(fz_energies* e) {
   e->potential = e->bond + e->angle + e->urey_bradley + e->improper + e->dihedral + e->vdw + e->elec + e->pme +
                  e->ewald_excl + e->ewald_self;
   e->kinetic = fx(a[A_KIN]);
-  const double dof = 3.0 * topo.n - 6.0 - topo.n_constraints;
+  // 3 N_mobile - 6 - N_constraints (src/evalprops.cu:221-225)
+  const double dof = 3.0 * (topo.n - topo.n_fixed) - 6.0 - topo.n_cons_active;
   e->temperature = dof > 0 ? 2.0 * e->kinetic / (kRC * dof) : 0.0;
   for (int k = 0; k < 9; k++) e->virial[k] = fx(a[A_VIR + k]);
   e->pairs_within_cutoff = (int64_t)a[A_NPAIR];
@@ -623,7 +718,11 @@ int Engine::prepare() {
   for (auto* s : sys_)
     if (s->mixed != L_.mixed) return set_error(FZ_ERR_ARG, "a batch must use one precision");
   L_.lj_sep = 1;
-  for (auto* s : sys_) L_.lj_sep &= s->dv.lj_sep;
+  L_.com_any = 0;
+  for (auto* s : sys_) {
+    L_.lj_sep &= s->dv.lj_sep;
+    L_.com_any |= s->dv.com_on;
+  }
   std::vector<int> atoms, slots, cols, clus, terms, ccl, fx_, fy_, fz_, bricks;
   int kmax = 4;
   for (auto* s : sys_) kmax = std::max({kmax, s->dv.K[0], s->dv.K[1], s->dv.K[2]});
@@ -688,6 +787,7 @@ void Engine::record_forces(bool ev, bool lite) {
     (void)hipEventRecord(ev_join_[k], side_[k]);
     (void)hipStreamWaitEvent(L_.stream, ev_join_[k], 0);
   }
+  if (L_.com_any) launch_restraint(L_);
 }
 
 void Engine::record_step(bool ev) {

@@ -44,6 +44,10 @@ struct HostTopo {
   std::vector<int> excl_off, excl_list;
   std::vector<int4> ccl_atoms, ccl_cons; std::vector<double4> ccl_d2;
   int n_constraints = 0;
+  std::vector<int2> cons_ab; std::vector<double> cons_len;  // the constraint list as given
+  std::vector<uint8_t> fixed;  // consfix: atoms held in place
+  int n_fixed = 0;
+  int n_cons_active = 0;       // constraints with a mobile atom (the ones SHAKE solves)
   double q2sum = 0.0;
 };
 
@@ -76,6 +80,9 @@ struct System {
   int upload_positions(const double* xyz);
   int upload_fixed(const uint64_t* u);
   int upload_velocities(const double* v);
+  int upload_mdyn();
+  int set_fixed(const uint8_t* fixed);
+  int set_com_restraint(const int32_t* group, const double dist[3], double fc);
   int download_state(double* xyz, double* v, uint64_t* u);
   int energies(fz_energies* e);
 };

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_kick_drift(const SysDev* __restrict__ S
   const int i = lb * 256 + threadIdx.x;
   if (i >= d.n_atoms) return;
   const int sl = d.atom_slot[i];
-  const double h = 0.5 * d.dt * d.minv[i];
+  const double h = 0.5 * d.dt * d.mdyn[i];
   const double vx = d.vx[i] + h * take_force<MIXED>(d, sl, 0, true);
   const double vy = d.vy[i] + h * take_force<MIXED>(d, sl, 1, true);
   const double vz = d.vz[i] + h * take_force<MIXED>(d, sl, 2, true);
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_kick(const SysDev* __restrict__ S, int 
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (i >= d.n_atoms) return;
   const int sl = d.atom_slot[i];
-  const double h = 0.5 * d.dt * d.minv[i];
+  const double h = 0.5 * d.dt * d.mdyn[i];
   d.vx[i] += h * take_force<MIXED>(d, sl, 0, false);
   d.vy[i] += h * take_force<MIXED>(d, sl, 1, false);
   d.vz[i] += h * take_force<MIXED>(d, sl, 2, false);
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int
   load_rel(d, k, d.ox, d.oy, d.oz, ro);
 #pragma unroll
   for (int m = 0; m < 4; m++) {
-    mi[m] = k.at[m] >= 0 ? d.minv[k.at[m]] : 0.0;
+    mi[m] = k.at[m] >= 0 ? d.mdyn[k.at[m]] : 0.0;
     dr[m] = V3{0.0, 0.0, 0.0};
   }
   V3 g[3];
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(128) void k_rattle(const SysDev* __restrict__ S, in
   for (int m = 0; m < 4; m++) {
     const int a = k.at[m] >= 0 ? k.at[m] : k.at[0];
     v[m] = V3{d.vx[a], d.vy[a], d.vz[a]};
-    mi[m] = k.at[m] >= 0 ? d.minv[a] : 0.0;
+    mi[m] = k.at[m] >= 0 ? d.mdyn[a] : 0.0;
   }
   V3 rc[3];
   double A[3][3], b[3];
@@ -300,8 +300,9 @@ __global__ __launch_bounds__(256) void k_maxwell(const SysDev* __restrict__ S, i
     const double u3 = ((double)c[2] + 0.5) * s, u4 = ((double)c[3] + 0.5) * s;
     const double m = 1.0 / d.minv[i], vm = sqrt(RCG * T / m);
     const double ra = sqrt(-2.0 * log(u1)), rb = sqrt(-2.0 * log(u3));
-    const double vx = vm * ra * cos(2.0 * PI_D * u2), vy = vm * ra * sin(2.0 * PI_D * u2);
-    const double vz = vm * rb * cos(2.0 * PI_D * u4);
+    const double mob = d.mdyn[i] != 0.0 ? 1.0 : 0.0;  // fixed atoms: zero velocity
+    const double vx = mob * vm * ra * cos(2.0 * PI_D * u2), vy = mob * vm * ra * sin(2.0 * PI_D * u2);
+    const double vz = mob * vm * rb * cos(2.0 * PI_D * u4);
     d.vx[i] = vx; d.vy[i] = vy; d.vz[i] = vz;
     p[0] = m * vx; p[1] = m * vy; p[2] = m * vz;
   }
@@ -309,13 +310,14 @@ __global__ __launch_bounds__(256) void k_maxwell(const SysDev* __restrict__ S, i
   block_acc_fixed<3>(p, d.acc, slots);
 }
 
-// v_i -= P / (N m_i) (src/init.cu:94-118)
+// v_i -= P / (N m_i) over the N mobile atoms (src/init.cu:94-118; with fixed
+// atoms the reference sums over its first nAtomwoseg atoms, src/init.cu:77-127)
 __global__ __launch_bounds__(256) void k_momentum(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = locate(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (i >= d.n_atoms) return;
-  const double f = d.minv[i] / d.n_atoms;
+  const double f = d.n_mobile > 0 ? d.mdyn[i] / d.n_mobile : 0.0;
   d.vx[i] -= from_fixed_e(d.acc[A_MOM]) * f;
   d.vy[i] -= from_fixed_e(d.acc[A_MOM + 1]) * f;
   d.vz[i] -= from_fixed_e(d.acc[A_MOM + 2]) * f;
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(256) void k_fmax(const SysDev* __restrict__ S, int 
   const SysDev& d = S[s_];
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   double f2 = 0.0;
-  if (i < d.n_atoms) {
+  if (i < d.n_atoms && d.mdyn[i] != 0.0) {
     const int sl = d.atom_slot[i];
     const double fx = take_force<MIXED>(d, sl, 0, false), fy = take_force<MIXED>(d, sl, 1, false),
                  fz = take_force<MIXED>(d, sl, 2, false);
@@ -354,6 +356,7 @@ __global__ __launch_bounds__(256) void k_min_move(const SysDev* __restrict__ S, 
   const int sl = d.atom_slot[i];
   const unsigned long long ux = d.ux[i], uy = d.uy[i], uz = d.uz[i];
   d.ox[i] = ux; d.oy[i] = uy; d.oz[i] = uz;
+  if (d.mdyn[i] == 0.0) return;  // fixed atom
   d.ux[i] = ux + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, 0, false) * d.isc[0]);
   d.uy[i] = uy + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, 1, false) * d.isc[1]);
   d.uz[i] = uz + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, 2, false) * d.isc[2]);
@@ -373,7 +376,7 @@ __global__ __launch_bounds__(256) void k_min_ref_move(const SysDev* __restrict__
   const unsigned long long ux = d.ux[i], uy = d.uy[i], uz = d.uz[i];
   d.ox[i] = ux; d.oy[i] = uy; d.oz[i] = uz;
   const int method = (int)d.mstep[1];
-  if (method == 0) return;
+  if (method == 0 || d.mdyn[i] == 0.0) return;  // no move / fixed atom
   const double fac = d.mstep[0];
   const int sl = d.atom_slot[i];
   const double F[3] = {take_force<MIXED>(d, sl, 0, false), take_force<MIXED>(d, sl, 1, false),

@@ -32,6 +32,7 @@ struct Launch {
   int fft_lds = 0;           // dynamic LDS bytes for the FFT kernels
   int fft_nl = 16;           // lines per FFT workgroup
   int lj_sep = 0;            // every system follows the LJ combination rule
+  int com_any = 0;           // some system has a COM restraint
 };
 
 void launch_nblist(const Launch& L, bool always_rebuild_check);
@@ -50,5 +51,6 @@ void launch_maxwell(const Launch& L, double T, unsigned long long seed, unsigned
 void launch_zero_acc(const Launch& L);
 void launch_clear_shake_flag(const Launch& L);
 void launch_zero_forces(const Launch& L);
+void launch_restraint(const Launch& L);  // COM restraint forces (systems with com_on)
 
 }  // namespace fz

@@ -17,7 +17,7 @@ constexpr int MAX_FFT = 256;        // largest PME grid dimension
 enum AccSlot {
   A_BOND = 0, A_ANGLE, A_UB, A_IMPR, A_DIHED, A_VDW, A_ELEC, A_PME, A_EWEXCL,
   A_KIN, A_VIR /* 9 slots */, A_NPAIR = A_VIR + 9, A_MOM /* 3 slots */, A_FMAX = A_MOM + 3,
-  A_NSLOT = A_FMAX + 1
+  A_COM /* 6 slots: sum m x of group 0, then group 1 */, A_NSLOT = A_COM + 6
 };
 
 // Device-side status flags.
@@ -67,7 +67,19 @@ struct SysDev {
   unsigned long long *ox, *oy, *oz;   // positions before the drift (SHAKE reference)
   double *vx, *vy, *vz;
   double *q, *minv;
+  // inverse mass used by the integrator, SHAKE/RATTLE and the minimisers: minv
+  // for mobile atoms, 0 for fixed atoms (consfix), which therefore keep zero
+  // velocity and never move
+  double *mdyn;
+  int n_mobile;
   int *type;
+  // ---- harmonic COM restraint between two atom groups (consharm) ------
+  // Every evaluation, the atoms of group 1 receive fc * (com_d + c0 - c1)
+  // (minimum image), c_g the mass-weighted mean of group g's positions taken in
+  // [-L/2, L/2).  Group sums of m x go through A_COM (fixed point, 2^28).
+  int com_on;
+  int *com_grp;          // [n_atoms] 0, 1 or -1
+  double com_d[3], com_fc, com_im[2];  // offset, force constant, 1 / group masses
   // ---- cluster-sorted arrays (slot order) ----------------------------
   int *slot_atom;        // [n_slots_cap], -1 = padding
   int *atom_slot;        // [n_atoms]

@@ -79,7 +79,8 @@ EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create
            "fz_server_create", "fz_server_submit", "fz_server_poll", "fz_server_job_info", "fz_server_job_result",
            "fz_server_release", "fz_server_estimate_bytes", "fz_server_destroy",
            "fz_minimize_ex", "fz_batch_minimize_ex", "fz_restart_write", "fz_restart_read", "fz_dcd_open",
-           "fz_dcd_write_frame", "fz_dcd_close"]
+           "fz_dcd_write_frame", "fz_dcd_close", "fz_set_fixed_atoms", "fz_set_com_restraint",
+           "fz_deck_com_restraint", "fz_deck_fixed_atoms"]
 
 MIN_ADAPTIVE_SD, MIN_REF_CG, MIN_REF_SD = 0, 1, 2
 
@@ -168,6 +169,10 @@ def lib():
         L.fz_deck_coordinates.restype = _dp
         L.fz_deck_settings.argtypes = [vp, C.POINTER(RunSettings)]
         L.fz_deck_psf_counts.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.fz_deck_com_restraint.argtypes = [vp, C.POINTER(C.c_int32), _dp, _dp]
+        L.fz_deck_fixed_atoms.argtypes = [vp, C.POINTER(C.c_uint8)]
+        L.fz_set_fixed_atoms.argtypes = [vp, C.POINTER(C.c_uint8)]
+        L.fz_set_com_restraint.argtypes = [vp, C.POINTER(C.c_int32), _dp, C.c_double]
         L.fz_format_energy_header.argtypes = [C.c_char_p, C.c_size_t]
         L.fz_format_energy_row.argtypes = [C.POINTER(Energies), C.c_double, C.c_char_p, C.c_size_t]
         L.fz_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
@@ -305,6 +310,26 @@ class System:
 
     def reassign_velocities(self, T, seed, counter=0):
         _check(lib().fz_reassign_velocities(self.h, float(T), int(seed), int(counter)), "fz_reassign_velocities")
+
+    def set_fixed_atoms(self, fixed):
+        """consfix: fixed[i] true holds atom i in place (None releases all)."""
+        if fixed is None:
+            _check(lib().fz_set_fixed_atoms(self.h, None), "fz_set_fixed_atoms")
+            return
+        m = np.ascontiguousarray(np.asarray(fixed).astype(np.uint8))
+        assert m.shape == (self.n,)
+        _check(lib().fz_set_fixed_atoms(self.h, _p(m, C.c_uint8)), "fz_set_fixed_atoms")
+
+    def set_com_restraint(self, group, dist=(10.0, 10.0, 20.0), fc=0.01):
+        """consharm: group[i] in {0, 1, -1}; group-1 atoms receive fc * (dist + c0 - c1)."""
+        if group is None:
+            _check(lib().fz_set_com_restraint(self.h, None, None, 0.0), "fz_set_com_restraint")
+            return
+        g = np.ascontiguousarray(np.asarray(group, dtype=np.int32))
+        assert g.shape == (self.n,)
+        dd = np.ascontiguousarray(np.asarray(dist, dtype=np.float64))
+        _check(lib().fz_set_com_restraint(self.h, _p(g, C.c_int32), _p(dd, C.c_double), float(fc)),
+               "fz_set_com_restraint")
 
     def forces(self):
         f = np.empty((self.n, 3))
@@ -571,6 +596,24 @@ class Deck:
         c = np.zeros(4, np.int64)
         _check(lib().fz_deck_psf_counts(self.h, _p(c, C.c_int64)), "fz_deck_psf_counts")
         return dict(zip(["bonds", "angles", "dihedrals", "impropers"], c.tolist()))
+
+    def com_restraint(self):
+        """(group, dist, fc) of the consharm keywords, or None."""
+        n = self.topology()["n_atoms"]
+        g = np.zeros(n, np.int32)
+        dd = np.zeros(3)
+        fc = C.c_double(0.0)
+        r = lib().fz_deck_com_restraint(self.h, _p(g, C.c_int32), _p(dd, C.c_double), C.byref(fc))
+        _check(min(r, 0), "fz_deck_com_restraint")
+        return (g, dd, fc.value) if r == 1 else None
+
+    def fixed_atoms(self):
+        """consfix mask (uint8 per atom)."""
+        n = self.topology()["n_atoms"]
+        m = np.zeros(n, np.uint8)
+        r = lib().fz_deck_fixed_atoms(self.h, _p(m, C.c_uint8))
+        _check(min(r, 0), "fz_deck_fixed_atoms")
+        return m
 
 
 def energy_header():

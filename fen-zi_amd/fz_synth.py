@@ -377,9 +377,11 @@ def ensemble_member(n_target, chain_frac, seed, cutoff=10.0):
 
 
 def write_deck(topo, xyz, params, directory, name="synth", temperature=300.0, steps=1000, shake=True,
-               reassign_every=0, output_every=100, seed=1):
+               reassign_every=0, output_every=100, seed=1, psf_seg=None, crd_seg=None, extra=()):
     """Write a FEN ZI input deck (.in / .psf / .prm / .crd) for a system built
-    by make_system, with string atom types (no .rtf), in CHARMM formats."""
+    by make_system, with string atom types (no .rtf), in CHARMM formats.
+    psf_seg / crd_seg: per-atom segment names of the PSF and CRD files (default
+    "SYS"); extra: further keyword lines of the .in file."""
     import os
     os.makedirs(directory, exist_ok=True)
     n = int(topo["n_atoms"])
@@ -391,7 +393,8 @@ def write_deck(topo, xyz, params, directory, name="synth", temperature=300.0, st
         for i in range(n):
             t = int(topo["type"][i])
             f.write("%8d %-4s %-4d %-4s %-4s %-4s %14.6f %14.6f %8d\n" % (
-                i + 1, "SYS", i // 3 + 1, res[i], tnames[t], tnames[t], topo["charge"][i], topo["mass"][i], 0))
+                i + 1, psf_seg[i] if psf_seg is not None else "SYS", i // 3 + 1, res[i], tnames[t], tnames[t],
+                topo["charge"][i], topo["mass"][i], 0))
 
         def section(tag, arr, width, per_line):
             arr = np.asarray(arr).reshape(-1, width) + 1
@@ -437,7 +440,8 @@ def write_deck(topo, xyz, params, directory, name="synth", temperature=300.0, st
         for i in range(n):
             t = int(topo["type"][i])
             f.write("%10d%10d  %-8s  %-8s%20.10f%20.10f%20.10f  %-8s  %-8d%20.10f\n" % (
-                i + 1, i // 3 + 1, res[i], tnames[t], xyz[i, 0], xyz[i, 1], xyz[i, 2], "SYS", i // 3 + 1, 0.0))
+                i + 1, i // 3 + 1, res[i], tnames[t], xyz[i, 0], xyz[i, 1], xyz[i, 2],
+                crd_seg[i] if crd_seg is not None else "SYS", i // 3 + 1, 0.0))
     box = params["box"]
     K = params["K"]
     with open(os.path.join(directory, name + ".in"), "w") as f:
@@ -451,6 +455,8 @@ def write_deck(topo, xyz, params, directory, name="synth", temperature=300.0, st
         f.write("cellBasisVector1 %.10f 0 0\ncellBasisVector2 0 %.10f 0\ncellBasisVector3 0 0 %.10f\n" % tuple(box))
         f.write("PMEGridSizeX %d\nPMEGridSizeY %d\nPMEGridSizeZ %d\n" % tuple(K))
         f.write("shake %s %g\nprinttostdout 1\n" % ("on" if shake else "off", params.get("shake_tol", 1e-8)))
+        for line in extra:
+            f.write(line + "\n")
     return os.path.join(directory, name + ".in")
 
 

@@ -125,6 +125,21 @@ int fz_get_fixed_positions(fz_system *sys, uint64_t *u);
  * momentum removed, constraint-projected.  Replaces InitVelocity
  * (src/init.cu:19) and its time(NULL) seeding. */
 int fz_reassign_velocities(fz_system *sys, double temperature, uint64_t seed, uint64_t counter);
+/* Fixed atoms (consfix; src/prmread.cu:699-722,1665-1728, mobility v.w of the
+ * half kick src/mdcuda.cu:70-80): fixed[i] != 0 holds atom i in place (zero
+ * velocity; not moved by dynamics, SHAKE/RATTLE or the minimisers).
+ * Constraints between fixed atoms are dropped; the temperature counts
+ * 3 N_mobile - 6 - N_constraints degrees of freedom (src/evalprops.cu:222-225).
+ * fixed = NULL releases every atom. */
+int fz_set_fixed_atoms(fz_system *sys, const uint8_t *fixed);
+/* Harmonic centre-of-mass restraint (consharm / consharmdist / consharmfc,
+ * src/prmread.cu:674-690; reduce_COM src/cell_based_kernels.cu:713-866,
+ * restraint_force src/mdcuda.cu:481-516): group[i] in {0, 1, -1}.  Each
+ * evaluation the atoms of group 1 receive fc * (dist + c0 - c1), minimum image,
+ * c_g the mass-weighted mean of group g's positions in [-L/2, L/2); group 0
+ * receives no force and no energy is defined (as in the reference).
+ * group = NULL or fc = 0 removes the restraint. */
+int fz_set_com_restraint(fz_system *sys, const int32_t *group, const double dist[3], double fc);
 /* Forces (nullable, [n][3]) and energies (nullable) at the current positions.
  * Replaces ComputeAccelGPU (src/mdcuda.cu:557) + EvalProps' reductions
  * (src/evalprops.cu:17). */
@@ -292,6 +307,14 @@ typedef struct fz_run_settings {
 int fz_deck_settings(const fz_deck *deck, fz_run_settings *out);
 /* Section counts of the PSF as read: bonds, angles, dihedrals, impropers. */
 int fz_deck_psf_counts(const fz_deck *deck, int64_t *counts);
+/* The deck's COM restraint: group[n_atoms] from the CRD segment names of the
+ * consharm keyword (0 first, 1 second, -1 other), dist (consharmdist, default
+ * 10 10 20) and fc (consharmfc, default 0.01).  Returns 1 if consharm is set,
+ * 0 if not (group, dist and fc untouched), negative on error. */
+int fz_deck_com_restraint(const fz_deck *deck, int32_t *group, double *dist, double *fc);
+/* The deck's fixed atoms: fixed[n_atoms] = 1 for atoms whose PSF segment is
+ * listed by consfix.  Returns the number of fixed atoms. */
+int fz_deck_fixed_atoms(const fz_deck *deck, uint8_t *fixed);
 
 /* Energy table in the reference's format (print_output_headers / EvalProps,
  * src/evalprops.cu:248-267,311-343).  Return the number of characters written. */

@@ -57,7 +57,19 @@ struct or_sys {
   int n_clusters;
   or_cluster *clusters;
   int n_cons_total;
+  /* fixed atoms: mobile[i] = 0 holds atom i; NULL = all mobile */
+  uint8_t *mobile;
+  int n_fixed;
+  /* COM restraint */
+  int com_on;
+  int32_t *com_grp;
+  double com_d[3], com_fc, com_m[2];
 };
+
+/* inverse mass of the integrator (0 for a fixed atom) */
+static inline double dyn_minv(const or_sys *s, int i) {
+  return (s->mobile && !s->mobile[i]) ? 0.0 : 1.0 / s->t.mass[i];
+}
 
 static int cmp_u64(const void *a, const void *b) {
   uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
@@ -164,12 +176,15 @@ static int uf_find(int *p, int x) { while (p[x] != x) { p[x] = p[p[x]]; x = p[x]
 static void build_clusters(or_sys *s) {
   const or_topo *t = &s->t;
   int n = t->n_atoms, nc = t->n_constraints;
-  s->n_clusters = 0; s->clusters = NULL; s->n_cons_total = nc;
+  free(s->clusters);
+  s->n_clusters = 0; s->clusters = NULL; s->n_cons_total = 0;
   if (nc == 0) return;
   int *par = (int *)malloc(sizeof(int) * (size_t)n);
   for (int i = 0; i < n; i++) par[i] = i;
   for (int c = 0; c < nc; c++) {
-    int a = uf_find(par, t->constraint_atoms[2 * c]), b = uf_find(par, t->constraint_atoms[2 * c + 1]);
+    int a0 = t->constraint_atoms[2 * c], b0 = t->constraint_atoms[2 * c + 1];
+    if (s->mobile && !s->mobile[a0] && !s->mobile[b0]) continue;
+    int a = uf_find(par, a0), b = uf_find(par, b0);
     if (a != b) par[a] = b;
   }
   int *cid = (int *)malloc(sizeof(int) * (size_t)n);
@@ -177,6 +192,8 @@ static void build_clusters(or_sys *s) {
   s->clusters = (or_cluster *)calloc((size_t)nc, sizeof(or_cluster));
   for (int c = 0; c < nc; c++) {
     int a = t->constraint_atoms[2 * c], b = t->constraint_atoms[2 * c + 1];
+    if (s->mobile && !s->mobile[a] && !s->mobile[b]) continue; /* both fixed */
+    s->n_cons_total++;
     int r = uf_find(par, a);
     if (cid[r] < 0) cid[r] = s->n_clusters++;
     or_cluster *cl = &s->clusters[cid[r]];
@@ -203,7 +220,63 @@ or_sys *or_create(const or_topo *t, const or_params *p) {
 void or_destroy(or_sys *s) {
   if (!s) return;
   free(s->excl); free(s->ew); free(s->p14); free(s->xoff); free(s->xlist); free(s->clusters);
+  free(s->mobile); free(s->com_grp);
   free(s);
+}
+
+int or_set_fixed(or_sys *s, const uint8_t *fixed) {
+  int n = s->t.n_atoms, nf = 0;
+  if (fixed)
+    for (int i = 0; i < n; i++) nf += fixed[i] ? 1 : 0;
+  if (nf == n) return -1;
+  free(s->mobile);
+  s->mobile = NULL;
+  s->n_fixed = nf;
+  if (nf > 0) {
+    s->mobile = (uint8_t *)malloc((size_t)n);
+    for (int i = 0; i < n; i++) s->mobile[i] = fixed[i] ? 0 : 1;
+  }
+  build_clusters(s);
+  return 0;
+}
+
+int or_set_com_restraint(or_sys *s, const int32_t *group, const double dist[3], double fc) {
+  int n = s->t.n_atoms;
+  s->com_on = 0;
+  if (!group || fc == 0.0) return 0;
+  free(s->com_grp);
+  s->com_grp = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  s->com_m[0] = s->com_m[1] = 0.0;
+  for (int i = 0; i < n; i++) {
+    s->com_grp[i] = group[i];
+    if (group[i] == 0 || group[i] == 1) s->com_m[group[i]] += s->t.mass[i];
+  }
+  if (!(s->com_m[0] > 0 && s->com_m[1] > 0)) return -1;
+  for (int d = 0; d < 3; d++) s->com_d[d] = dist[d];
+  s->com_fc = fc;
+  s->com_on = 1;
+  return 0;
+}
+
+/* group COMs from positions in [-L/2, L/2) (the reference's wrap,
+ * src/cell_based_kernels.cu:738-763), force fc (d + c0 - c1) on group 1 */
+static void com_restraint(const or_sys *s, const uint64_t *u, double *f) {
+  int n = s->t.n_atoms;
+  double sum[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < n; i++) {
+    int g = s->com_grp[i];
+    if (g < 0) continue;
+    for (int d = 0; d < 3; d++) sum[g][d] += s->t.mass[i] * ((double)(int64_t)u[3 * i + d] * s->p.box[d] * TWO_POW_M64);
+  }
+  double dr[3];
+  for (int d = 0; d < 3; d++) {
+    double x = s->com_d[d] + (sum[0][d] / s->com_m[0] - sum[1][d] / s->com_m[1]);
+    x -= s->p.box[d] * rint(x / s->p.box[d]);
+    dr[d] = s->com_fc * x;
+  }
+  for (int i = 0; i < n; i++)
+    if (s->com_grp[i] == 1)
+      for (int d = 0; d < 3; d++) f[3 * i + d] += dr[d];
 }
 
 int or_set_threads(int n) {
@@ -780,6 +853,7 @@ void or_forces(or_sys *s, const uint64_t *u, double *f, double *e, double *vir, 
   if (flags & OR_BONDED) bonded(s, u, ff, ee, vv);
   if (flags & OR_EWALD) ewald_direct(s, u, ff, ee, vv);
   else if (flags & OR_PME) pme(s, u, ff, ee, vv);
+  if ((flags & OR_RESTRAINT) && s->com_on) com_restraint(s, u, ff);
   if (flags & OR_SELF) {
     double q2 = 0;
     for (int i = 0; i < n; i++) q2 += s->t.charge[i] * s->t.charge[i];
@@ -957,7 +1031,7 @@ static int shake_cluster(const or_sys *s, const or_cluster *cl, const uint64_t *
   for (int k = 0; k < na; k++) {
     fx_disp(unew, a0, cl->atom[k], sc, rn[k]);
     fx_disp(uold, a0, cl->atom[k], sc, ro[k]);
-    minv[k] = 1.0 / s->t.mass[cl->atom[k]];
+    minv[k] = dyn_minv(s, cl->atom[k]);
     dr[k][0] = dr[k][1] = dr[k][2] = 0.0;
   }
   double g[12][3], lam[12] = {0};
@@ -997,7 +1071,7 @@ static int shake_cluster(const or_sys *s, const or_cluster *cl, const uint64_t *
 static void rattle_cluster(const or_sys *s, const or_cluster *cl, const uint64_t *u, const double sc[3], double *v) {
   int na = cl->natom, nc = cl->ncons;
   double r[8][3], minv[8];
-  for (int k = 0; k < na; k++) { fx_disp(u, cl->atom[0], cl->atom[k], sc, r[k]); minv[k] = 1.0 / s->t.mass[cl->atom[k]]; }
+  for (int k = 0; k < na; k++) { fx_disp(u, cl->atom[0], cl->atom[k], sc, r[k]); minv[k] = dyn_minv(s, cl->atom[k]); }
   double rc_[12][3], A[12][12], b[12];
   for (int c = 0; c < nc; c++)
     for (int d = 0; d < 3; d++) rc_[c][d] = r[cl->c1[c]][d] - r[cl->c2[c]][d];
@@ -1048,8 +1122,8 @@ double or_kinetic(or_sys *s, const double *v) {
 }
 
 double or_temperature(or_sys *s, const double *v) {
-  /* T = 2 KE / (RC (3N - 6 - Ncons)) (src/evalprops.cu:221-225) */
-  return 2.0 * or_kinetic(s, v) / (OR_RC * (3.0 * s->t.n_atoms - 6.0 - s->t.n_constraints));
+  /* T = 2 KE / (RC (3 N_mobile - 6 - Ncons)) (src/evalprops.cu:221-225) */
+  return 2.0 * or_kinetic(s, v) / (OR_RC * (3.0 * (s->t.n_atoms - s->n_fixed) - 6.0 - s->n_cons_total));
 }
 
 /* Velocity Verlet with SHAKE after the drift and RATTLE after the second kick
@@ -1066,7 +1140,7 @@ int or_step(or_sys *s, uint64_t *u, double *v, int nsteps) {
   int rc = 0;
   for (int st = 0; st < nsteps; st++) {
     for (int i = 0; i < n; i++) {
-      double hm = 0.5 * dt / s->t.mass[i];
+      double hm = 0.5 * dt * dyn_minv(s, i);
       for (int d = 0; d < 3; d++) v[3 * i + d] += hm * f[3 * i + d];
     }
     memcpy(uold, u, sizeof(uint64_t) * 3 * (size_t)n);
@@ -1085,7 +1159,7 @@ int or_step(or_sys *s, uint64_t *u, double *v, int nsteps) {
     }
     or_forces(s, u, f, e, vir, OR_ALL);
     for (int i = 0; i < n; i++) {
-      double hm = 0.5 * dt / s->t.mass[i];
+      double hm = 0.5 * dt * dyn_minv(s, i);
       for (int d = 0; d < 3; d++) v[3 * i + d] += hm * f[3 * i + d];
     }
     for (int c = 0; c < s->n_clusters; c++) rattle_cluster(s, &s->clusters[c], u, sc, v);
@@ -1135,15 +1209,18 @@ void or_maxwell(or_sys *s, const uint64_t *u, double T, uint64_t seed, uint64_t 
     or_philox(ctr, key, r);
     double u1 = ((double)r[0] + 0.5) * (1.0 / 4294967296.0), u2 = ((double)r[1] + 0.5) * (1.0 / 4294967296.0);
     double u3 = ((double)r[2] + 0.5) * (1.0 / 4294967296.0), u4 = ((double)r[3] + 0.5) * (1.0 / 4294967296.0);
-    double m = s->t.mass[i], vm = sqrt(OR_RC * T / m);
+    double m = s->t.mass[i], vm = (s->mobile && !s->mobile[i]) ? 0.0 : sqrt(OR_RC * T / m);
     double ra = sqrt(-2.0 * log(u1)), rb = sqrt(-2.0 * log(u3));
     v[3 * i] = vm * ra * cos(2.0 * OR_PI * u2);
     v[3 * i + 1] = vm * ra * sin(2.0 * OR_PI * u2);
     v[3 * i + 2] = vm * rb * cos(2.0 * OR_PI * u4);
     for (int d = 0; d < 3; d++) P[d] += m * v[3 * i + d];
   }
+  /* momentum removed over the mobile atoms */
+  int nm = n - s->n_fixed;
   for (int i = 0; i < n; i++)
-    for (int d = 0; d < 3; d++) v[3 * i + d] -= P[d] / n / s->t.mass[i];
+    if (!s->mobile || s->mobile[i])
+      for (int d = 0; d < 3; d++) v[3 * i + d] -= P[d] / nm / s->t.mass[i];
   double sc[3] = {s->p.box[0] * TWO_POW_M64, s->p.box[1] * TWO_POW_M64, s->p.box[2] * TWO_POW_M64};
   for (int c = 0; c < s->n_clusters; c++) rattle_cluster(s, &s->clusters[c], u, sc, v);
 }

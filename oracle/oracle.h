@@ -33,7 +33,8 @@ enum {
   OR_PME = 8,         /* smooth PME reciprocal space (order 4) */
   OR_EWALD = 16,      /* direct Ewald k-sum reciprocal space (replaces OR_PME) */
   OR_SELF = 32,       /* Ewald self energy */
-  OR_ALL = OR_REAL | OR_EXCL | OR_BONDED | OR_PME | OR_SELF
+  OR_RESTRAINT = 64,  /* COM restraint forces (when set by or_set_com_restraint) */
+  OR_ALL = OR_REAL | OR_EXCL | OR_BONDED | OR_PME | OR_SELF | OR_RESTRAINT
 };
 
 typedef struct or_topo {
@@ -85,6 +86,19 @@ int64_t or_count_pairs_within(or_sys *s, const uint64_t *u, double rcut);
 int64_t or_cluster_pairs(or_sys *s, const uint64_t *u, int32_t nx, int32_t ny,
                          int32_t *atom_column, int32_t *slot_atom, int64_t n_slots_cap,
                          int64_t *n_slots, int32_t *pairs, uint64_t *masks, int64_t cap);
+
+/* Fixed atoms (consfix; src/prmread.cu:699-722,1665-1728; mobility v.w of the
+ * half kick, src/mdcuda.cu:70-80): fixed[i] != 0 holds atom i (zero inverse
+ * mass in the kicks, SHAKE and RATTLE, zero Maxwell velocity; constraints
+ * between fixed atoms dropped; temperature over 3 N_mobile - 6 - N_cons).
+ * NULL releases every atom.  Returns 0, or -1 if every atom would be fixed. */
+int or_set_fixed(or_sys *s, const uint8_t *fixed);
+/* Harmonic COM restraint (reduce_COM src/cell_based_kernels.cu:713-866,
+ * restraint_force src/mdcuda.cu:481-516): group[i] in {0, 1, -1}; with flag
+ * OR_RESTRAINT, every atom of group 1 receives fc * (dist + c0 - c1), minimum
+ * image, c_g the mass-weighted mean of group g's positions in [-L/2, L/2).
+ * No energy.  group NULL or fc 0 removes it. */
+int or_set_com_restraint(or_sys *s, const int32_t *group, const double dist[3], double fc);
 
 /* Velocity Verlet with SHAKE/RATTLE: nsteps steps; u, v updated in place. */
 int or_step(or_sys *s, uint64_t *u, double *v, int nsteps);

@@ -15,8 +15,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 E_NAMES = ["bond", "angle", "ub", "impr", "dihed", "vdw", "elec", "pme", "ewexcl", "ewself"]
-REAL, EXCL, BONDED, PME, EWALD, SELF = 1, 2, 4, 8, 16, 32
-ALL = REAL | EXCL | BONDED | PME | SELF
+REAL, EXCL, BONDED, PME, EWALD, SELF, RESTRAINT = 1, 2, 4, 8, 16, 32, 64
+ALL = REAL | EXCL | BONDED | PME | SELF | RESTRAINT
+RC = 0.001987  # kcal/(mol K), src/defs.in.h:156
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -84,6 +85,10 @@ def lib():
         L.or_constraint_residual.argtypes = [C.c_void_p, _up, _dp, _dp]
         L.or_constraint_residual.restype = C.c_double
         L.or_fft3d.argtypes = [_ip, _dp, _dp, C.c_int]
+        L.or_set_fixed.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
+        L.or_set_fixed.restype = C.c_int
+        L.or_set_com_restraint.argtypes = [C.c_void_p, _ip, _dp, C.c_double]
+        L.or_set_com_restraint.restype = C.c_int
         L.or_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         _lib = L
     return _lib
@@ -127,7 +132,29 @@ class Oracle:
         self.t, self.p = t, p
         self.n = t.n_atoms
         self.box = np.array(params["box"], dtype=np.float64)
+        self.fixed = np.zeros(self.n, bool)
         self.h = C.c_void_p(lib().or_create(C.byref(t), C.byref(p)))
+
+    def set_fixed(self, fixed):
+        """consfix mask (None releases every atom)."""
+        if fixed is None:
+            self.fixed = np.zeros(self.n, bool)
+            assert lib().or_set_fixed(self.h, None) == 0
+            return
+        m = np.ascontiguousarray(np.asarray(fixed).astype(np.uint8))
+        self._keep["fixed"] = m
+        assert lib().or_set_fixed(self.h, _p(m, C.c_uint8)) == 0
+        self.fixed = m.astype(bool)
+
+    def set_com_restraint(self, group, dist=(10.0, 10.0, 20.0), fc=0.01):
+        """consharm: group[i] in {0, 1, -1}; group-1 atoms receive fc * (dist + c0 - c1)."""
+        if group is None:
+            assert lib().or_set_com_restraint(self.h, None, None, 0.0) == 0
+            return
+        g = np.ascontiguousarray(np.asarray(group, dtype=np.int32))
+        dd = np.ascontiguousarray(np.asarray(dist, dtype=np.float64))
+        self._keep["com_grp"], self._keep["com_d"] = g, dd
+        assert lib().or_set_com_restraint(self.h, _p(g, C.c_int32), _p(dd, C.c_double), float(fc)) == 0
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -217,11 +244,12 @@ class Oracle:
                 p2 = (Fp * Fp).sum(1)
                 p2 = np.where(p2 < 1e-8, f2, p2)
                 g = np.where(p2 > 0, f2 / np.where(p2 > 0, p2, 1.0), 0.0)
-                H = (f + g[:, None] * H) * cgfac
-                Fp = f.copy()
-                h = H
+                Hn = (f + g[:, None] * H) * cgfac
+                H = np.where(self.fixed[:, None], H, Hn)  # fixed atoms keep no direction
+                Fp = np.where(self.fixed[:, None], Fp, f)
+                h = np.where(self.fixed[:, None], 0.0, H)
             else:
-                h = f * sdfac
+                h = np.where(self.fixed[:, None], 0.0, f * sdfac)
             u = u + np.rint(h / sc).astype(np.int64).view(np.uint64)
             u, _ = self.shake_positions(uold, u)
             fn, e, _ = self.forces(u)

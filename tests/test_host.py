@@ -179,3 +179,35 @@ def test_dcd_layout(tmp_path):
         for k in range(3):
             col = np.frombuffer(recs[4 + 4 * f + k], dtype="<f4")
             assert np.array_equal(col, (x[:, k] + shift).astype(np.float32))
+
+
+def test_deck_restraint_and_fixed_segments(tmp_path):
+    """consharm / consharmdist / consharmfc select groups by the CRD segment name,
+    consfix fixes atoms by the PSF segment name (both case-insensitive, as the
+    reference's cicompare)."""
+    topo, xyz, params = systems.slab(n_chains=3, n_waters=40, L=22.0, seed=2, K=(24, 24, 24))
+    n = topo["n_atoms"]
+    crd = np.array(["BULK"] * n, dtype=object)
+    crd[:38] = "MEMA"
+    crd[38:76] = "memb"
+    psf = np.array(["SYS"] * n, dtype=object)
+    psf[76:114] = "FIX1"
+    psf[-6:] = "fix2"
+    path = fz_synth.write_deck(topo, xyz, dict(params, dt_ps=0.001), str(tmp_path), "seg", psf_seg=psf, crd_seg=crd,
+                               extra=["consharm mema MEMB", "consharmdist 1.5 -2 3", "consharmfc 0.25",
+                                      "consfix 2 fix1 FIX2"])
+    deck = fenzi.Deck(path)
+    group, dist, fc = deck.com_restraint()
+    assert list(np.flatnonzero(group == 0)) == list(range(38))
+    assert list(np.flatnonzero(group == 1)) == list(range(38, 76))
+    assert (group[76:] == -1).all()
+    assert np.allclose(dist, [1.5, -2.0, 3.0]) and fc == 0.25
+    fixed = deck.fixed_atoms()
+    assert list(np.flatnonzero(fixed)) == list(range(76, 114)) + list(range(n - 6, n))
+    deck.close()
+    # without the keywords: no restraint, nothing fixed
+    path = fz_synth.write_deck(topo, xyz, dict(params, dt_ps=0.001), str(tmp_path), "plain")
+    deck = fenzi.Deck(path)
+    assert deck.com_restraint() is None
+    assert deck.fixed_atoms().sum() == 0
+    deck.close()

@@ -401,3 +401,96 @@ def test_erfc_fit_relative_error():
     t = 1 / (1 + vals["FZ_ERFC_P"] * x)
     y = np.exp(-x * x) * t * np.polyval(c[::-1], t)
     assert np.abs(y / erfc(x) - 1).max() < 2e-7
+
+
+def _free_atoms(masses):
+    """Non-interacting atoms of the given masses (no charges, no LJ, no terms)."""
+    n = len(masses)
+    t = _two_atom_bond(0.0, 1.0)
+    t.update(n_atoms=n, charge=np.zeros(n), mass=np.asarray(masses, float), type=np.zeros(n, np.int32),
+             n_bonds=0, bond_atoms=np.zeros(0, np.int32), bond_k=np.zeros(0), bond_r0=np.zeros(0))
+    return t
+
+
+def test_com_restraint_is_the_gradient_of_a_harmonic_well():
+    """With a one-atom group 1 the restraint force is -grad of fc/2 |mi(x1 - c0 - d)|^2
+    (minimum image), c0 the mass-weighted centre of group 0; group 0 and
+    ungrouped atoms feel nothing."""
+    L = 20.0
+    params = dict(box=(L, L, L), cutoff=5.0, cuton=4.0, pairlist=6.0, beta=0.3, K=(16, 16, 16))
+    masses = [12.0, 16.0, 1.0, 14.0]
+    o = oracle.Oracle(_free_atoms(masses), params)
+    group = [0, 0, 1, -1]
+    dist, fc = np.array([3.0, -2.0, 6.5]), 0.7
+    o.set_com_restraint(group, dist, fc)
+    # atom 2 sits across the boundary from group 0's centre, so the minimum image matters
+    x = np.array([[1.0, 2.0, -3.0], [2.5, 1.0, -2.0], [-8.5, 4.0, 7.0], [0.0, 0.0, 0.0]])
+
+    def well(xx):
+        c0 = (masses[0] * xx[0] + masses[1] * xx[1]) / (masses[0] + masses[1])
+        r = xx[2] - c0 - dist
+        r -= L * np.round(r / L)
+        return 0.5 * fc * (r ** 2).sum()
+
+    f, _, _ = o.forces(o.to_fixed(x), flags=oracle.RESTRAINT)
+    h = 1e-5
+    for k in range(3):
+        xp, xm = x.copy(), x.copy()
+        xp[2, k] += h
+        xm[2, k] -= h
+        assert abs(f[2, k] + (well(xp) - well(xm)) / (2 * h)) < 1e-7
+    assert np.abs(f[[0, 1, 3]]).max() == 0.0
+    assert np.abs(f[2]).max() > 1.0
+
+
+def test_com_restraint_uses_mass_weighted_centres():
+    """Moving two group-0 atoms so that their mass-weighted centre stays put leaves
+    the force unchanged; translating group 0 by t changes it by fc t; every atom
+    of group 1 receives the same force."""
+    L = 30.0
+    params = dict(box=(L, L, L), cutoff=5.0, cuton=4.0, pairlist=6.0, beta=0.3, K=(16, 16, 16))
+    masses = [12.0, 16.0, 1.0, 14.0, 2.0]
+    o = oracle.Oracle(_free_atoms(masses), params)
+    o.set_com_restraint([0, 0, 1, 1, -1], (1.0, 2.0, 3.0), 0.25)
+    x = np.array([[1.0, 2.0, -3.0], [2.5, 1.0, -2.0], [4.0, 4.0, 4.0], [5.0, 3.0, 6.0], [0.0, 0.0, 0.0]])
+    f0, _, _ = o.forces(o.to_fixed(x), flags=oracle.RESTRAINT)
+    assert np.array_equal(f0[2], f0[3])
+    y = x.copy()
+    step = np.array([0.4, -0.3, 0.2])
+    y[0] += step * masses[1]
+    y[1] -= step * masses[0]
+    f1, _, _ = o.forces(o.to_fixed(y), flags=oracle.RESTRAINT)
+    assert np.abs(f1 - f0).max() < 1e-9
+    t = np.array([0.5, -0.25, 1.0])
+    z = x.copy()
+    z[:2] += t
+    f2, _, _ = o.forces(o.to_fixed(z), flags=oracle.RESTRAINT)
+    assert np.abs(f2[2] - f0[2] - 0.25 * t).max() < 1e-9
+
+
+def test_fixed_atoms_stay_put_and_constraints_between_them_are_dropped():
+    topo, xyz, params = systems.water_box(150, 17.0, seed=4)
+    params = dict(params, cutoff=8.0, cuton=6.0, pairlist=9.0, K=(20, 20, 20), dt_ps=0.001)
+    o = oracle.Oracle(topo, params)
+    fixed = np.zeros(topo["n_atoms"], np.uint8)
+    fixed[:30] = 1          # waters 0-9 entirely
+    o.set_fixed(fixed)
+    u0 = o.to_fixed(xyz)
+    v = o.maxwell(u0, 300.0, 2, 0)
+    assert np.abs(v[fixed == 1]).max() == 0.0
+    m = np.asarray(topo["mass"])
+    assert np.abs((m[:, None] * v).sum(0)).max() < 1e-9  # whole molecules fixed: momentum-free draw
+    fixed[30] = 1           # and the oxygen of water 10 (its constraints still act on its hydrogens)
+    o.set_fixed(fixed)
+    v = o.maxwell(u0, 300.0, 2, 0)
+    assert np.abs(v[fixed == 1]).max() == 0.0
+    u, v, rc = o.step(u0, v, 5)
+    assert rc == 0
+    assert np.array_equal(u[fixed == 1], u0[fixed == 1])
+    assert np.abs(v[fixed == 1]).max() == 0.0
+    assert not np.array_equal(u[31], u0[31])  # water 10's hydrogens move
+    res, vres = o.constraint_residual(u, v)
+    assert res < params["shake_tol"] and vres < 1e-9
+    # 3 N_mobile - 6 - N_constraints: the 10 fixed waters' 30 constraints are gone
+    dof = 3 * (topo["n_atoms"] - 31) - 6 - (topo["n_constraints"] - 30)
+    assert abs(o.temperature(v) - 2 * o.kinetic(v) / (oracle.RC * dof)) < 1e-12 * o.temperature(v)
