@@ -756,7 +756,14 @@ int Engine::prepare() {
   FZ_TRY(make_work(L_.w_ffty, fy_));
   FZ_TRY(make_work(L_.w_fftz, fz_));
   FZ_TRY(make_work(L_.w_bricks, bricks));
-  L_.pair_grid = tuned_grid_ > 0 ? tuned_grid_ : 2048;
+  // Default persistent pair grid: about 16 list chunks per block (a list holds
+  // ~19 cluster-pair entries per atom at water density, 32 entries per chunk),
+  // so that small systems leave compute units to the bonded and PME branches
+  // running beside the pair kernel.
+  long long atoms_total = 0;
+  for (auto* s : sys_) atoms_total += s->dv.n_atoms;
+  const int grid0 = (int)std::min<long long>(4096, std::max<long long>(256, (atoms_total / 27 + 63) / 64 * 64));
+  L_.pair_grid = tuned_grid_ > 0 ? tuned_grid_ : grid0;
   prepared_ = true;
   gens_.clear();
   for (auto* s : sys_) gens_.push_back(s->gen);
