@@ -72,6 +72,11 @@ static int build_ccl(HostTopo& h) {
   for (int c = 0; c < (int)h.cons_ab.size(); c++)
     if (!(h.fixed[h.cons_ab[c].x] && h.fixed[h.cons_ab[c].y])) act.push_back(c);
   h.n_cons_active = (int)act.size();
+  std::vector<uint8_t> in_cl(n, 0);
+  for (int c : act) in_cl[h.cons_ab[c].x] = in_cl[h.cons_ab[c].y] = 1;
+  h.free_atoms.clear();
+  for (int i = 0; i < n; i++)
+    if (!in_cl[i]) h.free_atoms.push_back(i);
   if (act.empty()) return FZ_OK;
   std::vector<int> par(n);
   std::iota(par.begin(), par.end(), 0);
@@ -311,6 +316,7 @@ int System::alloc_device() {
   d.pi2_beta2 = kPi * kPi / (par.beta * par.beta);
   d.use_excl_list = h.use_excl_list;
   d.n_ccl = (int)h.ccl_atoms.size();
+  d.n_free = (int)h.free_atoms.size();
   d.n_bonds = (int)h.bond_ij.size();
   d.n_angles = (int)h.angle_ijk.size();
   d.n_diheds = (int)h.dihe_ijkl.size();
@@ -325,6 +331,7 @@ int System::alloc_device() {
   FZ_TRY(dalloc(&d.q, n)); FZ_TRY(dalloc(&d.minv, n)); FZ_TRY(dalloc(&d.type, n));
   FZ_TRY(up(d.q, h.q)); FZ_TRY(up(d.minv, h.minv)); FZ_TRY(up(d.type, h.type));
   FZ_TRY(dalloc(&d.mdyn, n));
+  FZ_TRY(dalloc(&d.tick, 1));
   FZ_TRY(upload_mdyn());
   FZ_TRY(dalloc(&d.com_grp, n));
   FZ_TRY(dalloc(&d.atom_slot, n)); FZ_TRY(dalloc(&d.atom_col, n));
@@ -379,6 +386,7 @@ int System::alloc_device() {
   FZ_TRY(dalloc(&d.ccl_atoms, h.ccl_atoms.size())); FZ_TRY(up(d.ccl_atoms, h.ccl_atoms));
   FZ_TRY(dalloc(&d.ccl_cons, h.ccl_cons.size())); FZ_TRY(up(d.ccl_cons, h.ccl_cons));
   FZ_TRY(dalloc(&d.ccl_d2, h.ccl_d2.size())); FZ_TRY(up(d.ccl_d2, h.ccl_d2));
+  FZ_TRY(dalloc(&d.free_atoms, h.free_atoms.size())); FZ_TRY(up(d.free_atoms, h.free_atoms));
   // PME
   {
     const size_t K3 = (size_t)d.K[0] * d.K[1] * d.K[2];
@@ -485,7 +493,7 @@ int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   FZ_TRY(dalloc(&d.search_cnt, clusters_cap + 1));
   FZ_TRY(dalloc(&d.search_tmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));
   FZ_TRY(dalloc(&d.search_mtmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));
-  FZ_TRY(dalloc(&d.pl_off, clusters_cap + 1));
+  FZ_TRY(dalloc(&d.tile_cnt, (clusters_cap + 63) / 64));
   FZ_TRY(dalloc(&d.pl_i, pairs_cap, false));
   FZ_TRY(dalloc(&d.pl_j, pairs_cap, false));
   FZ_TRY(dalloc(&d.pl_mask, pairs_cap, false));
@@ -520,6 +528,8 @@ int System::set_fixed(const uint8_t* fixed) {
     FZ_TRY(dalloc(&d.ccl_cons, topo.ccl_cons.size())); FZ_TRY(up(d.ccl_cons, topo.ccl_cons));
     FZ_TRY(dalloc(&d.ccl_d2, topo.ccl_d2.size())); FZ_TRY(up(d.ccl_d2, topo.ccl_d2));
   }
+  d.n_free = (int)topo.free_atoms.size();
+  FZ_TRY(dalloc(&d.free_atoms, topo.free_atoms.size())); FZ_TRY(up(d.free_atoms, topo.free_atoms));
   FZ_TRY(upload_mdyn());
   std::vector<double> v((size_t)3 * n);
   FZ_TRY(download_state(nullptr, v.data(), nullptr));
@@ -723,7 +733,7 @@ int Engine::prepare() {
     L_.lj_sep &= s->dv.lj_sep;
     L_.com_any |= s->dv.com_on;
   }
-  std::vector<int> atoms, slots, cols, clus, terms, ccl, fx_, fy_, fz_, bricks;
+  std::vector<int> atoms, slots, cols, clus, terms, ccl, stp, fx_, fy_, fz_, bricks;
   int kmax = 4;
   for (auto* s : sys_) kmax = std::max({kmax, s->dv.K[0], s->dv.K[1], s->dv.K[2]});
   const int csz = L_.mixed ? 8 : 16;
@@ -740,6 +750,7 @@ int Engine::prepare() {
     const int nt = d.n_bonds + d.n_angles + d.n_diheds + d.n_imprs + d.n_ew + d.n_p14;
     terms.push_back((nt + 255) / 256);
     ccl.push_back((d.n_ccl + 127) / 128);
+    stp.push_back((d.n_ccl + d.n_free + 127) / 128);
     const int ng = (d.Kxh + NL - 1) / NL;
     fx_.push_back((d.K[1] * d.K[2] + NL - 1) / NL);
     fy_.push_back(d.K[2] * ng);
@@ -752,6 +763,7 @@ int Engine::prepare() {
   FZ_TRY(make_work(L_.w_clus, clus));
   FZ_TRY(make_work(L_.w_terms, terms));
   FZ_TRY(make_work(L_.w_ccl, ccl));
+  FZ_TRY(make_work(L_.w_step, stp));
   FZ_TRY(make_work(L_.w_fftx, fx_));
   FZ_TRY(make_work(L_.w_ffty, fy_));
   FZ_TRY(make_work(L_.w_fftz, fz_));
@@ -764,6 +776,9 @@ int Engine::prepare() {
   for (auto* s : sys_) atoms_total += s->dv.n_atoms;
   const int grid0 = (int)std::min<long long>(4096, std::max<long long>(256, (atoms_total / 27 + 63) / 64 * 64));
   L_.pair_grid = tuned_grid_ > 0 ? tuned_grid_ : grid0;
+  // Spread by bricks from 64K atoms per launch; below, the single atomic kernel
+  // is cheaper than the brick pipeline's five launches.  Same grid either way.
+  L_.spread_bricks = atoms_total >= 65536 ? 1 : 0;
   prepared_ = true;
   gens_.clear();
   for (auto* s : sys_) gens_.push_back(s->gen);
@@ -776,8 +791,8 @@ int Engine::ensure() {
   return stale ? prepare() : FZ_OK;
 }
 
-void Engine::record_forces(bool ev, bool lite) {
-  launch_nblist(L_, true);
+void Engine::record_forces(bool ev, bool lite, bool check) {
+  launch_nblist(L_, check);
   launch_gather_sorted(L_);
   // fork: bonded and PME beside the pair kernel (errors of these calls surface
   // at the stream's next synchronisation or at capture end)
@@ -798,11 +813,9 @@ void Engine::record_forces(bool ev, bool lite) {
 }
 
 void Engine::record_step(bool ev) {
-  launch_kick_drift(L_);
-  launch_shake(L_);
-  record_forces(ev);
-  launch_kick(L_);
-  launch_rattle(L_);
+  launch_step_first(L_);
+  record_forces(ev, false, false);
+  launch_step_second(L_);
   if (ev) launch_kinetic(L_);
 }
 
@@ -850,6 +863,7 @@ int Engine::grow_and_reset(int si, int bits) {
   FZ_HIP(hipMemset(d.flags, 0, F_NSLOT * sizeof(int)));
   FZ_HIP(hipMemset(d.col_count, 0, d.ncol * sizeof(int)));
   FZ_HIP(hipMemset(d.col_fill, 0, d.ncol * sizeof(int)));
+  FZ_HIP(hipMemset(d.tick, 0, sizeof(unsigned)));
   s->need_rebuild = true;
   return FZ_OK;
 }
@@ -1087,31 +1101,29 @@ int Engine::phase_times(int nsteps, double* out) {
   for (int k = 0; k < NP; k++) out[k] = 0.0;
   int rc = FZ_OK;
   for (int st = 0; st < nsteps && rc == FZ_OK; st++) {
-    hipEventRecord(ev[0], L_.stream);
-    launch_kick_drift(L_);
-    launch_shake(L_);
-    hipEventRecord(ev[1], L_.stream);
-    launch_nblist(L_, true);
-    hipEventRecord(ev[2], L_.stream);
+    (void)hipEventRecord(ev[0], L_.stream);
+    launch_step_first(L_);
+    (void)hipEventRecord(ev[1], L_.stream);
+    launch_nblist(L_, false);
+    (void)hipEventRecord(ev[2], L_.stream);
     launch_gather_sorted(L_);
-    hipEventRecord(ev[3], L_.stream);
+    (void)hipEventRecord(ev[3], L_.stream);
     launch_pair(L_, 0);
-    hipEventRecord(ev[4], L_.stream);
+    (void)hipEventRecord(ev[4], L_.stream);
     launch_bonded(L_, false);
-    hipEventRecord(ev[5], L_.stream);
+    (void)hipEventRecord(ev[5], L_.stream);
     launch_pme_part(L_, 0, false);
-    hipEventRecord(ev[6], L_.stream);
+    (void)hipEventRecord(ev[6], L_.stream);
     launch_pme_part(L_, 1, false);
-    hipEventRecord(ev[7], L_.stream);
+    (void)hipEventRecord(ev[7], L_.stream);
     launch_pme_part(L_, 2, false);
-    hipEventRecord(ev[8], L_.stream);
-    launch_kick(L_);
-    launch_rattle(L_);
-    hipEventRecord(ev[9], L_.stream);
+    (void)hipEventRecord(ev[8], L_.stream);
+    launch_step_second(L_);
+    (void)hipEventRecord(ev[9], L_.stream);
     if (hipEventSynchronize(ev[9]) != hipSuccess) rc = set_error(FZ_ERR_HIP, "event synchronize failed");
     for (int k = 0; k < NP && rc == FZ_OK; k++) {
       float ms = 0.f;
-      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
       out[k] += ms;
     }
   }

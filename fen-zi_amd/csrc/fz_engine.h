@@ -43,6 +43,7 @@ struct HostTopo {
   int use_excl_list = 0;
   std::vector<int> excl_off, excl_list;
   std::vector<int4> ccl_atoms, ccl_cons; std::vector<double4> ccl_d2;
+  std::vector<int> free_atoms;  // atoms in no constraint cluster, ascending
   int n_constraints = 0;
   std::vector<int2> cons_ab; std::vector<double> cons_len;  // the constraint list as given
   std::vector<uint8_t> fixed;  // consfix: atoms held in place
@@ -136,7 +137,7 @@ class Engine {
   int make_work(Work& w, const std::vector<int>& blocks);
   int capture(bool ev, hipGraphExec_t* out);
   void record_step(bool ev);
-  void record_forces(bool ev, bool lite = false);
+  void record_forces(bool ev, bool lite = false, bool check = true);
   int grow_and_reset(int s, int bits);
   void drop_graphs();
   int upload_desc();

@@ -28,19 +28,9 @@ __device__ __forceinline__ double take_force(const SysDev& d, int sl, int comp, 
   }
 }
 
-// first half kick + drift; clears the force accumulators, the energy
-// accumulators and the rebuild flag for the coming evaluation
+// first half kick + drift of atom i; clears its force accumulators
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_kick_drift(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
-  const SysDev& d = S[s_];
-  const int lb = blockIdx.x - pre[s_];
-  if (lb == 0) {
-    if (threadIdx.x < A_NSLOT) d.acc[threadIdx.x] = 0ull;
-    if (threadIdx.x == 0) d.flags[F_REBUILD] = 0;
-  }
-  const int i = lb * 256 + threadIdx.x;
-  if (i >= d.n_atoms) return;
+__device__ __forceinline__ void kick_drift_atom(const SysDev& d, int i) {
   const int sl = d.atom_slot[i];
   const double h = 0.5 * d.dt * d.mdyn[i];
   const double vx = d.vx[i] + h * take_force<MIXED>(d, sl, 0, true);
@@ -54,12 +44,9 @@ __global__ __launch_bounds__(256) void k_kick_drift(const SysDev* __restrict__ S
   d.uz[i] = uz + (unsigned long long)__double2ll_rn(d.dt * vz * d.isc[2]);
 }
 
+// second half kick of atom i
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_kick(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
-  const SysDev& d = S[s_];
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
-  if (i >= d.n_atoms) return;
+__device__ __forceinline__ void kick_atom(const SysDev& d, int i) {
   const int sl = d.atom_slot[i];
   const double h = 0.5 * d.dt * d.mdyn[i];
   d.vx[i] += h * take_force<MIXED>(d, sl, 0, false);
@@ -138,12 +125,7 @@ __device__ __forceinline__ void load_rel(const SysDev& d, const CCl& k, const un
   }
 }
 
-__global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
-  const SysDev& d = S[s_];
-  const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
-  if (c >= d.n_ccl) return;
-  const CCl k = load_ccl(d, c);
+__device__ __forceinline__ void shake_ccl(const SysDev& d, const CCl& k) {
   V3 rn[4], ro[4], dr[4];
   double mi[4];
   load_rel(d, k, d.ux, d.uy, d.uz, rn);
@@ -215,12 +197,14 @@ __global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int
   }
 }
 
-__global__ __launch_bounds__(128) void k_rattle(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+__global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = locate(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
-  if (c >= d.n_ccl) return;
-  const CCl k = load_ccl(d, c);
+  if (c < d.n_ccl) shake_ccl(d, load_ccl(d, c));
+}
+
+__device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k) {
   V3 r[4], v[4];
   double mi[4];
   load_rel(d, k, d.ux, d.uy, d.uz, r);
@@ -259,6 +243,79 @@ __global__ __launch_bounds__(128) void k_rattle(const SysDev* __restrict__ S, in
     }
     d.vx[a] = s.x; d.vy[a] = s.y; d.vz[a] = s.z;
   }
+}
+
+__global__ __launch_bounds__(128) void k_rattle(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
+  if (c < d.n_ccl) rattle_ccl(d, load_ccl(d, c));
+}
+
+// ---- fused step halves ---------------------------------------------------------
+// Thread t of a system handles constraint cluster t (t < n_ccl: kick and drift
+// of its atoms, then SHAKE) or the atom free_atoms[t - n_ccl] of no cluster.
+
+// true when atom i moved more than half the skin since the last list build
+// (the same test as k_check)
+__device__ __forceinline__ bool moved_atom(const SysDev& d, int i) {
+  const double dx = fx_d(d.rx[i], d.ux[i], d.sc[0]);
+  const double dy = fx_d(d.ry[i], d.uy[i], d.sc[1]);
+  const double dz = fx_d(d.rz[i], d.uz[i], d.sc[2]);
+  return dx * dx + dy * dy + dz * dz > d.skin2;
+}
+
+// first half kick, drift, SHAKE, and the list-rebuild test; clears the force
+// and energy accumulators
+template <bool MIXED>
+__global__ __launch_bounds__(128) void k_step_first(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int lb = blockIdx.x - pre[s_];
+  if (lb == 0 && threadIdx.x < A_NSLOT) d.acc[threadIdx.x] = 0ull;
+  const int t = lb * 128 + threadIdx.x;
+  bool move = false;
+  if (t < d.n_ccl) {
+    const CCl k = load_ccl(d, t);
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+      if (k.at[m] >= 0) kick_drift_atom<MIXED>(d, k.at[m]);
+    shake_ccl(d, k);
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+      if (k.at[m] >= 0) move = move || moved_atom(d, k.at[m]);
+  } else if (t < d.n_ccl + d.n_free) {
+    const int i = d.free_atoms[t - d.n_ccl];
+    kick_drift_atom<MIXED>(d, i);
+    move = moved_atom(d, i);
+  }
+  if (__any(move) && (threadIdx.x & 63) == 0) d.flags[F_REBUILD] = 1;
+}
+
+// second half kick and RATTLE
+template <bool MIXED>
+__global__ __launch_bounds__(128) void k_step_second(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int t = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
+  if (t < d.n_ccl) {
+    const CCl k = load_ccl(d, t);
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+      if (k.at[m] >= 0) kick_atom<MIXED>(d, k.at[m]);
+    rattle_ccl(d, k);
+  } else if (t < d.n_ccl + d.n_free) {
+    kick_atom<MIXED>(d, d.free_atoms[t - d.n_ccl]);
+  }
+}
+
+void launch_step_first(const Launch& L) {
+  if (L.mixed) hipLaunchKernelGGL(k_step_first<true>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+  else hipLaunchKernelGGL(k_step_first<false>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+}
+void launch_step_second(const Launch& L) {
+  if (L.mixed) hipLaunchKernelGGL(k_step_second<true>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+  else hipLaunchKernelGGL(k_step_second<false>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
 }
 
 // ---- kinetic energy, velocities ---------------------------------------------------
@@ -460,14 +517,6 @@ __global__ __launch_bounds__(256) void k_zero_forces(const SysDev* __restrict__ 
   F[2 * d.n_slots_cap + sl] = 0ull;
 }
 
-void launch_kick_drift(const Launch& L) {
-  if (L.mixed) hipLaunchKernelGGL(k_kick_drift<true>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
-  else hipLaunchKernelGGL(k_kick_drift<false>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
-}
-void launch_kick(const Launch& L) {
-  if (L.mixed) hipLaunchKernelGGL(k_kick<true>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
-  else hipLaunchKernelGGL(k_kick<false>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
-}
 void launch_shake(const Launch& L) {
   if (L.w_ccl.total) hipLaunchKernelGGL(k_shake, dim3(L.w_ccl.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_ccl.pre);
 }

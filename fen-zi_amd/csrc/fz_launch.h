@@ -23,6 +23,7 @@ struct Launch {
   Work w_clus;    // 4 clusters (waves) per block
   Work w_terms;   // 256 bonded terms per block
   Work w_ccl;     // 128 constraint clusters per block
+  Work w_step;    // 128 per block: constraint clusters, then free atoms
   Work w_fftx;    // x-line groups
   Work w_ffty;    // y-line groups
   Work w_fftz;    // z-line groups
@@ -33,18 +34,20 @@ struct Launch {
   int fft_nl = 16;           // lines per FFT workgroup
   int lj_sep = 0;            // every system follows the LJ combination rule
   int com_any = 0;           // some system has a COM restraint
+  int spread_bricks = 1;     // PME spread by bricks (1) or by global atomics (0)
 };
 
-void launch_nblist(const Launch& L, bool always_rebuild_check);
+// check: run the displacement test first (the fused step kernel does it otherwise)
+void launch_nblist(const Launch& L, bool check);
 void launch_gather_sorted(const Launch& L);
 void launch_pair(const Launch& L, int ev);  // 0 forces, 1 + FP32 energies, 2 + FP64 energies
 void launch_bonded(const Launch& L, bool ev);
 void launch_pme(const Launch& L, bool ev, int conv_mode);
 void launch_pme_fft_only(const Launch& L, int conv_mode);
 void launch_pme_part(const Launch& L, int part, bool ev);  // 0 spread, 1 FFT + convolution, 2 gather
-void launch_kick_drift(const Launch& L);
+void launch_step_first(const Launch& L);   // kick, drift, SHAKE, rebuild test
+void launch_step_second(const Launch& L);  // kick, RATTLE
 void launch_shake(const Launch& L);
-void launch_kick(const Launch& L);
 void launch_rattle(const Launch& L);
 void launch_kinetic(const Launch& L);
 void launch_maxwell(const Launch& L, double T, unsigned long long seed, unsigned long long counter);

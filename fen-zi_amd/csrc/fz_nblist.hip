@@ -47,6 +47,23 @@ __device__ __forceinline__ bool skip_build(const SysDev& d) {
   return d.flags[F_REBUILD] == 0 || d.flags[F_OVERFLOW] != 0;
 }
 
+// Last block done: each of the system's nblocks blocks takes a ticket after a
+// fence once its writes are out; the block that draws the last ticket sees
+// every other block's writes and runs the system's single-block step (no block
+// ever waits for another).  The counter is left at zero for the next launch.
+__device__ __forceinline__ bool last_block(unsigned* ctr, int nblocks) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    last = atomicAdd(ctr, 1u) == (unsigned)(nblocks - 1);
+    if (last) atomicExch(ctr, 0u);
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last != 0;
+}
+
 __device__ __forceinline__ int column_of(const SysDev& d, int i) {
   int cx = (int)__umul64hi(d.ux[i], (unsigned long long)d.ncx);
   int cy = (int)__umul64hi(d.uy[i], (unsigned long long)d.ncy);
@@ -60,41 +77,15 @@ __device__ __forceinline__ int column_of(const SysDev& d, int i) {
 // the LDS table holds use global atomics directly.
 constexpr int BIN_LDS = 4096;
 
-__global__ __launch_bounds__(256) void k_bin_count(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  SYS_FROM(pre);
-  if (skip_build(d)) return;
-  __shared__ int hist[BIN_LDS];
-  const bool lds = d.ncol <= BIN_LDS;  // uniform over the block
-  const int i = lb * 256 + threadIdx.x;
-  if (lds) {
-    for (int k = threadIdx.x; k < d.ncol; k += 256) hist[k] = 0;
-    __syncthreads();
-  }
-  if (i < d.n_atoms) {
-    const int c = column_of(d, i);
-    d.atom_col[i] = c;
-    if (lds) atomicAdd(hist + c, 1);
-    else atomicAdd(d.col_count + c, 1);
-  }
-  if (lds) {
-    __syncthreads();
-    for (int k = threadIdx.x; k < d.ncol; k += 256) {
-      const int v = hist[k];
-      if (v) atomicAdd(d.col_count + k, v);
-    }
-  }
-}
-
-// one block per system: clusters per column, exclusive scan, capacity checks
-__global__ __launch_bounds__(1024) void k_col_scan(const SysDev* __restrict__ S, int nsys) {
-  const SysDev& d = S[blockIdx.x];
-  if (skip_build(d)) return;
-  __shared__ int part[1024];
+// clusters per column, exclusive scan, capacity checks (one block of 256 threads)
+__device__ void col_scan_block(const SysDev& d) {
+  constexpr int NT = 256;
+  __shared__ int part[NT];
   __shared__ int bad;
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
   const int n = d.ncol;
-  const int per = (n + 1023) / 1024;
+  const int per = (n + NT - 1) / NT;
   const int c0 = threadIdx.x * per, c1 = min(n, c0 + per);
   int sum = 0;
   for (int c = c0; c < c1; c++) {
@@ -104,7 +95,7 @@ __global__ __launch_bounds__(1024) void k_col_scan(const SysDev* __restrict__ S,
   }
   part[threadIdx.x] = sum;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+  for (int o = 1; o < NT; o <<= 1) {  // Hillis-Steele inclusive scan
     int v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
     __syncthreads();
     part[threadIdx.x] += v;
@@ -116,8 +107,8 @@ __global__ __launch_bounds__(1024) void k_col_scan(const SysDev* __restrict__ S,
     d.col_fill[c] = 0;
     run += (d.col_count[c] + CLUSTER - 1) / CLUSTER;
   }
-  if (threadIdx.x == 1023) {
-    int total = part[1023];
+  if (threadIdx.x == NT - 1) {
+    int total = part[NT - 1];
     d.col_clu0[n] = total;
     *d.n_clusters = total;
     // slot-order multiplier for the PME spread: odd, near 0.618 ns, coprime to ns
@@ -135,6 +126,36 @@ __global__ __launch_bounds__(1024) void k_col_scan(const SysDev* __restrict__ S,
     if (total > d.n_clusters_cap || total * CLUSTER > d.n_slots_cap) ov |= OV_SLOTS;
     if (ov) atomicOr(d.flags + F_OVERFLOW, ov);
   }
+}
+
+
+__global__ __launch_bounds__(256) void k_bin_count(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  SYS_FROM(pre);
+  if (skip_build(d)) return;
+  __shared__ int hist[BIN_LDS];
+  const bool lds = d.ncol <= BIN_LDS;  // uniform over the block
+  const int i = lb * 256 + threadIdx.x;
+  if (lds) {
+    for (int k = threadIdx.x; k < d.ncol; k += 256) hist[k] = 0;
+    __syncthreads();
+  }
+  if (i < d.n_atoms) {
+    const int c = column_of(d, i);
+    d.atom_col[i] = c;
+    if (lds) atomicAdd(hist + c, 1);
+    else atomicAdd(d.col_count + c, 1);
+  }
+  // clear the per-tile entry counts of the coming pair search
+  const int ntile = (d.n_clusters_cap + 63) >> 6, nb = pre[s_ + 1] - pre[s_];
+  for (int k = lb * 256 + threadIdx.x; k < ntile; k += nb * 256) d.tile_cnt[k] = 0;
+  if (lds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < d.ncol; k += 256) {
+      const int v = hist[k];
+      if (v) atomicAdd(d.col_count + k, v);
+    }
+  }
+  if (last_block(d.tick + 0, pre[s_ + 1] - pre[s_])) col_scan_block(d);
 }
 
 // places each atom in its column's range of bin_tmp (order inside a column is
@@ -369,8 +390,9 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
   if (skip_build(d)) return;
   const int lane = threadIdx.x & 63;
   const int il = lane & 7, jl = lane >> 3;
-  const int I = lb * 4 + (threadIdx.x >> 6);
-  if (I >= *d.n_clusters) return;
+  const int I0 = lb * 4 + (threadIdx.x >> 6);
+  if (I0 >= *d.n_clusters) return;
+  const int I = I0;
   const int nsc = d.n_slots_cap;
   const unsigned* su32 = (const unsigned*)d.su;  // high words at odd indices
   const int ai = d.slot_atom[I * CLUSTER + il];
@@ -492,55 +514,48 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
   }
   if (fill) test_hits();
   if (lane == 0) {
-    d.search_cnt[I] = count < SEARCH_TMP_CAP ? count : SEARCH_TMP_CAP;
+    const int kept = count < SEARCH_TMP_CAP ? count : SEARCH_TMP_CAP;
+    d.search_cnt[I] = kept;
+    if (kept) atomicAdd(d.tile_cnt + (I >> 6), kept);
     if (over) atomicOr(d.flags + F_OVERFLOW, OV_SEARCH);
   }
 }
 
-// one block per system: exclusive scan of per-cluster counts into list offsets
-__global__ __launch_bounds__(1024) void k_pair_scan(const SysDev* __restrict__ S, int nsys) {
-  const SysDev& d = S[blockIdx.x];
-  if (skip_build(d)) return;
-  __shared__ long long part[1024];
-  const int n = *d.n_clusters;
-  const int per = (n + 1023) / 1024;
-  const int c0 = threadIdx.x * per, c1 = min(n, c0 + per);
-  long long sum = 0;
-  for (int c = c0; c < c1; c++) sum += d.search_cnt[c];
-  part[threadIdx.x] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  long long run = part[threadIdx.x] - sum;
-  for (int c = c0; c < c1; c++) {
-    d.pl_off[c] = run;
-    run += d.search_cnt[c];
-  }
-  if (threadIdx.x == 1023) {
-    long long total = part[1023];
-    d.pl_off[n] = total;
-    if (total > d.n_pairs_cap) {
-      atomicOr(d.flags + F_OVERFLOW, OV_PAIRS);
-      *d.n_pairs = 0;
-    } else {
-      *d.n_pairs = total;
-    }
-    d.flags[F_NBUILD] += 1;
-  }
-}
-
-// one wave per I cluster: copy its entries into place
+// One wave per I cluster: copy its entries into place.  The list offset of I
+// is the sum of the entry counts of the 64-cluster tiles before I's tile plus
+// the counts of the clusters before I inside it; the list total is the sum of
+// every tile.  Each wave forms both from the tile counts, so no separate scan
+// pass is needed.  A list over capacity is flagged and left unwritten.
 __global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
   const int lane = threadIdx.x & 63;
   const int I = lb * 4 + (threadIdx.x >> 6);
-  if (I >= *d.n_clusters) return;
-  const long long off = d.pl_off[I];
+  const int n = *d.n_clusters;
+  if (I >= n) return;
+  const int t = I >> 6, nt = (n + 63) >> 6;
+  long long before = 0, total = 0;
+  for (int k = lane; k < nt; k += 64) {
+    const long long v = d.tile_cnt[k];
+    total += v;
+    if (k < t) before += v;
+  }
+  const int k0 = (t << 6) + lane;
+  if (k0 < I) before += d.search_cnt[k0];
+  before = wave_sum(before);
+  total = wave_sum(total);
+  if (total > d.n_pairs_cap) {
+    if (I == 0 && lane == 0) {
+      atomicOr(d.flags + F_OVERFLOW, OV_PAIRS);
+      *d.n_pairs = 0;
+    }
+    return;
+  }
+  if (I == 0 && lane == 0) {
+    *d.n_pairs = total;
+    d.flags[F_NBUILD] += 1;
+  }
+  const long long off = before;
   const int cnt = d.search_cnt[I];
   const int* src = d.search_tmp + (size_t)I * SEARCH_TMP_CAP;
   const unsigned long long* msrc = d.search_mtmp + (size_t)I * SEARCH_TMP_CAP;
@@ -551,30 +566,14 @@ __global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S,
   }
 }
 
-// single block: per-system pair-kernel work items (chunks of PAIR_CHUNK entries)
-__global__ void k_chunk_prefix(const SysDev* __restrict__ S, int nsys, int* __restrict__ chunk_pre) {
-  if (threadIdx.x != 0) return;
-  int run = 0;
-  chunk_pre[0] = 0;
-  for (int s = 0; s < nsys; s++) {
-    long long np = S[s].flags[F_OVERFLOW] ? 0 : *S[s].n_pairs;
-    run += (int)((np + PAIR_CHUNK - 1) / PAIR_CHUNK);
-    chunk_pre[s + 1] = run;
-  }
-}
-
 // ---------------------------------------------------------------------------
-void launch_nblist(const Launch& L, bool always_rebuild_check) {
-  (void)always_rebuild_check;
-  hipLaunchKernelGGL(k_check, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+void launch_nblist(const Launch& L, bool check) {
+  if (check) hipLaunchKernelGGL(k_check, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
   hipLaunchKernelGGL(k_bin_count, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
-  hipLaunchKernelGGL(k_col_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.d_sys, L.nsys);
   hipLaunchKernelGGL(k_bin_scatter, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
   hipLaunchKernelGGL(k_col_sort, dim3(L.w_cols.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_cols.pre);
   hipLaunchKernelGGL(k_pair_search, dim3(L.w_clus.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_clus.pre);
-  hipLaunchKernelGGL(k_pair_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.d_sys, L.nsys);
   hipLaunchKernelGGL(k_pair_fill, dim3(L.w_clus.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_clus.pre);
-  hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(64), 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
 }
 
 }  // namespace fz

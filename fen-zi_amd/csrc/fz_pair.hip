@@ -28,8 +28,23 @@ struct Slot64 {
 };
 
 // ---------------------------------------------------------------------------
+// Slot-ordered copies of positions, charges and LJ parameters for the pair
+// kernel.  Block 0 also lays out the pair kernel's work items: per system,
+// the number of PAIR_CHUNK-entry chunks of its list (none after an overflow).
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+__global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
+                                                       int* __restrict__ chunk_pre) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // pair-kernel work items per system, and the rebuild flags are spent
+    int run = 0;
+    chunk_pre[0] = 0;
+    for (int s = 0; s < nsys; s++) {
+      const long long np = S[s].flags[F_OVERFLOW] ? 0 : *S[s].n_pairs;
+      run += (int)((np + PAIR_CHUNK - 1) / PAIR_CHUNK);
+      chunk_pre[s + 1] = run;
+      S[s].flags[F_REBUILD] = 0;
+    }
+  }
   const int s_ = locate(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
@@ -56,9 +71,11 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict_
 
 void launch_gather_sorted(const Launch& L) {
   if (L.mixed)
-    hipLaunchKernelGGL(k_gather_sorted<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    hipLaunchKernelGGL(k_gather_sorted<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre,
+                       L.chunk_pre);
   else
-    hipLaunchKernelGGL(k_gather_sorted<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    hipLaunchKernelGGL(k_gather_sorted<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre,
+                       L.chunk_pre);
 }
 
 // ---------------------------------------------------------------------------

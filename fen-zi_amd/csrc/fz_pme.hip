@@ -63,6 +63,64 @@ __device__ __forceinline__ R bspline4_k(R f, int k) {
 }
 
 // ---------------------------------------------------------------------------
+// Charge spreading with global atomics, for launches over few atoms, where the
+// brick pipeline's five launches cost more than the contention they remove.
+// One atom per wave iteration, one of its 4x4x4 grid points per lane: the 64
+// atomics of an instruction fall on 16 rows of 4 consecutive x points.  Slots
+// are visited with a multiplier coprime to their count (set at each list
+// build), so that concurrent waves work on distant atoms.  The wave's 64 atoms
+// are read and placed on the grid in parallel first (lane k: the k-th atom).
+// Qacc must be zero before; the x pass of the FFT clears it after reading.
+// The mixed-form grid is the same integer sum as the brick path's.
+__device__ __forceinline__ float bcast(float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); }
+__device__ __forceinline__ double bcast(double v, int k) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), k), __builtin_amdgcn_readlane(__double2loint(v), k));
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_spread_atomic(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  const int s_ = locate(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int lane = threadIdx.x & 63;
+  const int t0 = ((blockIdx.x - pre[s_]) * 4 + (threadIdx.x >> 6)) * 64;
+  const int ns = *d.n_clusters * CLUSTER;
+  const unsigned long long mult = (unsigned long long)d.n_clusters[1];
+  const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
+  const int ca = lane & 3, cb = (lane >> 2) & 3, cc = lane >> 4;
+  R q = R(0), fx = R(0), fy = R(0), fz = R(0);
+  int gx0 = 0, gy0 = 0, gz0 = 0;
+  bool has = false;
+  if (t0 + lane < ns) {
+    const int sl = (int)(((unsigned long long)(t0 + lane) * mult) % (unsigned long long)ns);
+    const int a = d.slot_atom[sl];
+    if (a >= 0) {
+      q = (R)d.q[a];
+      has = q != R(0);
+      gx0 = spline_pos<R>(d.ux[a], K0, fx);
+      gy0 = spline_pos<R>(d.uy[a], K1, fy);
+      gz0 = spline_pos<R>(d.uz[a], K2, fz);
+    }
+  }
+  unsigned long long bal = __ballot(has);
+  while (bal) {
+    const int k = __ffsll((long long)bal) - 1;
+    bal &= bal - 1ull;
+    int gx = __builtin_amdgcn_readlane(gx0, k) + ca;
+    int gy = __builtin_amdgcn_readlane(gy0, k) + cb;
+    int gz = __builtin_amdgcn_readlane(gz0, k) + cc;
+    gx -= gx >= K0 ? K0 : 0;
+    gy -= gy >= K1 ? K1 : 0;
+    gz -= gz >= K2 ? K2 : 0;
+    const R v = bcast(q, k) * bspline4_k<R>(bcast(fx, k), ca) * bspline4_k<R>(bcast(fy, k), cb) *
+                bspline4_k<R>(bcast(fz, k), cc);
+    const size_t g = ((size_t)gz * K1 + gy) * K0 + gx;
+    if (MIXED) atomicAdd((unsigned long long*)d.Qacc + g, fixed_f32((float)v));
+    else unsafeAtomicAdd((double*)d.Qacc + g, (double)v);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Charge spreading by bricks.  The grid is cut into bricks of 8^3 points; an
 // atom belongs to the brick holding the first point of its 4^3 stencil, so its
 // stencil lies inside the brick's (8+3)^3 block.  Per step:
@@ -255,6 +313,11 @@ __global__ __launch_bounds__(256) void k_brick_reduce(const SysDev* __restrict__
 
 void launch_spread(const Launch& L) {
   dim3 b(256);
+  if (!L.spread_bricks) {
+    if (L.mixed) hipLaunchKernelGGL(k_spread_atomic<true>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    else hipLaunchKernelGGL(k_spread_atomic<false>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    return;
+  }
   if (L.mixed) {
     hipLaunchKernelGGL(k_brick_count<true>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
     hipLaunchKernelGGL(k_brick_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.d_sys, L.nsys);
@@ -367,9 +430,10 @@ __device__ __forceinline__ const C* stage_tw(C* lds, const C* __restrict__ tw, i
   return lds;
 }
 
-// x pass, forward: real grid -> half spectrum
+// x pass, forward: real grid -> half spectrum (cleared after reading when asked)
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_fft_x_r2c(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre, int NL) {
+__global__ __launch_bounds__(256) void k_fft_x_r2c(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre, int NL,
+                                                    int clear) {
   using R = typename std::conditional<MIXED, float, double>::type;
   using C = typename C2<R>::t;
   extern __shared__ char smem[];
@@ -385,8 +449,15 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const SysDev* __restrict__ S,
     C v; v.x = R(0); v.y = R(0);
     if (l0 + line < (size_t)nlines) {
       const size_t g = (l0 + line) * K0 + x;
-      if (MIXED) v.x = (R)from_fixed_acc(((const unsigned long long*)d.Qacc)[g]);
-      else v.x = (R)((const double*)d.Qacc)[g];
+      if (MIXED) {
+        unsigned long long* Q = (unsigned long long*)d.Qacc;
+        v.x = (R)from_fixed_acc(Q[g]);
+        if (clear) Q[g] = 0ull;
+      } else {
+        double* Q = (double*)d.Qacc;
+        v.x = (R)Q[g];
+        if (clear) Q[g] = 0.0;
+      }
     }
     b0[e] = v;
   }
@@ -601,7 +672,9 @@ template <bool MIXED>
 static void launch_fft_passes(const Launch& L, bool ev, int mode) {
   const size_t lds = L.fft_lds;
   const int NL = L.fft_nl;
-  hipLaunchKernelGGL(k_fft_x_r2c<MIXED>, dim3(L.w_fftx.total), dim3(256), lds, L.stream, L.d_sys, L.nsys, L.w_fftx.pre, NL);
+  // the atomic spread adds into a zero grid: the x pass clears it after reading
+  hipLaunchKernelGGL(k_fft_x_r2c<MIXED>, dim3(L.w_fftx.total), dim3(256), lds, L.stream, L.d_sys, L.nsys, L.w_fftx.pre, NL,
+                     L.spread_bricks ? 0 : 1);
   hipLaunchKernelGGL(k_fft_y<MIXED>, dim3(L.w_ffty.total), dim3(256), lds, L.stream, L.d_sys, L.nsys, L.w_ffty.pre, NL, -1);
   if (ev) hipLaunchKernelGGL((k_fft_z_conv<MIXED, true>), dim3(L.w_fftz.total), dim3(256), lds, L.stream, L.d_sys, L.nsys, L.w_fftz.pre, NL, mode);
   else hipLaunchKernelGGL((k_fft_z_conv<MIXED, false>), dim3(L.w_fftz.total), dim3(256), lds, L.stream, L.d_sys, L.nsys, L.w_fftz.pre, NL, mode);

@@ -47,6 +47,7 @@ struct SysDev {
   int Kxh;               // K[0]/2 + 1
   int use_excl_list;     // 1: exclusions by sorted list, 0: 64-bit window bit vectors
   int n_ccl;             // constraint clusters
+  int n_free;            // atoms in no constraint cluster
   // bonded term counts (flat index order: bonds, angles, dihedrals, impropers, ew pairs, 1-4 pairs)
   int n_bonds, n_angles, n_diheds, n_imprs, n_ew, n_p14;
   // ---- geometry / method parameters ----------------------------------
@@ -95,10 +96,11 @@ struct SysDev {
   unsigned long long *bb; // [6 * n_clusters_cap] lo/hi per axis
   unsigned long long *su; // [3 * n_slots_cap] slot-ordered positions at the last build (x, y, z blocks)
   int *n_clusters;       // [0] clusters of the last build, [1] spread slot multiplier
+  unsigned *tick;        // [1] last-block ticket of the binning count
   int *search_cnt;       // [n_clusters_cap + 1]
   int *search_tmp;       // [n_clusters_cap * SEARCH_TMP_CAP] J of each kept entry
   unsigned long long *search_mtmp; // [n_clusters_cap * SEARCH_TMP_CAP] lane mask of each kept entry
-  long long *pl_off;     // [n_clusters_cap + 1]
+  int *tile_cnt;         // [ceil(n_clusters_cap / 64)] kept entries per 64 consecutive I clusters
   int *pl_i, *pl_j;      // [n_pairs_cap]
   unsigned long long *pl_mask; // [n_pairs_cap]
   long long *n_pairs;    // device scalar
@@ -143,6 +145,7 @@ struct SysDev {
   void *br_part;   // [n_bricks * 1331] brick blocks (int64 fixed point / double)
   // ---- constraints --------------------------------------------------------
   int4 *ccl_atoms;    // up to 4 atoms (-1 = none)
+  int *free_atoms;    // [n_free] atoms in no constraint cluster
   int4 *ccl_cons;     // x,y,z: constraint c as (local a | local b << 8); w: count
   double4 *ccl_d2;    // squared lengths (up to 3)
   // ---- accumulators / flags ---------------------------------------------

@@ -173,6 +173,33 @@ def test_batch_matches_alone_bitwise(ctx):
     b.close()
 
 
+def test_brick_and_atomic_spread_give_the_same_trajectory(ctx):
+    """A launch over 64K atoms or more spreads PME charges by grid bricks, a smaller
+    one by global atomics: three ~23K-atom systems run alone (atomic path) and as
+    one batch (brick path) follow bitwise identical mixed-form trajectories."""
+    # PME grids that are not multiples of the brick edge: blocks wrap across the boundary
+    cfgs = [systems.slab(n_chains=6, n_waters=7700, L=62.0, seed=20 + k, K=(60, 54, 50)) for k in range(3)]
+    assert sum(int(c[0]["n_atoms"]) for c in cfgs) >= 65536
+    assert max(int(c[0]["n_atoms"]) for c in cfgs) < 65536
+    alone = []
+    for topo, xyz, params in cfgs:
+        s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+        s.reassign_velocities(300.0, 6, 0)
+        s.step(10)
+        alone.append(s.get_fixed())
+        s.close()
+    mix = []
+    for topo, xyz, params in cfgs:
+        s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+        s.reassign_velocities(300.0, 6, 0)
+        mix.append(s)
+    b = fenzi.Batch(ctx, mix)
+    b.step(10)
+    for s, u_ref in zip(mix, alone):
+        assert np.array_equal(s.get_fixed(), u_ref)
+    b.close()
+
+
 @pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
 def test_minimize_lowers_energy_and_keeps_constraints(ctx, prec):
     topo, xyz, params = systems.slab()
