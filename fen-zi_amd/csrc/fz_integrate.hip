@@ -28,7 +28,8 @@ __device__ __forceinline__ double take_force(const SysDev& d, int sl, int comp, 
   }
 }
 
-// first half kick + drift of atom i; clears its force accumulators
+// first half kick + drift of atom i (an atom of no constraint cluster); clears
+// its force accumulators
 template <bool MIXED>
 __device__ __forceinline__ void kick_drift_atom(const SysDev& d, int i) {
   const int sl = d.atom_slot[i];
@@ -38,7 +39,6 @@ __device__ __forceinline__ void kick_drift_atom(const SysDev& d, int i) {
   const double vz = d.vz[i] + h * take_force<MIXED>(d, sl, 2, true);
   d.vx[i] = vx; d.vy[i] = vy; d.vz[i] = vz;
   const unsigned long long ux = d.ux[i], uy = d.uy[i], uz = d.uz[i];
-  d.ox[i] = ux; d.oy[i] = uy; d.oz[i] = uz;
   d.ux[i] = ux + (unsigned long long)__double2ll_rn(d.dt * vx * d.isc[0]);
   d.uy[i] = uy + (unsigned long long)__double2ll_rn(d.dt * vy * d.isc[1]);
   d.uz[i] = uz + (unsigned long long)__double2ll_rn(d.dt * vz * d.isc[2]);
@@ -98,21 +98,39 @@ __device__ __forceinline__ bool solve3(const double (&A)[3][3], double (&b)[3], 
   return true;
 }
 
-// Small vectors indexed by run-time cluster-local indices are read and written
-// through select chains over fully unrolled loops, so the cluster state stays
-// in registers (no scratch).
+// The constraints of a cluster enter through its incidence matrix
+// B[j][m] = +1 if atom m is the first atom of constraint j, -1 if it is the
+// second, else 0.  Differences such as r[c1[j]] - r[c2[j]] are formed as sums
+// over m weighted by B: every index is then a compile-time constant and the
+// cluster state stays in registers (indexing by c1/c2 at run time put it in
+// scratch memory).  Each sum has at most two non-zero terms, so it rounds
+// exactly as the plain difference.
 struct V3 { double x, y, z; };
 __device__ __forceinline__ V3 v3sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ double v3dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V3 pick4(const V3 (&p)[4], int i) {
-  V3 r = p[0];
-  if (i == 1) r = p[1];
-  if (i == 2) r = p[2];
-  if (i == 3) r = p[3];
-  return r;
+__device__ __forceinline__ void incidence(const CCl& k, double (&B)[3][4]) {
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int m = 0; m < 4; m++) B[j][m] = (double)((k.c1[j] == m) - (k.c2[j] == m));
 }
-__device__ __forceinline__ double pick4(const double (&p)[4], int i) {
-  return i == 0 ? p[0] : i == 1 ? p[1] : i == 2 ? p[2] : p[3];
+__device__ __forceinline__ V3 bsum(const double (&Bj)[4], const V3 (&p)[4]) {
+  V3 s = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int m = 0; m < 4; m++) { s.x += Bj[m] * p[m].x; s.y += Bj[m] * p[m].y; s.z += Bj[m] * p[m].z; }
+  return s;
+}
+// coupling weights w[j][e] = sum_m B[j][m] B[e][m] mi[m]
+__device__ __forceinline__ void couplings(const double (&B)[3][4], const double (&mi)[4], double (&w)[3][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < 4; m++) s += (B[j][m] * B[e][m]) * mi[m];
+      w[j][e] = s;
+    }
 }
 
 __device__ __forceinline__ void load_rel(const SysDev& d, const CCl& k, const unsigned long long* ux,
@@ -125,28 +143,24 @@ __device__ __forceinline__ void load_rel(const SysDev& d, const CCl& k, const un
   }
 }
 
-__device__ __forceinline__ void shake_ccl(const SysDev& d, const CCl& k) {
-  V3 rn[4], ro[4], dr[4];
-  double mi[4];
-  load_rel(d, k, d.ux, d.uy, d.uz, rn);
-  load_rel(d, k, d.ox, d.oy, d.oz, ro);
+// SHAKE iterations on one cluster: rn and ro are the new and old positions
+// relative to the cluster's first atom, mi the inverse masses (0 for padding);
+// dr receives the position corrections.  False if it did not converge.
+__device__ __forceinline__ bool shake_core(const SysDev& d, const CCl& k, const V3 (&rn)[4], const V3 (&ro)[4],
+                                           const double (&mi)[4], V3 (&dr)[4]) {
+  double B[3][4];
+  incidence(k, B);
 #pragma unroll
-  for (int m = 0; m < 4; m++) {
-    mi[m] = k.at[m] >= 0 ? d.mdyn[k.at[m]] : 0.0;
-    dr[m] = V3{0.0, 0.0, 0.0};
-  }
-  V3 g[3];
+  for (int m = 0; m < 4; m++) dr[m] = V3{0.0, 0.0, 0.0};
+  V3 g[3], r0[3];
   double lam[3] = {0.0, 0.0, 0.0};
   // coupling weights w[j][e]: d(r_c1j - r_c2j)/d(lambda_e) = w[j][e] * g_e
   double w[3][3];
+  couplings(B, mi, w);
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    g[j] = v3sub(pick4(ro, k.c1[j]), pick4(ro, k.c2[j]));
-#pragma unroll
-    for (int e = 0; e < 3; e++) {
-      const int p = k.c1[j], q = k.c2[j], ep = k.c1[e], eq = k.c2[e];
-      w[j][e] = ((p == ep) - (p == eq)) * pick4(mi, p) - ((q == ep) - (q == eq)) * pick4(mi, q);
-    }
+    g[j] = bsum(B[j], ro);   // old constraint vectors
+    r0[j] = bsum(B[j], rn);  // unconstrained new ones
   }
   bool ok = false;
   for (int it = 0; it < 200; it++) {
@@ -154,7 +168,8 @@ __device__ __forceinline__ void shake_ccl(const SysDev& d, const CCl& k) {
     double sig[3], worst = 0.0;
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      rc[j] = v3sub(v3sub(pick4(rn, k.c1[j]), pick4(rn, k.c2[j])), v3sub(pick4(dr, k.c2[j]), pick4(dr, k.c1[j])));
+      const V3 e = bsum(B[j], dr);
+      rc[j] = V3{r0[j].x + e.x, r0[j].y + e.y, r0[j].z + e.z};
       sig[j] = j < k.nc ? v3dot(rc[j], rc[j]) - k.d2[j] : 0.0;
       if (j < k.nc) worst = fmax(worst, fabs(sig[j]) / (2.0 * k.d2[j]));
     }
@@ -175,14 +190,23 @@ __device__ __forceinline__ void shake_ccl(const SysDev& d, const CCl& k) {
       V3 s = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int j = 0; j < 3; j++) {
-        const double f = (k.c1[j] == m ? 1.0 : 0.0) - (k.c2[j] == m ? 1.0 : 0.0);
-        const double l = j < k.nc ? lam[j] * f * mi[m] : 0.0;
+        const double l = j < k.nc ? lam[j] * B[j][m] * mi[m] : 0.0;
         s.x += l * g[j].x; s.y += l * g[j].y; s.z += l * g[j].z;
       }
       dr[m] = s;
     }
   }
-  if (!ok) d.flags[F_SHAKE_FAIL] = 1;
+  return ok;
+}
+
+__device__ __forceinline__ void shake_ccl(const SysDev& d, const CCl& k) {
+  V3 rn[4], ro[4], dr[4];
+  double mi[4];
+  load_rel(d, k, d.ux, d.uy, d.uz, rn);
+  load_rel(d, k, d.ox, d.oy, d.oz, ro);
+#pragma unroll
+  for (int m = 0; m < 4; m++) mi[m] = k.at[m] >= 0 ? d.mdyn[k.at[m]] : 0.0;
+  if (!shake_core(d, k, rn, ro, mi, dr)) d.flags[F_SHAKE_FAIL] = 1;
   const double idt = 1.0 / d.dt;
 #pragma unroll
   for (int m = 0; m < 4; m++) {
@@ -215,19 +239,17 @@ __device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k) {
     mi[m] = k.at[m] >= 0 ? d.mdyn[a] : 0.0;
   }
   V3 rc[3];
-  double A[3][3], b[3];
+  double A[3][3], b[3], B[3][4], w[3][3];
+  incidence(k, B);
+  couplings(B, mi, w);
 #pragma unroll
-  for (int j = 0; j < 3; j++) rc[j] = v3sub(pick4(r, k.c1[j]), pick4(r, k.c2[j]));
+  for (int j = 0; j < 3; j++) rc[j] = bsum(B[j], r);
 #pragma unroll
   for (int j = 0; j < 3; j++) {
-    const int p = k.c1[j], q = k.c2[j];
-    b[j] = j < k.nc ? -v3dot(v3sub(pick4(v, p), pick4(v, q)), rc[j]) : 0.0;
+    b[j] = j < k.nc ? -v3dot(bsum(B[j], v), rc[j]) : 0.0;
 #pragma unroll
-    for (int e = 0; e < 3; e++) {
-      const int ep = k.c1[e], eq = k.c2[e];
-      const double w = ((p == ep) - (p == eq)) * pick4(mi, p) - ((q == ep) - (q == eq)) * pick4(mi, q);
-      A[j][e] = (j < k.nc && e < k.nc) ? v3dot(rc[j], rc[e]) * w : (j == e ? 1.0 : 0.0);
-    }
+    for (int e = 0; e < 3; e++)
+      A[j][e] = (j < k.nc && e < k.nc) ? v3dot(rc[j], rc[e]) * w[j][e] : (j == e ? 1.0 : 0.0);
   }
   if (!solve3(A, b, 3)) return;
 #pragma unroll
@@ -237,8 +259,7 @@ __device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k) {
     V3 s = v[m];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      const double f = (k.c1[j] == m ? 1.0 : 0.0) - (k.c2[j] == m ? 1.0 : 0.0);
-      const double l = j < k.nc ? b[j] * f * mi[m] : 0.0;
+      const double l = j < k.nc ? b[j] * B[j][m] * mi[m] : 0.0;
       s.x += l * rc[j].x; s.y += l * rc[j].y; s.z += l * rc[j].z;
     }
     d.vx[a] = s.x; d.vy[a] = s.y; d.vz[a] = s.z;
@@ -265,6 +286,77 @@ __device__ __forceinline__ bool moved_atom(const SysDev& d, int i) {
   return dx * dx + dy * dy + dz * dz > d.skin2;
 }
 
+// Kick, drift and SHAKE of one constraint cluster with its state in registers:
+// every load is issued before the first store, so a thread waits on memory
+// about twice instead of once per atom and pass.  The arithmetic is that of
+// kick_drift_atom followed by shake_ccl.  Returns the list-rebuild test of the
+// cluster's atoms.
+template <bool MIXED>
+__device__ __forceinline__ bool step_first_ccl(const SysDev& d, const CCl& k) {
+  unsigned long long* U[3] = {d.ux, d.uy, d.uz};
+  const unsigned long long* R[3] = {d.rx, d.ry, d.rz};
+  double* Vp[3] = {d.vx, d.vy, d.vz};
+  int sl[4];
+  double mi[4], v[4][3], f[4][3];
+  unsigned long long u[4][3], o[4][3];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int a = k.at[m] >= 0 ? k.at[m] : k.at[0];  // padding repeats the first atom and is not stored
+    sl[m] = d.atom_slot[a];
+    mi[m] = k.at[m] >= 0 ? d.mdyn[a] : 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) { v[m][c] = Vp[c][a]; u[m][c] = U[c][a]; }
+  }
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) f[m][c] = take_force<MIXED>(d, sl[m], c, false);
+  const double h0 = 0.5 * d.dt;
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const double h = h0 * mi[m];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      v[m][c] = v[m][c] + h * f[m][c];
+      o[m][c] = u[m][c];
+      u[m][c] = u[m][c] + (unsigned long long)__double2ll_rn(d.dt * v[m][c] * d.isc[c]);
+    }
+  }
+  V3 rn[4], ro[4], dr[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    rn[m] = V3{fx_d(u[0][0], u[m][0], d.sc[0]), fx_d(u[0][1], u[m][1], d.sc[1]), fx_d(u[0][2], u[m][2], d.sc[2])};
+    ro[m] = V3{fx_d(o[0][0], o[m][0], d.sc[0]), fx_d(o[0][1], o[m][1], d.sc[1]), fx_d(o[0][2], o[m][2], d.sc[2])};
+  }
+  if (!shake_core(d, k, rn, ro, mi, dr)) d.flags[F_SHAKE_FAIL] = 1;
+  const double idt = 1.0 / d.dt;
+  bool move = false;
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int a = k.at[m];
+    if (a < 0) continue;
+    const double e[3] = {dr[m].x, dr[m].y, dr[m].z};
+    double r2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const unsigned long long un = u[m][c] + (unsigned long long)__double2ll_rn(e[c] * d.isc[c]);
+      U[c][a] = un;
+      Vp[c][a] = v[m][c] + e[c] * idt;
+      const double x = fx_d(R[c][a], un, d.sc[c]);
+      r2 = c == 0 ? x * x : r2 + x * x;
+    }
+    move = move || r2 > d.skin2;
+    if (MIXED) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) ((unsigned long long*)d.Fs)[(size_t)c * d.n_slots_cap + sl[m]] = 0ull;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) ((double*)d.Fs)[(size_t)c * d.n_slots_cap + sl[m]] = 0.0;
+    }
+  }
+  return move;
+}
+
 // first half kick, drift, SHAKE, and the list-rebuild test; clears the force
 // and energy accumulators
 template <bool MIXED>
@@ -276,14 +368,7 @@ __global__ __launch_bounds__(128) void k_step_first(const SysDev* __restrict__ S
   const int t = lb * 128 + threadIdx.x;
   bool move = false;
   if (t < d.n_ccl) {
-    const CCl k = load_ccl(d, t);
-#pragma unroll
-    for (int m = 0; m < 4; m++)
-      if (k.at[m] >= 0) kick_drift_atom<MIXED>(d, k.at[m]);
-    shake_ccl(d, k);
-#pragma unroll
-    for (int m = 0; m < 4; m++)
-      if (k.at[m] >= 0) move = move || moved_atom(d, k.at[m]);
+    move = step_first_ccl<MIXED>(d, load_ccl(d, t));
   } else if (t < d.n_ccl + d.n_free) {
     const int i = d.free_atoms[t - d.n_ccl];
     kick_drift_atom<MIXED>(d, i);

@@ -128,7 +128,11 @@ __device__ void col_scan_block(const SysDev& d) {
   }
 }
 
-
+// Binning count.  FUSED: the block that finishes last (ticket) also runs the
+// column scan.  Every block then needs a device-scope release fence, which
+// writes back its L2 and costs of the order of 0.5 us per block and XCD, so
+// launches of many blocks use the separate k_col_scan instead.
+template <bool FUSED>
 __global__ __launch_bounds__(256) void k_bin_count(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
@@ -155,7 +159,14 @@ __global__ __launch_bounds__(256) void k_bin_count(const SysDev* __restrict__ S,
       if (v) atomicAdd(d.col_count + k, v);
     }
   }
-  if (last_block(d.tick + 0, pre[s_ + 1] - pre[s_])) col_scan_block(d);
+  if (FUSED && last_block(d.tick + 0, pre[s_ + 1] - pre[s_])) col_scan_block(d);
+}
+
+// one block per system
+__global__ __launch_bounds__(256) void k_col_scan(const SysDev* __restrict__ S, int nsys) {
+  const SysDev& d = S[blockIdx.x];
+  if (skip_build(d)) return;
+  col_scan_block(d);
 }
 
 // places each atom in its column's range of bin_tmp (order inside a column is
@@ -569,7 +580,12 @@ __global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S,
 // ---------------------------------------------------------------------------
 void launch_nblist(const Launch& L, bool check) {
   if (check) hipLaunchKernelGGL(k_check, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
-  hipLaunchKernelGGL(k_bin_count, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  if (L.w_atoms.total <= 512) {
+    hipLaunchKernelGGL(k_bin_count<true>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+  } else {
+    hipLaunchKernelGGL(k_bin_count<false>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+    hipLaunchKernelGGL(k_col_scan, dim3(L.nsys), dim3(256), 0, L.stream, L.d_sys, L.nsys);
+  }
   hipLaunchKernelGGL(k_bin_scatter, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
   hipLaunchKernelGGL(k_col_sort, dim3(L.w_cols.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_cols.pre);
   hipLaunchKernelGGL(k_pair_search, dim3(L.w_clus.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_clus.pre);
