@@ -292,6 +292,11 @@ int System::alloc_device() {
   const int n = h.n;
   d.n_atoms = n;
   d.n_types = h.ntypes;
+  {  // odd multiplier near 0.618 n, coprime to n
+    unsigned long long m = (unsigned long long)(0.6180339887 * n) | 1ull;
+    while (std::gcd(m, (unsigned long long)n) != 1ull) m += 2ull;
+    d.spread_mult = m;
+  }
   const double V = par.box[0] * par.box[1] * par.box[2];
   for (int k = 0; k < 3; k++) {
     d.L[k] = par.box[k];
@@ -678,6 +683,7 @@ Engine::~Engine() {
     if (side_[k]) (void)hipStreamDestroy(side_[k]);
   }
   if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+  if (ev_early_) (void)hipEventDestroy(ev_early_);
   if (L_.stream) (void)hipStreamDestroy(L_.stream);
 }
 
@@ -719,6 +725,7 @@ int Engine::prepare() {
     if (!ev_join_[k]) FZ_HIP(hipEventCreateWithFlags(&ev_join_[k], hipEventDisableTiming));
   }
   if (!ev_fork_) FZ_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+  if (!ev_early_) FZ_HIP(hipEventCreateWithFlags(&ev_early_, hipEventDisableTiming));
   if (!d_sys_) FZ_HIP(hipMalloc(&d_sys_, ns * sizeof(SysDev)));
   if (!L_.chunk_pre) FZ_HIP(hipMalloc(&L_.chunk_pre, (ns + 1) * sizeof(int)));
   FZ_TRY(upload_desc());
@@ -792,18 +799,26 @@ int Engine::ensure() {
 }
 
 void Engine::record_forces(bool ev, bool lite, bool check) {
+  // Fork: bonded and PME beside the pair kernel (errors of these calls surface
+  // at the stream's next synchronisation or at capture end).  The atomic
+  // spread visits atoms, not cluster slots, so with it the PME branch starts
+  // before the list build and only its force gather waits for the new slots.
+  Launch side[2] = {L_, L_};
+  for (int k = 0; k < 2; k++) side[k].stream = side_[k];
+  const bool early = !L_.spread_bricks;
+  if (early) {
+    (void)hipEventRecord(ev_early_, L_.stream);
+    (void)hipStreamWaitEvent(side_[1], ev_early_, 0);
+    launch_pme_part(side[1], 0, ev);
+    launch_pme_part(side[1], 1, ev);
+  }
   launch_nblist(L_, check);
   launch_gather_sorted(L_);
-  // fork: bonded and PME beside the pair kernel (errors of these calls surface
-  // at the stream's next synchronisation or at capture end)
   (void)hipEventRecord(ev_fork_, L_.stream);
-  Launch side[2] = {L_, L_};
-  for (int k = 0; k < 2; k++) {
-    side[k].stream = side_[k];
-    (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
-  }
+  for (int k = 0; k < 2; k++) (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
   launch_bonded(side[0], ev);
-  launch_pme(side[1], ev, 0);
+  if (early) launch_pme_part(side[1], 2, ev);
+  else launch_pme(side[1], ev, 0);
   launch_pair(L_, ev ? (lite ? 1 : 2) : 0);
   for (int k = 0; k < 2; k++) {
     (void)hipEventRecord(ev_join_[k], side_[k]);

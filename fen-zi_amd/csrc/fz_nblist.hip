@@ -111,16 +111,6 @@ __device__ void col_scan_block(const SysDev& d) {
     int total = part[NT - 1];
     d.col_clu0[n] = total;
     *d.n_clusters = total;
-    // slot-order multiplier for the PME spread: odd, near 0.618 ns, coprime to ns
-    const unsigned ns = (unsigned)total * CLUSTER;
-    unsigned m = ((unsigned)(0.6180339887 * ns)) | 1u;
-    for (;;) {
-      unsigned a = m, b = ns > 0 ? ns : 1u;
-      while (b) { unsigned r = a % b; a = b; b = r; }
-      if (a == 1u || ns == 0u) break;
-      m += 2u;
-    }
-    d.n_clusters[1] = (int)m;
     int ov = 0;
     if (bad) ov |= OV_COLUMN;
     if (total > d.n_clusters_cap || total * CLUSTER > d.n_slots_cap) ov |= OV_SLOTS;

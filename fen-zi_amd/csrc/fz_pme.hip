@@ -66,10 +66,11 @@ __device__ __forceinline__ R bspline4_k(R f, int k) {
 // Charge spreading with global atomics, for launches over few atoms, where the
 // brick pipeline's five launches cost more than the contention they remove.
 // One atom per wave iteration, one of its 4x4x4 grid points per lane: the 64
-// atomics of an instruction fall on 16 rows of 4 consecutive x points.  Slots
-// are visited with a multiplier coprime to their count (set at each list
-// build), so that concurrent waves work on distant atoms.  The wave's 64 atoms
-// are read and placed on the grid in parallel first (lane k: the k-th atom).
+// atomics of an instruction fall on 16 rows of 4 consecutive x points.  Atoms
+// are visited with a multiplier coprime to their count, so that concurrent
+// waves work on distant atoms; the kernel reads no cluster slots and can run
+// before the list build.  The wave's 64 atoms are read and placed on the grid
+// in parallel first (lane k: the k-th atom).
 // Qacc must be zero before; the x pass of the FFT clears it after reading.
 // The mixed-form grid is the same integer sum as the brick path's.
 __device__ __forceinline__ float bcast(float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); }
@@ -84,23 +85,20 @@ __global__ __launch_bounds__(256) void k_spread_atomic(const SysDev* __restrict_
   const SysDev& d = S[s_];
   const int lane = threadIdx.x & 63;
   const int t0 = ((blockIdx.x - pre[s_]) * 4 + (threadIdx.x >> 6)) * 64;
-  const int ns = *d.n_clusters * CLUSTER;
-  const unsigned long long mult = (unsigned long long)d.n_clusters[1];
+  const int na = d.n_atoms;
+  const unsigned long long mult = d.spread_mult;
   const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
   const int ca = lane & 3, cb = (lane >> 2) & 3, cc = lane >> 4;
   R q = R(0), fx = R(0), fy = R(0), fz = R(0);
   int gx0 = 0, gy0 = 0, gz0 = 0;
   bool has = false;
-  if (t0 + lane < ns) {
-    const int sl = (int)(((unsigned long long)(t0 + lane) * mult) % (unsigned long long)ns);
-    const int a = d.slot_atom[sl];
-    if (a >= 0) {
-      q = (R)d.q[a];
-      has = q != R(0);
-      gx0 = spline_pos<R>(d.ux[a], K0, fx);
-      gy0 = spline_pos<R>(d.uy[a], K1, fy);
-      gz0 = spline_pos<R>(d.uz[a], K2, fz);
-    }
+  if (t0 + lane < na) {
+    const int a = (int)(((unsigned long long)(t0 + lane) * mult) % (unsigned long long)na);
+    q = (R)d.q[a];
+    has = q != R(0);
+    gx0 = spline_pos<R>(d.ux[a], K0, fx);
+    gy0 = spline_pos<R>(d.uy[a], K1, fy);
+    gz0 = spline_pos<R>(d.uz[a], K2, fz);
   }
   unsigned long long bal = __ballot(has);
   while (bal) {
@@ -314,8 +312,8 @@ __global__ __launch_bounds__(256) void k_brick_reduce(const SysDev* __restrict__
 void launch_spread(const Launch& L) {
   dim3 b(256);
   if (!L.spread_bricks) {
-    if (L.mixed) hipLaunchKernelGGL(k_spread_atomic<true>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
-    else hipLaunchKernelGGL(k_spread_atomic<false>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    if (L.mixed) hipLaunchKernelGGL(k_spread_atomic<true>, dim3(L.w_atoms.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+    else hipLaunchKernelGGL(k_spread_atomic<false>, dim3(L.w_atoms.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
     return;
   }
   if (L.mixed) {

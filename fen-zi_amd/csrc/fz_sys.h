@@ -48,6 +48,7 @@ struct SysDev {
   int use_excl_list;     // 1: exclusions by sorted list, 0: 64-bit window bit vectors
   int n_ccl;             // constraint clusters
   int n_free;            // atoms in no constraint cluster
+  unsigned long long spread_mult;  // atom visiting order of the atomic PME spread: odd, coprime to n_atoms
   // bonded term counts (flat index order: bonds, angles, dihedrals, impropers, ew pairs, 1-4 pairs)
   int n_bonds, n_angles, n_diheds, n_imprs, n_ew, n_p14;
   // ---- geometry / method parameters ----------------------------------
@@ -95,7 +96,7 @@ struct SysDev {
   int *clu_col;          // [n_clusters_cap]
   unsigned long long *bb; // [6 * n_clusters_cap] lo/hi per axis
   unsigned long long *su; // [3 * n_slots_cap] slot-ordered positions at the last build (x, y, z blocks)
-  int *n_clusters;       // [0] clusters of the last build, [1] spread slot multiplier
+  int *n_clusters;       // [0] clusters of the last build
   unsigned *tick;        // [1] last-block ticket of the binning count
   int *search_cnt;       // [n_clusters_cap + 1]
   int *search_tmp;       // [n_clusters_cap * SEARCH_TMP_CAP] J of each kept entry
