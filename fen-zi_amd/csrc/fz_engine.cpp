@@ -495,9 +495,11 @@ int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   FZ_TRY(dalloc(&d.bb, (size_t)6 * clusters_cap));
   FZ_TRY(dalloc(&d.su, (size_t)3 * ns));
   FZ_TRY(dalloc(&d.br_slot, ns, false));
-  FZ_TRY(dalloc(&d.search_cnt, clusters_cap + 1));
-  FZ_TRY(dalloc(&d.search_tmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));
-  FZ_TRY(dalloc(&d.search_mtmp, (size_t)clusters_cap * SEARCH_TMP_CAP, false));
+  // small systems search each I cluster with several waves (more waves in flight)
+  d.search_parts = clusters_cap < 8192 ? 4 : clusters_cap < 16384 ? 2 : 1;
+  FZ_TRY(dalloc(&d.search_cnt, (size_t)clusters_cap * d.search_parts + 1));
+  FZ_TRY(dalloc(&d.search_tmp, (size_t)clusters_cap * d.search_parts * SEARCH_TMP_CAP, false));
+  FZ_TRY(dalloc(&d.search_mtmp, (size_t)clusters_cap * d.search_parts * SEARCH_TMP_CAP, false));
   FZ_TRY(dalloc(&d.tile_cnt, (clusters_cap + 63) / 64));
   FZ_TRY(dalloc(&d.pl_i, pairs_cap, false));
   FZ_TRY(dalloc(&d.pl_j, pairs_cap, false));
@@ -740,7 +742,13 @@ int Engine::prepare() {
     L_.lj_sep &= s->dv.lj_sep;
     L_.com_any |= s->dv.com_on;
   }
-  std::vector<int> atoms, slots, cols, clus, terms, ccl, stp, fx_, fy_, fz_, bricks;
+  std::vector<int> atoms, slots, cols, clus, srch, terms, ccl, stp, fx_, fy_, fz_, bricks;
+  // Waves per I cluster in the pair search: up to each system's search_parts
+  // when the launch holds few clusters (latency bound: too few waves in
+  // flight), one when it holds many.
+  long long clusters_total = 0;
+  for (auto* s : sys_) clusters_total += s->dv.n_clusters_cap;
+  L_.search_cap = clusters_total <= 16384 ? 4 : 1;
   int kmax = 4;
   for (auto* s : sys_) kmax = std::max({kmax, s->dv.K[0], s->dv.K[1], s->dv.K[2]});
   const int csz = L_.mixed ? 8 : 16;
@@ -754,6 +762,7 @@ int Engine::prepare() {
     slots.push_back((d.n_slots_cap + 255) / 256);
     cols.push_back(d.ncol);
     clus.push_back((d.n_clusters_cap + 3) / 4);
+    srch.push_back((d.n_clusters_cap * search_split(d.search_parts, L_.search_cap) + 3) / 4);
     const int nt = d.n_bonds + d.n_angles + d.n_diheds + d.n_imprs + d.n_ew + d.n_p14;
     terms.push_back((nt + 255) / 256);
     ccl.push_back((d.n_ccl + 127) / 128);
@@ -768,6 +777,7 @@ int Engine::prepare() {
   FZ_TRY(make_work(L_.w_slots, slots));
   FZ_TRY(make_work(L_.w_cols, cols));
   FZ_TRY(make_work(L_.w_clus, clus));
+  FZ_TRY(make_work(L_.w_search, srch));
   FZ_TRY(make_work(L_.w_terms, terms));
   FZ_TRY(make_work(L_.w_ccl, ccl));
   FZ_TRY(make_work(L_.w_step, stp));
@@ -867,8 +877,8 @@ int Engine::grow_and_reset(int si, int bits) {
   if (bits & OV_PAIRS) pairs = (long long)(pairs * 1.6) + 4096;
   if (bits & OV_SLOTS) clus = (int)(clus * 1.3) + 64;
   if (bits & (OV_COLUMN | OV_SEARCH))
-    return set_error(FZ_ERR_CAPACITY, "neighbour search capacity exceeded (column occupancy or J list > %d); "
-                     "use a finer column grid", SEARCH_TMP_CAP);
+    return set_error(FZ_ERR_CAPACITY, "neighbour search capacity exceeded (column occupancy or more than %d J clusters "
+                     "in a search part); use a finer column grid", SEARCH_TMP_CAP);
   // the neighbour arrays are the last allocations of the system: free them
   const size_t keep = s->nb_alloc0;
   for (size_t k = keep; k < s->allocs.size(); k++) (void)hipFree(s->allocs[k]);

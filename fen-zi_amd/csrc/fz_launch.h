@@ -12,6 +12,9 @@ struct Work {
   int total = 0;
 };
 
+// pair-search waves per I cluster of a system: its buffer parts, at most the launch's cap
+__host__ __device__ inline int search_split(int parts, int cap) { return parts < cap ? parts : cap; }
+
 struct Launch {
   hipStream_t stream = nullptr;
   const SysDev* d_sys = nullptr;  // device array of nsys descriptors
@@ -21,6 +24,7 @@ struct Launch {
   Work w_slots;   // 256 slots per block
   Work w_cols;    // one block per column
   Work w_clus;    // 4 clusters (waves) per block
+  Work w_search;  // 4 search waves per block, search_split(parts, search_cap) waves per cluster
   Work w_terms;   // 256 bonded terms per block
   Work w_ccl;     // 128 constraint clusters per block
   Work w_step;    // 128 per block: constraint clusters, then free atoms
@@ -35,6 +39,7 @@ struct Launch {
   int lj_sep = 0;            // every system follows the LJ combination rule
   int com_any = 0;           // some system has a COM restraint
   int spread_bricks = 1;     // PME spread by bricks (1) or by global atomics (0)
+  int search_cap = 1;  // cap on the pair-search waves per I cluster (1, 2 or 4)
 };
 
 // check: run the displacement test first (the fused step kernel does it otherwise)

@@ -386,22 +386,31 @@ __device__ __forceinline__ float r2_list32(const SysDev& d, unsigned ax, unsigne
 //     self pair, and the FP32 list distance is below the threshold.  An entry
 //     whose 64-bit lane mask is empty is dropped.  The next hit's atoms are
 //     loaded while the current one is tested.
-__global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+//  With split > 1, split consecutive waves share one I cluster, each taking a
+//  contiguous share of its candidate columns and writing its own buffer part;
+//  the fill concatenates the parts in order, which is the order one wave
+//  would have produced.
+// (six waves per SIMD: the search waits on memory for every hit)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_pair_search(
+    const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre, int split_cap) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
   const int lane = threadIdx.x & 63;
   const int il = lane & 7, jl = lane >> 3;
-  const int I0 = lb * 4 + (threadIdx.x >> 6);
-  if (I0 >= *d.n_clusters) return;
-  const int I = I0;
+  const int split = search_split(d.search_parts, split_cap);
+  const int gw = lb * 4 + (threadIdx.x >> 6);
+  const int I = gw / split, part = gw - I * split;
+  if (I >= *d.n_clusters) return;
+  const int pcap = d.search_parts * SEARCH_TMP_CAP / split;  // entries per part
   const int nsc = d.n_slots_cap;
   const unsigned* su32 = (const unsigned*)d.su;  // high words at odd indices
   const int ai = d.slot_atom[I * CLUSTER + il];
   const unsigned aix = su32[2 * (I * CLUSTER + il) + 1], aiy = su32[2 * (nsc + I * CLUSTER + il) + 1],
                  aiz = su32[2 * (2 * nsc + I * CLUSTER + il) + 1];
   const unsigned long long wa = (ai >= 0 && !d.use_excl_list) ? d.excl_bits[ai] : 0ull;
-  unsigned long long* mout = d.search_mtmp + (size_t)I * SEARCH_TMP_CAP;
-  int* out = d.search_tmp + (size_t)I * SEARCH_TMP_CAP;
+  const size_t obase = (size_t)I * d.search_parts * SEARCH_TMP_CAP + (size_t)part * pcap;
+  unsigned long long* mout = d.search_mtmp + obase;
+  int* out = d.search_tmp + obase;
   unsigned long long bi[6];
 #pragma unroll
   for (int e = 0; e < 6; e++) bi[e] = d.bb[6 * (size_t)I + e];
@@ -458,7 +467,7 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
       if (on) on = r2_list32(d, aix, aiy, aiz, bx, by, bz) < d.rl2f;
       const unsigned long long m = __ballot(on);
       if (m) {
-        if (count < SEARCH_TMP_CAP) {
+        if (count < pcap) {
           if (lane == 0) { out[count] = Jk; mout[count] = m; }
         } else {
           over = true;
@@ -479,11 +488,12 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
   };
 
   const int nq = nyv * nxv;
-  for (int g = 0; g < nq; g += 64) {
+  const int q0 = part * nq / split, q1 = (part + 1) * nq / split;
+  for (int g = q0; g < q1; g += 64) {
     // per lane: one candidate column and its (up to two) cluster ranges
     int r0a = 0, r0e = 0, r1a = 0, r1e = 0;
     const int q = g + lane;
-    if (q < nq) {
+    if (q < q1) {
       const int iy = q / nxv, ix = q - iy * nxv;
       const int yy = ally ? iy : (cy + iy - d.ky_range + 2 * d.ncy) % d.ncy;
       const int xx = allx ? ix : (cx + ix - d.kx_range + 2 * d.ncx) % d.ncx;
@@ -507,7 +517,7 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
         r0a = max(r0a, I); r1a = max(r1a, I);
       }
     }
-    const int nl = min(64, nq - g);
+    const int nl = min(64, q1 - g);
     for (int k = 0; k < nl; k++) {
       append(__builtin_amdgcn_readlane(r0a, k), __builtin_amdgcn_readlane(r0e, k));
       append(__builtin_amdgcn_readlane(r1a, k), __builtin_amdgcn_readlane(r1e, k));
@@ -515,8 +525,8 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
   }
   if (fill) test_hits();
   if (lane == 0) {
-    const int kept = count < SEARCH_TMP_CAP ? count : SEARCH_TMP_CAP;
-    d.search_cnt[I] = kept;
+    const int kept = count < pcap ? count : pcap;
+    d.search_cnt[I * split + part] = kept;
     if (kept) atomicAdd(d.tile_cnt + (I >> 6), kept);
     if (over) atomicOr(d.flags + F_OVERFLOW, OV_SEARCH);
   }
@@ -527,13 +537,16 @@ __global__ __launch_bounds__(256) void k_pair_search(const SysDev* __restrict__ 
 // the counts of the clusters before I inside it; the list total is the sum of
 // every tile.  Each wave forms both from the tile counts, so no separate scan
 // pass is needed.  A list over capacity is flagged and left unwritten.
-__global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+__global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
+                                                   int split_cap) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
   const int lane = threadIdx.x & 63;
   const int I = lb * 4 + (threadIdx.x >> 6);
   const int n = *d.n_clusters;
   if (I >= n) return;
+  const int split = search_split(d.search_parts, split_cap);
+  const int pcap = d.search_parts * SEARCH_TMP_CAP / split;
   const int t = I >> 6, nt = (n + 63) >> 6;
   long long before = 0, total = 0;
   for (int k = lane; k < nt; k += 64) {
@@ -541,8 +554,7 @@ __global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S,
     total += v;
     if (k < t) before += v;
   }
-  const int k0 = (t << 6) + lane;
-  if (k0 < I) before += d.search_cnt[k0];
+  for (int k = (t << 6) * split + lane; k < I * split; k += 64) before += d.search_cnt[k];
   before = wave_sum(before);
   total = wave_sum(total);
   if (total > d.n_pairs_cap) {
@@ -556,14 +568,18 @@ __global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S,
     *d.n_pairs = total;
     d.flags[F_NBUILD] += 1;
   }
-  const long long off = before;
-  const int cnt = d.search_cnt[I];
-  const int* src = d.search_tmp + (size_t)I * SEARCH_TMP_CAP;
-  const unsigned long long* msrc = d.search_mtmp + (size_t)I * SEARCH_TMP_CAP;
-  for (int k = lane; k < cnt; k += 64) {
-    d.pl_i[off + k] = I;
-    d.pl_j[off + k] = src[k];
-    d.pl_mask[off + k] = msrc[k];
+  long long off = before;
+  for (int p = 0; p < split; p++) {
+    const int cnt = d.search_cnt[I * split + p];
+    const size_t base = (size_t)I * d.search_parts * SEARCH_TMP_CAP + (size_t)p * pcap;
+    const int* src = d.search_tmp + base;
+    const unsigned long long* msrc = d.search_mtmp + base;
+    for (int k = lane; k < cnt; k += 64) {
+      d.pl_i[off + k] = I;
+      d.pl_j[off + k] = src[k];
+      d.pl_mask[off + k] = msrc[k];
+    }
+    off += cnt;
   }
 }
 
@@ -578,8 +594,10 @@ void launch_nblist(const Launch& L, bool check) {
   }
   hipLaunchKernelGGL(k_bin_scatter, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
   hipLaunchKernelGGL(k_col_sort, dim3(L.w_cols.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_cols.pre);
-  hipLaunchKernelGGL(k_pair_search, dim3(L.w_clus.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_clus.pre);
-  hipLaunchKernelGGL(k_pair_fill, dim3(L.w_clus.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_clus.pre);
+  hipLaunchKernelGGL(k_pair_search, dim3(L.w_search.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_search.pre,
+                     L.search_cap);
+  hipLaunchKernelGGL(k_pair_fill, dim3(L.w_clus.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_clus.pre,
+                     L.search_cap);
 }
 
 }  // namespace fz

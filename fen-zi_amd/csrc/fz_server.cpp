@@ -91,7 +91,8 @@ int64_t estimate_bytes(const fz_topology* t, const fz_params* p, int precision) 
   const double terms = (double)t->n_bonds + t->n_angles + t->n_dihedrals + t->n_impropers;
   double b = n * 200.0;                                       // per-atom state, exclusions, snapshot
   b += clusters * CLUSTER * (precision == FZ_MIXED ? 60.0 : 76.0);  // slots, forces, binning
-  b += clusters * (SEARCH_TMP_CAP * 12.0 + 80.0);             // search buffers, boxes, offsets
+  const double parts = clusters < 8192 ? 4.0 : clusters < 16384 ? 2.0 : 1.0;  // search parts per I cluster
+  b += clusters * (parts * SEARCH_TMP_CAP * 12.0 + 80.0);     // search buffers, boxes, offsets
   b += pairs * 16.0;                                          // pair list
   b += K3 * (precision == FZ_MIXED ? 16.0 : 32.0);             // PME grids
   b += terms * 40.0 + 3 * n * 8.0;                            // bonded terms, exclusion pairs

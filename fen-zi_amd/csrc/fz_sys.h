@@ -98,9 +98,10 @@ struct SysDev {
   unsigned long long *su; // [3 * n_slots_cap] slot-ordered positions at the last build (x, y, z blocks)
   int *n_clusters;       // [0] clusters of the last build
   unsigned *tick;        // [1] last-block ticket of the binning count
-  int *search_cnt;       // [n_clusters_cap + 1]
-  int *search_tmp;       // [n_clusters_cap * SEARCH_TMP_CAP] J of each kept entry
-  unsigned long long *search_mtmp; // [n_clusters_cap * SEARCH_TMP_CAP] lane mask of each kept entry
+  int *search_cnt;       // [n_clusters_cap * search_parts + 1] kept entries per (I, part)
+  int search_parts;      // 1, 2 or 4: search waves per I cluster the buffers are sized for
+  int *search_tmp;       // [n_clusters_cap * search_parts * SEARCH_TMP_CAP] J of each kept entry
+  unsigned long long *search_mtmp; // [same] lane mask of each kept entry
   int *tile_cnt;         // [ceil(n_clusters_cap / 64)] kept entries per 64 consecutive I clusters
   int *pl_i, *pl_j;      // [n_pairs_cap]
   unsigned long long *pl_mask; // [n_pairs_cap]
