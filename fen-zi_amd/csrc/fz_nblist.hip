@@ -445,36 +445,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       hit = bbox_within(bi, bj, d);
     }
     unsigned long long bal = __ballot(hit);
-    if (!bal) return;
-    int Jn = __builtin_amdgcn_readlane(cand, __ffsll((long long)bal) - 1);
-    bal &= bal - 1ull;
-    int sl = Jn * CLUSTER + jl;
-    int bn = d.slot_atom[sl];
-    unsigned bxn = su32[2 * sl + 1], byn = su32[2 * (nsc + sl) + 1], bzn = su32[2 * (2 * nsc + sl) + 1];
-    for (;;) {
-      const int Jk = Jn, b = bn;
-      const unsigned bx = bxn, by = byn, bz = bzn;
-      const bool more = bal != 0ull;
-      if (more) {
-        Jn = __builtin_amdgcn_readlane(cand, __ffsll((long long)bal) - 1);
-        bal &= bal - 1ull;
-        sl = Jn * CLUSTER + jl;
-        bn = d.slot_atom[sl];
-        bxn = su32[2 * sl + 1]; byn = su32[2 * (nsc + sl) + 1]; bzn = su32[2 * (2 * nsc + sl) + 1];
-      }
-      bool on = ai >= 0 && b >= 0 && (I != Jk || il < jl);
-      if (on) on = !excluded_w(d, wa, ai, b);
-      if (on) on = r2_list32(d, aix, aiy, aiz, bx, by, bz) < d.rl2f;
-      const unsigned long long m = __ballot(on);
-      if (m) {
-        if (count < pcap) {
-          if (lane == 0) { out[count] = Jk; mout[count] = m; }
-        } else {
-          over = true;
+    // hits in groups of four: the atoms of all four are loaded before the
+    // first is tested, so a wave waits on memory once per group
+    while (bal) {
+      int Jg[4];
+#pragma unroll
+      for (int h = 0; h < 4; h++) {
+        Jg[h] = -1;
+        if (bal) {
+          Jg[h] = __builtin_amdgcn_readlane(cand, __ffsll((long long)bal) - 1);
+          bal &= bal - 1ull;
         }
-        count++;
       }
-      if (!more) break;
+      int bg[4];
+      unsigned xg[4], yg[4], zg[4];
+#pragma unroll
+      for (int h = 0; h < 4; h++) {
+        bg[h] = -1; xg[h] = yg[h] = zg[h] = 0u;
+        if (Jg[h] >= 0) {
+          const int sl = Jg[h] * CLUSTER + jl;
+          bg[h] = d.slot_atom[sl];
+          xg[h] = su32[2 * sl + 1]; yg[h] = su32[2 * (nsc + sl) + 1]; zg[h] = su32[2 * (2 * nsc + sl) + 1];
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 4; h++) {
+        if (Jg[h] < 0) break;
+        const int Jk = Jg[h], b = bg[h];
+        bool on = ai >= 0 && b >= 0 && (I != Jk || il < jl);
+        if (on) on = !excluded_w(d, wa, ai, b);
+        if (on) on = r2_list32(d, aix, aiy, aiz, xg[h], yg[h], zg[h]) < d.rl2f;
+        const unsigned long long m = __ballot(on);
+        if (m) {
+          if (count < pcap) {
+            if (lane == 0) { out[count] = Jk; mout[count] = m; }
+          } else {
+            over = true;
+          }
+          count++;
+        }
+      }
     }
   };
   auto append = [&](int a, int e) {  // J in [a, e), uniform
