@@ -487,16 +487,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       }
     }
   };
-  auto append = [&](int a, int e) {  // J in [a, e), uniform
-    while (a < e) {
-      const int m = min(e - a, 64 - fill);
-      if (lane >= fill && lane < fill + m) cand = a + (lane - fill);
-      fill += m;
-      a += m;
-      if (fill == 64) { test_hits(); fill = 0; }
-    }
-  };
-
   const int nq = nyv * nxv;
   const int q0 = part * nq / split, q1 = (part + 1) * nq / split;
   for (int g = q0; g < q1; g += 64) {
@@ -527,10 +517,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         r0a = max(r0a, I); r1a = max(r1a, I);
       }
     }
-    const int nl = min(64, q1 - g);
-    for (int k = 0; k < nl; k++) {
-      append(__builtin_amdgcn_readlane(r0a, k), __builtin_amdgcn_readlane(r0e, k));
-      append(__builtin_amdgcn_readlane(r1a, k), __builtin_amdgcn_readlane(r1e, k));
+    // The group's candidates in order (column by column, each column's first
+    // range then its second) are numbered by an inclusive scan of the per-lane
+    // counts; each free slot of the 64-candidate block finds its column by a
+    // binary search over the scan (lane permutes), so the block fills without
+    // a scalar loop over columns.
+    const int l0 = max(r0e - r0a, 0), l1 = max(r1e - r1a, 0);
+    int incl = l0 + l1;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    for (int base = 0; base < total;) {
+      const int m = min(total - base, 64 - fill);
+      // every lane takes part in the permutes; lanes outside [fill, fill + m) discard
+      const int p = base + lane - fill;
+      int lo = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {  // first k with incl[k] > p
+        const int v = __shfl(incl, lo + step - 1, 64);
+        if (v <= p) lo += step;
+      }
+      const int k = min(lo, 63);
+      const int off = p - (__shfl(incl, k, 64) - __shfl(l0 + l1, k, 64));
+      const int c0 = __shfl(r0a, k, 64), n0 = __shfl(l0, k, 64), c1 = __shfl(r1a, k, 64);
+      if (lane >= fill && lane < fill + m) cand = off < n0 ? c0 + off : c1 + (off - n0);
+      fill += m;
+      base += m;
+      if (fill == 64) { test_hits(); fill = 0; }
     }
   }
   if (fill) test_hits();
