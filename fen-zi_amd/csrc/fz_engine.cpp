@@ -392,6 +392,7 @@ int System::alloc_device() {
   FZ_TRY(dalloc(&d.ccl_cons, h.ccl_cons.size())); FZ_TRY(up(d.ccl_cons, h.ccl_cons));
   FZ_TRY(dalloc(&d.ccl_d2, h.ccl_d2.size())); FZ_TRY(up(d.ccl_d2, h.ccl_d2));
   FZ_TRY(dalloc(&d.free_atoms, h.free_atoms.size())); FZ_TRY(up(d.free_atoms, h.free_atoms));
+  FZ_TRY(dalloc(&d.br_atom, n, false));
   // PME
   {
     const size_t K3 = (size_t)d.K[0] * d.K[1] * d.K[2];
@@ -494,7 +495,6 @@ int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   FZ_TRY(dalloc(&d.clu_col, clusters_cap));
   FZ_TRY(dalloc(&d.bb, (size_t)6 * clusters_cap));
   FZ_TRY(dalloc(&d.su, (size_t)3 * ns));
-  FZ_TRY(dalloc(&d.br_slot, ns, false));
   // small systems search each I cluster with several waves (more waves in flight)
   d.search_parts = clusters_cap < 8192 ? 4 : clusters_cap < 16384 ? 2 : 1;
   FZ_TRY(dalloc(&d.search_cnt, (size_t)clusters_cap * d.search_parts + 1));
@@ -810,25 +810,22 @@ int Engine::ensure() {
 
 void Engine::record_forces(bool ev, bool lite, bool check) {
   // Fork: bonded and PME beside the pair kernel (errors of these calls surface
-  // at the stream's next synchronisation or at capture end).  The atomic
-  // spread visits atoms, not cluster slots, so with it the PME branch starts
-  // before the list build and only its force gather waits for the new slots.
+  // at the stream's next synchronisation or at capture end).  The charge
+  // spread (either path) visits atoms, not cluster slots, so the PME branch
+  // starts before the list build and only its force gather waits for the new
+  // slots.
   Launch side[2] = {L_, L_};
   for (int k = 0; k < 2; k++) side[k].stream = side_[k];
-  const bool early = !L_.spread_bricks;
-  if (early) {
-    (void)hipEventRecord(ev_early_, L_.stream);
-    (void)hipStreamWaitEvent(side_[1], ev_early_, 0);
-    launch_pme_part(side[1], 0, ev);
-    launch_pme_part(side[1], 1, ev);
-  }
+  (void)hipEventRecord(ev_early_, L_.stream);
+  (void)hipStreamWaitEvent(side_[1], ev_early_, 0);
+  launch_pme_part(side[1], 0, ev);
+  launch_pme_part(side[1], 1, ev);
   launch_nblist(L_, check);
   launch_gather_sorted(L_);
   (void)hipEventRecord(ev_fork_, L_.stream);
   for (int k = 0; k < 2; k++) (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
   launch_bonded(side[0], ev);
-  if (early) launch_pme_part(side[1], 2, ev);
-  else launch_pme(side[1], ev, 0);
+  launch_pme_part(side[1], 2, ev);
   launch_pair(L_, ev ? (lite ? 1 : 2) : 0);
   for (int k = 0; k < 2; k++) {
     (void)hipEventRecord(ev_join_[k], side_[k]);

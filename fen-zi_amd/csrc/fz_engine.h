@@ -124,9 +124,9 @@ class Engine {
   hipGraphExec_t g_plain_ = nullptr, g_ev_ = nullptr;
   // The force evaluation forks after the list build: the pair kernel on the
   // main stream, bonded terms and the PME chain on two side streams (default
-  // priority), joined before the second half kick.  With the atomic spread the
-  // PME branch forks before the build (ev_early_) and only its force gather
-  // waits for the build's slots (ev_fork_).  Every force and energy
+  // priority), joined before the second half kick.  The PME branch forks
+  // before the build (ev_early_) and only its force gather waits for the
+  // build's slots (ev_fork_).  Every force and energy
   // goes into fixed-point (mixed) or FP64 accumulators by atomic adds, so the
   // branches need no other ordering.
   hipStream_t side_[2] = {nullptr, nullptr};

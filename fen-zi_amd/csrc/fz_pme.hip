@@ -139,21 +139,20 @@ constexpr int BRK = 8;          // brick edge (grid points)
 constexpr int BRH = BRK + 3;    // block edge: brick + the stencil's reach
 constexpr int BRV = BRH * BRH * BRH;
 
-__device__ __forceinline__ int n_slots_used(const SysDev& d) { return *d.n_clusters * CLUSTER; }
-
-// Slots are spatially sorted, so the lanes of a wave fall into a few bricks:
-// one atomic per distinct brick of the wave, ranks in lane order after it.
+// Atoms, not cluster slots, are visited, so the spread needs nothing from the
+// list build.  Consecutive atoms are mostly bonded neighbours, so the lanes of
+// a wave fall into few bricks: one atomic per distinct brick of the wave,
+// ranks in lane order after it.
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_brick_count(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using R = typename std::conditional<MIXED, float, double>::type;
   const int s_ = locate(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
-  const bool inside = sl < n_slots_used(d);
+  const int a = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const bool inside = a < d.n_atoms;
   int h = -1;
   if (inside) {
-    const int a = d.slot_atom[sl];
-    if (a >= 0 && (R)d.q[a] != R(0)) {
+    if ((R)d.q[a] != R(0)) {
       R f;
       const int gx = spline_pos<R>(d.ux[a], d.K[0], f);
       const int gy = spline_pos<R>(d.uy[a], d.K[1], f);
@@ -175,7 +174,7 @@ __global__ __launch_bounds__(256) void k_brick_count(const SysDev* __restrict__ 
       rank = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
     todo &= ~m;
   }
-  if (inside) d.br_slot[sl] = make_int2(h, rank);
+  if (inside) d.br_atom[a] = make_int2(h, rank);
 }
 
 // one block per system
@@ -209,11 +208,10 @@ __global__ __launch_bounds__(256) void k_brick_scatter(const SysDev* __restrict_
   using R = typename std::conditional<MIXED, float, double>::type;
   const int s_ = locate(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
-  if (sl >= n_slots_used(d)) return;
-  const int2 r = d.br_slot[sl];
+  const int a = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  if (a >= d.n_atoms) return;
+  const int2 r = d.br_atom[a];
   if (r.x < 0) return;
-  const int a = d.slot_atom[sl];
   R fx, fy, fz;
   const int gx = spline_pos<R>(d.ux[a], d.K[0], fx);
   const int gy = spline_pos<R>(d.uy[a], d.K[1], fy);
@@ -317,15 +315,15 @@ void launch_spread(const Launch& L) {
     return;
   }
   if (L.mixed) {
-    hipLaunchKernelGGL(k_brick_count<true>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    hipLaunchKernelGGL(k_brick_count<true>, dim3(L.w_atoms.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
     hipLaunchKernelGGL(k_brick_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.d_sys, L.nsys);
-    hipLaunchKernelGGL(k_brick_scatter<true>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    hipLaunchKernelGGL(k_brick_scatter<true>, dim3(L.w_atoms.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
     hipLaunchKernelGGL(k_brick_spread<true>, dim3(L.w_bricks.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_bricks.pre);
     hipLaunchKernelGGL(k_brick_reduce<true>, dim3(L.w_bricks.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_bricks.pre);
   } else {
-    hipLaunchKernelGGL(k_brick_count<false>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    hipLaunchKernelGGL(k_brick_count<false>, dim3(L.w_atoms.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
     hipLaunchKernelGGL(k_brick_scan, dim3(L.nsys), dim3(1024), 0, L.stream, L.d_sys, L.nsys);
-    hipLaunchKernelGGL(k_brick_scatter<false>, dim3(L.w_slots.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    hipLaunchKernelGGL(k_brick_scatter<false>, dim3(L.w_atoms.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
     hipLaunchKernelGGL(k_brick_spread<false>, dim3(L.w_bricks.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_bricks.pre);
     hipLaunchKernelGGL(k_brick_reduce<false>, dim3(L.w_bricks.total), b, 0, L.stream, L.d_sys, L.nsys, L.w_bricks.pre);
   }

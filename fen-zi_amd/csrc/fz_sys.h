@@ -142,7 +142,7 @@ struct SysDev {
   int n_bricks;
   int *br_cnt;     // [n_bricks] atoms per brick (counted each step, cleared by the scan)
   int *br_off;     // [n_bricks + 1] first record of each brick
-  int2 *br_slot;   // [n_slots_cap] (brick, rank in brick) of each slot's atom, brick -1: none
+  int2 *br_atom;   // [n_atoms] (brick, rank in brick) of each atom, brick -1: none
   void *br_rec;    // [n_atoms] 16 reals per atom: stencil weights x, y, z (4 each), q, block offset
   void *br_part;   // [n_bricks * 1331] brick blocks (int64 fixed point / double)
   // ---- constraints --------------------------------------------------------
