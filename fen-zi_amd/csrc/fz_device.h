@@ -120,4 +120,27 @@ struct PrecMixed {
   __device__ static __forceinline__ double get(const acc_t* p) { return from_fixed_acc(*p); }
 };
 
+// Slot-ordered copy of an atom's position and charge for the pair kernel:
+// int4 {x, y, z top 32 bits, FP32 charge} in the mixed form, Slot64 in FP64.
+struct Slot64 {
+  unsigned long long x, y, z;
+  double q;
+};
+__device__ __forceinline__ void store_slot_pos(const SysDev& d, int sl, unsigned long long x, unsigned long long y,
+                                               unsigned long long z, double q) {
+  if (d.mixed)
+    ((int4*)d.xs)[sl] = make_int4((int)(unsigned)(x >> 32), (int)(unsigned)(y >> 32), (int)(unsigned)(z >> 32),
+                                  __float_as_int((float)q));
+  else
+    ((Slot64*)d.xs)[sl] = Slot64{x, y, z, q};
+}
+// the whole slot (a < 0: padding): position, charge, LJ parameters, type
+__device__ __forceinline__ void store_slot(const SysDev& d, int sl, int a) {
+  if (a >= 0) store_slot_pos(d, sl, d.ux[a], d.uy[a], d.uz[a], d.q[a]);
+  else store_slot_pos(d, sl, 0ull, 0ull, 0ull, 0.0);
+  const int t = a >= 0 ? d.type[a] : 0;
+  if (d.mixed) d.lp[sl] = d.lj_sep ? d.lj_ah[t] : make_float2(__int_as_float(t), 0.0f);
+  d.ts[sl] = t;
+}
+
 }  // namespace fz

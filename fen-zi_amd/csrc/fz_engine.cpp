@@ -292,6 +292,7 @@ int System::alloc_device() {
   const int n = h.n;
   d.n_atoms = n;
   d.n_types = h.ntypes;
+  d.mixed = mixed;
   {  // odd multiplier near 0.618 n, coprime to n
     unsigned long long m = (unsigned long long)(0.6180339887 * n) | 1ull;
     while (std::gcd(m, (unsigned long long)n) != 1ull) m += 2ull;
@@ -729,7 +730,10 @@ int Engine::prepare() {
   if (!ev_fork_) FZ_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
   if (!ev_early_) FZ_HIP(hipEventCreateWithFlags(&ev_early_, hipEventDisableTiming));
   if (!d_sys_) FZ_HIP(hipMalloc(&d_sys_, ns * sizeof(SysDev)));
-  if (!L_.chunk_pre) FZ_HIP(hipMalloc(&L_.chunk_pre, (ns + 1) * sizeof(int)));
+  if (!L_.chunk_pre) {
+    FZ_HIP(hipMalloc(&L_.chunk_pre, (ns + 1) * sizeof(int)));
+    FZ_HIP(hipMemset(L_.chunk_pre, 0, (ns + 1) * sizeof(int)));
+  }
   FZ_TRY(upload_desc());
   L_.d_sys = d_sys_;
   L_.nsys = ns;
@@ -821,7 +825,10 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   launch_pme_part(side[1], 0, ev);
   launch_pme_part(side[1], 1, ev);
   launch_nblist(L_, check);
-  launch_gather_sorted(L_);
+  // In dynamics the slot copies are kept current by k_step_first and the list
+  // build, and a single system's work items are laid out by the build, so
+  // only batches and stand-alone evaluations run the slot gather.
+  if (check || L_.nsys > 1) launch_gather_sorted(L_);
   (void)hipEventRecord(ev_fork_, L_.stream);
   for (int k = 0; k < 2; k++) (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
   launch_bonded(side[0], ev);
@@ -950,6 +957,11 @@ int Engine::steps(int n) {
     }
     if (!g_plain_) FZ_TRY(capture(false, &g_plain_));
     if (!g_ev_) FZ_TRY(capture(true, &g_ev_));
+    // A single system's step graph has no slot gather: it relies on the slot
+    // copies and work items of the last list build or force evaluation, which
+    // may belong to another engine over the same system, so lay them out for
+    // this one first.
+    if (L_.nsys == 1) launch_gather_sorted(L_);
     for (int k = 0; k < n; k++) FZ_HIP(hipGraphLaunch(k == n - 1 ? g_ev_ : g_plain_, L_.stream));
     FZ_TRY(read_status());
     bool again = false;

@@ -38,10 +38,11 @@ __device__ __forceinline__ void kick_drift_atom(const SysDev& d, int i) {
   const double vy = d.vy[i] + h * take_force<MIXED>(d, sl, 1, true);
   const double vz = d.vz[i] + h * take_force<MIXED>(d, sl, 2, true);
   d.vx[i] = vx; d.vy[i] = vy; d.vz[i] = vz;
-  const unsigned long long ux = d.ux[i], uy = d.uy[i], uz = d.uz[i];
-  d.ux[i] = ux + (unsigned long long)__double2ll_rn(d.dt * vx * d.isc[0]);
-  d.uy[i] = uy + (unsigned long long)__double2ll_rn(d.dt * vy * d.isc[1]);
-  d.uz[i] = uz + (unsigned long long)__double2ll_rn(d.dt * vz * d.isc[2]);
+  const unsigned long long ux = d.ux[i] + (unsigned long long)__double2ll_rn(d.dt * vx * d.isc[0]);
+  const unsigned long long uy = d.uy[i] + (unsigned long long)__double2ll_rn(d.dt * vy * d.isc[1]);
+  const unsigned long long uz = d.uz[i] + (unsigned long long)__double2ll_rn(d.dt * vz * d.isc[2]);
+  d.ux[i] = ux; d.uy[i] = uy; d.uz[i] = uz;
+  store_slot_pos(d, sl, ux, uy, uz, d.q[i]);  // the pair kernel's slot copy
 }
 
 // second half kick of atom i
@@ -337,15 +338,18 @@ __device__ __forceinline__ bool step_first_ccl(const SysDev& d, const CCl& k) {
     if (a < 0) continue;
     const double e[3] = {dr[m].x, dr[m].y, dr[m].z};
     double r2 = 0.0;
+    unsigned long long un3[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const unsigned long long un = u[m][c] + (unsigned long long)__double2ll_rn(e[c] * d.isc[c]);
+      un3[c] = un;
       U[c][a] = un;
       Vp[c][a] = v[m][c] + e[c] * idt;
       const double x = fx_d(R[c][a], un, d.sc[c]);
       r2 = c == 0 ? x * x : r2 + x * x;
     }
     move = move || r2 > d.skin2;
+    store_slot_pos(d, sl[m], un3[0], un3[1], un3[2], d.q[a]);  // the pair kernel's slot copy
     if (MIXED) {
 #pragma unroll
       for (int c = 0; c < 3; c++) ((unsigned long long*)d.Fs)[(size_t)c * d.n_slots_cap + sl[m]] = 0ull;

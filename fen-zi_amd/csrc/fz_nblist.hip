@@ -248,6 +248,7 @@ __global__ __launch_bounds__(256) void k_col_sort(const SysDev* __restrict__ S, 
     d.su[base + k] = x;
     d.su[nsc + base + k] = y;
     d.su[2 * nsc + base + k] = z;
+    store_slot(d, base + k, a);  // the pair kernel's copy (kept current by k_step_first)
   }
   for (int m = threadIdx.x; m < nclu; m += 256) {
     unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0, 0, 0};
@@ -564,7 +565,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // every tile.  Each wave forms both from the tile counts, so no separate scan
 // pass is needed.  A list over capacity is flagged and left unwritten.
 __global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
-                                                   int split_cap) {
+                                                   int split_cap, int* __restrict__ chunk_pre) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
   const int lane = threadIdx.x & 63;
@@ -583,16 +584,20 @@ __global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S,
   for (int k = (t << 6) * split + lane; k < I * split; k += 64) before += d.search_cnt[k];
   before = wave_sum(before);
   total = wave_sum(total);
+  // a single-system launch also lays out the pair kernel's work items (its
+  // list's PAIR_CHUNK-entry chunks; batches do it in k_gather_sorted)
   if (total > d.n_pairs_cap) {
     if (I == 0 && lane == 0) {
       atomicOr(d.flags + F_OVERFLOW, OV_PAIRS);
       *d.n_pairs = 0;
+      if (nsys == 1) { chunk_pre[0] = 0; chunk_pre[1] = 0; }
     }
     return;
   }
   if (I == 0 && lane == 0) {
     *d.n_pairs = total;
     d.flags[F_NBUILD] += 1;
+    if (nsys == 1) { chunk_pre[0] = 0; chunk_pre[1] = (int)((total + PAIR_CHUNK - 1) / PAIR_CHUNK); }
   }
   long long off = before;
   for (int p = 0; p < split; p++) {
@@ -623,7 +628,7 @@ void launch_nblist(const Launch& L, bool check) {
   hipLaunchKernelGGL(k_pair_search, dim3(L.w_search.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_search.pre,
                      L.search_cap);
   hipLaunchKernelGGL(k_pair_fill, dim3(L.w_clus.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_clus.pre,
-                     L.search_cap);
+                     L.search_cap, L.chunk_pre);
 }
 
 }  // namespace fz

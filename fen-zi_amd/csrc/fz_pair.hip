@@ -22,11 +22,6 @@
 
 namespace fz {
 
-struct Slot64 {
-  unsigned long long x, y, z;
-  double q;
-};
-
 // ---------------------------------------------------------------------------
 // Slot-ordered copies of positions, charges and LJ parameters for the pair
 // kernel.  Block 0 also lays out the pair kernel's work items: per system,
@@ -49,24 +44,7 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict_
   const SysDev& d = S[s_];
   const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (sl >= *d.n_clusters * CLUSTER) return;
-  const int a = d.slot_atom[sl];
-  if (MIXED) {
-    int4 v = make_int4(0, 0, 0, 0);
-    if (a >= 0) {
-      v.x = (int)(unsigned)(d.ux[a] >> 32);
-      v.y = (int)(unsigned)(d.uy[a] >> 32);
-      v.z = (int)(unsigned)(d.uz[a] >> 32);
-      v.w = __float_as_int((float)d.q[a]);
-    }
-    ((int4*)d.xs)[sl] = v;
-    const int t = a >= 0 ? d.type[a] : 0;
-    d.lp[sl] = d.lj_sep ? d.lj_ah[t] : make_float2(__int_as_float(t), 0.0f);
-  } else {
-    Slot64 v = {0ull, 0ull, 0ull, 0.0};
-    if (a >= 0) v = Slot64{d.ux[a], d.uy[a], d.uz[a], d.q[a]};
-    ((Slot64*)d.xs)[sl] = v;
-  }
-  d.ts[sl] = a >= 0 ? d.type[a] : 0;
+  store_slot(d, sl, d.slot_atom[sl]);
 }
 
 void launch_gather_sorted(const Launch& L) {
@@ -279,6 +257,8 @@ struct EvAcc {
 // the combination rule (per-slot LJ parameters, no table lookup).
 template <int EV, bool SEP>
 __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S, int nsys, const int* __restrict__ chunk_pre) {
+  // the rebuild decision of this step is spent (single-system dynamics runs no slot gather)
+  if (nsys == 1 && blockIdx.x == 0 && threadIdx.x == 0) S[0].flags[F_REBUILD] = 0;
   const int lane = threadIdx.x & 63, il = lane & 7, jl = lane >> 3;
   const int total = chunk_pre[nsys];
   const int nw = gridDim.x * 4;
@@ -474,6 +454,8 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
 
 template <bool EV>
 __global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, int nsys, const int* __restrict__ chunk_pre) {
+  // the rebuild decision of this step is spent (single-system dynamics runs no slot gather)
+  if (nsys == 1 && blockIdx.x == 0 && threadIdx.x == 0) S[0].flags[F_REBUILD] = 0;
   const int lane = threadIdx.x & 63, il = lane & 7, jl = lane >> 3;
   const int total = chunk_pre[nsys];
   const int nw = gridDim.x * 4;

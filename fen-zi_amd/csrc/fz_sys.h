@@ -123,6 +123,7 @@ struct SysDev {
   // lj_sep = 1 and lj_ah[t] = (sqrt(eps_tt), sigma_tt / 2); the pair kernel then
   // forms the pair parameters from per-slot values instead of a table lookup.
   int lj_sep;
+  int mixed;             // precision form: slot layout of xs (int4 mixed, Slot64 FP64)
   float2 *lj_ah;                          // [n_types]
   float2 *lp;                             // [n_slots_cap] per slot: lj_ah[type] (lj_sep) or (type bits, 0)
   double2 *lj14;                          // 1-4 (eps, sigma^2)
