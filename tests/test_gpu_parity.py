@@ -173,6 +173,26 @@ def test_batch_matches_alone_bitwise(ctx):
     b.close()
 
 
+def test_second_engine_over_a_built_system_steps_identically(ctx):
+    """A batch of one over a system whose list another engine built starts its
+    dynamics without a rebuild; it lays out its own pair-kernel work first and
+    continues the trajectory bitwise."""
+    topo, xyz, params = systems.slab(seed=1)
+    a = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+    a.reassign_velocities(300.0, 4, 0)
+    a.step(10)
+    ref = a.get_fixed()
+    a.close()
+    b = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+    b.reassign_velocities(300.0, 4, 0)
+    b.step(5)
+    one = fenzi.Batch(ctx, [b])
+    one.step(5)
+    assert np.array_equal(b.get_fixed(), ref)
+    one.close()
+    b.close()
+
+
 def test_brick_and_atomic_spread_give_the_same_trajectory(ctx):
     """A launch over 64K atoms or more spreads PME charges by grid bricks, a smaller
     one by global atomics: three ~23K-atom systems run alone (atomic path) and as
