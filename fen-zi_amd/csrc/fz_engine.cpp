@@ -827,8 +827,10 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   launch_nblist(L_, check);
   // In dynamics the slot copies are kept current by k_step_first and the list
   // build, and a single system's work items are laid out by the build, so
-  // only batches and stand-alone evaluations run the slot gather.
-  if (check || L_.nsys > 1) launch_gather_sorted(L_);
+  // only stand-alone evaluations run the slot gather; a batch's step lays out
+  // its work items with one single-thread kernel.
+  if (check) launch_gather_sorted(L_);
+  else if (L_.nsys > 1) launch_chunk_prefix(L_);
   (void)hipEventRecord(ev_fork_, L_.stream);
   for (int k = 0; k < 2; k++) (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
   launch_bonded(side[0], ev);
@@ -957,11 +959,11 @@ int Engine::steps(int n) {
     }
     if (!g_plain_) FZ_TRY(capture(false, &g_plain_));
     if (!g_ev_) FZ_TRY(capture(true, &g_ev_));
-    // A single system's step graph has no slot gather: it relies on the slot
-    // copies and work items of the last list build or force evaluation, which
-    // may belong to another engine over the same system, so lay them out for
-    // this one first.
-    if (L_.nsys == 1) launch_gather_sorted(L_);
+    // The step graph has no slot gather: it relies on the slot copies and
+    // work items of the last list build or force evaluation, which may belong
+    // to another engine over the same systems, so lay them out for this one
+    // first.
+    launch_gather_sorted(L_);
     for (int k = 0; k < n; k++) FZ_HIP(hipGraphLaunch(k == n - 1 ? g_ev_ : g_plain_, L_.stream));
     FZ_TRY(read_status());
     bool again = false;

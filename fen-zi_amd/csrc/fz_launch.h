@@ -45,6 +45,7 @@ struct Launch {
 // check: run the displacement test first (the fused step kernel does it otherwise)
 void launch_nblist(const Launch& L, bool check);
 void launch_gather_sorted(const Launch& L);
+void launch_chunk_prefix(const Launch& L);  // batches: pair-kernel work items, rebuild flags cleared
 void launch_pair(const Launch& L, int ev);  // 0 forces, 1 + FP32 energies, 2 + FP64 energies
 void launch_bonded(const Launch& L, bool ev);
 void launch_pme(const Launch& L, bool ev, int conv_mode);

@@ -47,6 +47,25 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict_
   store_slot(d, sl, d.slot_atom[sl]);
 }
 
+// One thread: the pair kernel's work items of a batch (per system, the chunks
+// of its list; none after an overflow), and the rebuild decisions are spent.
+// The step graph of a batch runs this instead of the slot gather.
+__global__ void k_chunk_prefix(const SysDev* __restrict__ S, int nsys, int* __restrict__ chunk_pre) {
+  if (threadIdx.x != 0) return;
+  int run = 0;
+  chunk_pre[0] = 0;
+  for (int s = 0; s < nsys; s++) {
+    const long long np = S[s].flags[F_OVERFLOW] ? 0 : *S[s].n_pairs;
+    run += (int)((np + PAIR_CHUNK - 1) / PAIR_CHUNK);
+    chunk_pre[s + 1] = run;
+    S[s].flags[F_REBUILD] = 0;
+  }
+}
+
+void launch_chunk_prefix(const Launch& L) {
+  hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(64), 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+}
+
 void launch_gather_sorted(const Launch& L) {
   if (L.mixed)
     hipLaunchKernelGGL(k_gather_sorted<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre,
