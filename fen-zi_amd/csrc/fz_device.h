@@ -6,6 +6,10 @@
 
 #include "fz_sys.h"
 
+// global address space: pointers so qualified compile to global_* memory
+// instructions instead of generic (flat) ones
+#define FZ_GLOBAL __attribute__((address_space(1)))
+
 namespace fz {
 
 constexpr double CC = 332.0716;     // reference src/defs.in.h:157

@@ -396,36 +396,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre, int split_cap) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
+  // a private copy of the descriptor: the entry stores below cannot alias it,
+  // so its fields stay in registers instead of being reloaded per hit
+  const SysDev dv = d;
   const int lane = threadIdx.x & 63;
   const int il = lane & 7, jl = lane >> 3;
-  const int split = search_split(d.search_parts, split_cap);
+  const int split = search_split(dv.search_parts, split_cap);
   const int gw = lb * 4 + (threadIdx.x >> 6);
   const int I = gw / split, part = gw - I * split;
-  if (I >= *d.n_clusters) return;
-  const int pcap = d.search_parts * SEARCH_TMP_CAP / split;  // entries per part
-  const int nsc = d.n_slots_cap;
-  const unsigned* su32 = (const unsigned*)d.su;  // high words at odd indices
-  const int ai = d.slot_atom[I * CLUSTER + il];
+  if (I >= *dv.n_clusters) return;
+  const int pcap = dv.search_parts * SEARCH_TMP_CAP / split;  // entries per part
+  const int nsc = dv.n_slots_cap;
+  // global-address-space views: global_* loads and stores, which wait on the
+  // vector memory counter only (generic ones also wait on the LDS counter)
+  const FZ_GLOBAL unsigned* su32 = (const FZ_GLOBAL unsigned*)dv.su;  // high words at odd indices
+  const FZ_GLOBAL int* slot_atom = (const FZ_GLOBAL int*)dv.slot_atom;
+  const FZ_GLOBAL unsigned long long* bbg = (const FZ_GLOBAL unsigned long long*)dv.bb;
+  const int ai = slot_atom[I * CLUSTER + il];
   const unsigned aix = su32[2 * (I * CLUSTER + il) + 1], aiy = su32[2 * (nsc + I * CLUSTER + il) + 1],
                  aiz = su32[2 * (2 * nsc + I * CLUSTER + il) + 1];
-  const unsigned long long wa = (ai >= 0 && !d.use_excl_list) ? d.excl_bits[ai] : 0ull;
-  const size_t obase = (size_t)I * d.search_parts * SEARCH_TMP_CAP + (size_t)part * pcap;
-  unsigned long long* mout = d.search_mtmp + obase;
-  int* out = d.search_tmp + obase;
+  const unsigned long long wa = (ai >= 0 && !dv.use_excl_list) ? dv.excl_bits[ai] : 0ull;
+  const size_t obase = (size_t)I * dv.search_parts * SEARCH_TMP_CAP + (size_t)part * pcap;
+  FZ_GLOBAL unsigned long long* mout = (FZ_GLOBAL unsigned long long*)(dv.search_mtmp + obase);
+  FZ_GLOBAL int* out = (FZ_GLOBAL int*)(dv.search_tmp + obase);
   unsigned long long bi[6];
 #pragma unroll
-  for (int e = 0; e < 6; e++) bi[e] = d.bb[6 * (size_t)I + e];
-  const int cI = d.clu_col[I];
-  const int cx = cI % d.ncx, cy = cI / d.ncx;
-  const double ax0 = (double)bi[0] * d.sc[0], ax1 = (double)bi[1] * d.sc[0];
-  const double ay0 = (double)bi[2] * d.sc[1], ay1 = (double)bi[3] * d.sc[1];
-  const double wx = d.L[0] / d.ncx, wy = d.L[1] / d.ncy;
-  const bool allx = 2 * d.kx_range + 1 >= d.ncx, ally = 2 * d.ky_range + 1 >= d.ncy;
-  const int nyv = ally ? d.ncy : 2 * d.ky_range + 1;
-  const int nxv = allx ? d.ncx : 2 * d.kx_range + 1;
+  for (int e = 0; e < 6; e++) bi[e] = bbg[6 * (size_t)I + e];
+  const int cI = dv.clu_col[I];
+  const int cx = cI % dv.ncx, cy = cI / dv.ncx;
+  const double ax0 = (double)bi[0] * dv.sc[0], ax1 = (double)bi[1] * dv.sc[0];
+  const double ay0 = (double)bi[2] * dv.sc[1], ay1 = (double)bi[3] * dv.sc[1];
+  const double wx = dv.L[0] / dv.ncx, wy = dv.L[1] / dv.ncy;
+  const bool allx = 2 * dv.kx_range + 1 >= dv.ncx, ally = 2 * dv.ky_range + 1 >= dv.ncy;
+  const int nyv = ally ? dv.ncy : 2 * dv.ky_range + 1;
+  const int nxv = allx ? dv.ncx : 2 * dv.kx_range + 1;
   // z window [w0, w1] on the 2^64 circle
-  const int nzb = d.nzb;
-  const unsigned long long R = d.zwin;
+  const int nzb = dv.nzb;
+  const unsigned long long R = dv.zwin;
   const unsigned long long binw = ~0ull / (unsigned long long)nzb;
   const bool zfull = R >= (1ull << 61) || bi[5] - bi[4] >= ~0ull - 2 * R - binw;
   const unsigned long long w0 = bi[4] - R, w1 = bi[5] + R;
@@ -442,8 +449,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     if (lane < fill) {
       unsigned long long bj[6];
 #pragma unroll
-      for (int e = 0; e < 6; e++) bj[e] = d.bb[6 * (size_t)cand + e];
-      hit = bbox_within(bi, bj, d);
+      for (int e = 0; e < 6; e++) bj[e] = bbg[6 * (size_t)cand + e];
+      hit = bbox_within(bi, bj, dv);
     }
     unsigned long long bal = __ballot(hit);
     // hits in groups of four: the atoms of all four are loaded before the
@@ -465,7 +472,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         bg[h] = -1; xg[h] = yg[h] = zg[h] = 0u;
         if (Jg[h] >= 0) {
           const int sl = Jg[h] * CLUSTER + jl;
-          bg[h] = d.slot_atom[sl];
+          bg[h] = slot_atom[sl];
           xg[h] = su32[2 * sl + 1]; yg[h] = su32[2 * (nsc + sl) + 1]; zg[h] = su32[2 * (2 * nsc + sl) + 1];
         }
       }
@@ -474,8 +481,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         if (Jg[h] < 0) break;
         const int Jk = Jg[h], b = bg[h];
         bool on = ai >= 0 && b >= 0 && (I != Jk || il < jl);
-        if (on) on = !excluded_w(d, wa, ai, b);
-        if (on) on = r2_list32(d, aix, aiy, aiz, xg[h], yg[h], zg[h]) < d.rl2f;
+        if (on) on = !excluded_w(dv, wa, ai, b);
+        if (on) on = r2_list32(dv, aix, aiy, aiz, xg[h], yg[h], zg[h]) < dv.rl2f;
         const unsigned long long m = __ballot(on);
         if (m) {
           if (count < pcap) {
@@ -496,15 +503,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int q = g + lane;
     if (q < q1) {
       const int iy = q / nxv, ix = q - iy * nxv;
-      const int yy = ally ? iy : (cy + iy - d.ky_range + 2 * d.ncy) % d.ncy;
-      const int xx = allx ? ix : (cx + ix - d.kx_range + 2 * d.ncx) % d.ncx;
-      const double gy = col_gap(ay0, ay1, yy, wy, d.L[1]);
-      const double gx = col_gap(ax0, ax1, xx, wx, d.L[0]);
-      if (gx * gx + gy * gy < d.rl2) {
-        const int C = yy * d.ncx + xx;
-        const int j0 = d.col_clu0[C], nc = d.col_clu0[C + 1] - j0;
-        const int* th = d.zt_hi + (size_t)C * (nzb + 1);
-        const int* tl = d.zt_lo + (size_t)C * (nzb + 1);
+      const int yy = ally ? iy : (cy + iy - dv.ky_range + 2 * dv.ncy) % dv.ncy;
+      const int xx = allx ? ix : (cx + ix - dv.kx_range + 2 * dv.ncx) % dv.ncx;
+      const double gy = col_gap(ay0, ay1, yy, wy, dv.L[1]);
+      const double gx = col_gap(ax0, ax1, xx, wx, dv.L[0]);
+      if (gx * gx + gy * gy < dv.rl2) {
+        const int C = yy * dv.ncx + xx;
+        const FZ_GLOBAL int* clu0 = (const FZ_GLOBAL int*)dv.col_clu0;
+        const int j0 = clu0[C], nc = clu0[C + 1] - j0;
+        const FZ_GLOBAL int* th = (const FZ_GLOBAL int*)dv.zt_hi + (size_t)C * (nzb + 1);
+        const FZ_GLOBAL int* tl = (const FZ_GLOBAL int*)dv.zt_lo + (size_t)C * (nzb + 1);
         if (zfull) {
           r0a = 0; r0e = nc;
         } else if (!zwrap) {
@@ -553,9 +561,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   if (fill) test_hits();
   if (lane == 0) {
     const int kept = count < pcap ? count : pcap;
-    d.search_cnt[I * split + part] = kept;
-    if (kept) atomicAdd(d.tile_cnt + (I >> 6), kept);
-    if (over) atomicOr(d.flags + F_OVERFLOW, OV_SEARCH);
+    dv.search_cnt[I * split + part] = kept;
+    if (kept) atomicAdd(dv.tile_cnt + (I >> 6), kept);
+    if (over) atomicOr(dv.flags + F_OVERFLOW, OV_SEARCH);
   }
 }
 

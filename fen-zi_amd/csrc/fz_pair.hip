@@ -122,7 +122,6 @@ __device__ __forceinline__ void store_j(Acc* F, int cap, int J, unsigned long lo
 // Global-address-space views of the buffers: loads and atomics through them
 // compile to global_* instructions (SGPR base + VGPR offset) instead of flat_*,
 // which also keeps them out of the LDS counter the DPP/permute traffic waits on.
-#define FZ_GLOBAL __attribute__((address_space(1)))
 typedef FZ_GLOBAL int GI;
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v2f __attribute__((ext_vector_type(2)));
