@@ -117,7 +117,7 @@ enum { BE_BOND = 0, BE_ANGLE, BE_UB, BE_DIHED, BE_IMPR, BE_EWEXCL, BE_VDW, BE_EL
 
 template <class P, bool EV>
 __global__ __launch_bounds__(256) void k_bonded(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   int t = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   // atom window of the block

@@ -73,6 +73,11 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 // Largest s in [0, n) with pre[s] <= b (pre[0] = 0, non-decreasing).
+// System of block b of a launch over a work table: the table holds the n + 1
+// block prefixes followed by one system index per block (a single load).
+__device__ __forceinline__ int block_sys(const int* __restrict__ pre, int n, int b) { return pre[n + 1 + b]; }
+
+// index of the prefix interval holding b (binary search over n + 1 prefixes)
 __device__ __forceinline__ int locate(const int* __restrict__ pre, int n, int b) {
   int lo = 0, hi = n;
   while (hi - lo > 1) {

@@ -697,14 +697,18 @@ void Engine::drop_graphs() {
 }
 
 int Engine::make_work(Work& w, const std::vector<int>& blocks) {
+  // block prefixes per system, then the system of every block (block_sys)
   std::vector<int> pre(blocks.size() + 1, 0);
   for (size_t s = 0; s < blocks.size(); s++) pre[s + 1] = pre[s] + blocks[s];
+  const int total = pre.back();
+  for (size_t s = 0; s < blocks.size(); s++)
+    for (int k = 0; k < blocks[s]; k++) pre.push_back((int)s);
   int* d = nullptr;
   FZ_HIP(hipMalloc(&d, pre.size() * sizeof(int)));
   d_pre_.push_back(d);
   FZ_HIP(hipMemcpy(d, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
   w.pre = d;
-  w.total = pre.back();
+  w.total = total;
   return FZ_OK;
 }
 

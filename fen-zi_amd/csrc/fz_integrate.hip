@@ -223,7 +223,7 @@ __device__ __forceinline__ void shake_ccl(const SysDev& d, const CCl& k) {
 }
 
 __global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
   if (c < d.n_ccl) shake_ccl(d, load_ccl(d, c));
@@ -268,7 +268,7 @@ __device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k) {
 }
 
 __global__ __launch_bounds__(128) void k_rattle(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
   if (c < d.n_ccl) rattle_ccl(d, load_ccl(d, c));
@@ -365,7 +365,7 @@ __device__ __forceinline__ bool step_first_ccl(const SysDev& d, const CCl& k) {
 // and energy accumulators
 template <bool MIXED>
 __global__ __launch_bounds__(128) void k_step_first(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int lb = blockIdx.x - pre[s_];
   if (lb == 0 && threadIdx.x < A_NSLOT) d.acc[threadIdx.x] = 0ull;
@@ -384,7 +384,7 @@ __global__ __launch_bounds__(128) void k_step_first(const SysDev* __restrict__ S
 // second half kick and RATTLE
 template <bool MIXED>
 __global__ __launch_bounds__(128) void k_step_second(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int t = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
   if (t < d.n_ccl) {
@@ -410,7 +410,7 @@ void launch_step_second(const Launch& L) {
 // ---- kinetic energy, velocities ---------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_kinetic(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   double ke[1] = {0.0};
@@ -434,7 +434,7 @@ __device__ __forceinline__ void philox4x32(unsigned c[4], unsigned k0, unsigned 
 // v_d = sqrt(RC T / m) N(0,1) (src/init.cu:80-92); momentum accumulated for removal
 __global__ __launch_bounds__(256) void k_maxwell(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
                                                  double T, unsigned long long seed, unsigned long long counter) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   double p[3] = {0.0, 0.0, 0.0};
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(256) void k_maxwell(const SysDev* __restrict__ S, i
 // v_i -= P / (N m_i) over the N mobile atoms (src/init.cu:94-118; with fixed
 // atoms the reference sums over its first nAtomwoseg atoms, src/init.cu:77-127)
 __global__ __launch_bounds__(256) void k_momentum(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (i >= d.n_atoms) return;
@@ -474,7 +474,7 @@ __global__ __launch_bounds__(256) void k_momentum(const SysDev* __restrict__ S, 
 // largest |F|^2 per system; non-negative doubles order like their bit patterns
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_fmax(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   double f2 = 0.0;
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void k_fmax(const SysDev* __restrict__ S, int 
 // the previous positions are kept in o for SHAKE and for a rejected move
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_min_move(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (i >= d.n_atoms) return;
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(256) void k_min_move(const SysDev* __restrict__ S, 
 // when |F_prev|^2 < 1e-8); method 2, H = F * fac; method 0, no move.  u += H.
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_min_ref_move(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   const int n = d.n_atoms;
@@ -557,7 +557,7 @@ void launch_min_ref_move(const Launch& L) {
 
 __global__ __launch_bounds__(256) void k_min_restore(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
                                                      const int* __restrict__ reject) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (i >= d.n_atoms || !reject[s_]) return;
@@ -565,7 +565,7 @@ __global__ __launch_bounds__(256) void k_min_restore(const SysDev* __restrict__ 
 }
 
 __global__ __launch_bounds__(256) void k_zero_vel(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (i >= d.n_atoms) return;
@@ -596,7 +596,7 @@ __global__ void k_zero_acc(const SysDev* __restrict__ S, int nsys) {
 
 // clears the slot-ordered force accumulators (used before a stand-alone evaluation)
 __global__ __launch_bounds__(256) void k_zero_forces(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (sl >= d.n_slots_cap) return;

@@ -22,7 +22,7 @@
 namespace fz {
 
 #define SYS_FROM(pre)                                  \
-  const int s_ = locate(pre, nsys, blockIdx.x);        \
+  const int s_ = block_sys(pre, nsys, blockIdx.x);        \
   const SysDev& d = S[s_];                             \
   const int lb = blockIdx.x - pre[s_];                 \
   (void)lb;

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict_
       S[s].flags[F_REBUILD] = 0;
     }
   }
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (sl >= *d.n_clusters * CLUSTER) return;

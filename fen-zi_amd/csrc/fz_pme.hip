@@ -81,7 +81,7 @@ __device__ __forceinline__ double bcast(double v, int k) {
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_spread_atomic(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using R = typename std::conditional<MIXED, float, double>::type;
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int lane = threadIdx.x & 63;
   const int t0 = ((blockIdx.x - pre[s_]) * 4 + (threadIdx.x >> 6)) * 64;
@@ -146,7 +146,7 @@ constexpr int BRV = BRH * BRH * BRH;
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_brick_count(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using R = typename std::conditional<MIXED, float, double>::type;
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int a = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   const bool inside = a < d.n_atoms;
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(1024) void k_brick_scan(const SysDev* __restrict__ 
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_brick_scatter(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using R = typename std::conditional<MIXED, float, double>::type;
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int a = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (a >= d.n_atoms) return;
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void k_brick_spread(const SysDev* __restrict__
   using A = typename std::conditional<MIXED, unsigned long long, double>::type;
   __shared__ A blk[BRV];
   __shared__ R rec[64 * 16];
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int b = blockIdx.x - pre[s_];
   const int a0 = d.br_off[b], a1 = d.br_off[b + 1];
@@ -281,7 +281,7 @@ __device__ __forceinline__ int brick_cover(int x, int K, int nb, int* br, int* l
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_brick_reduce(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using A = typename std::conditional<MIXED, unsigned long long, double>::type;
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
   const int nb0 = d.nbr[0], nb1 = d.nbr[1];
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const SysDev* __restrict__ S,
   using R = typename std::conditional<MIXED, float, double>::type;
   using C = typename C2<R>::t;
   extern __shared__ char smem[];
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int lb = blockIdx.x - pre[s_];
   const int K0 = d.K[0], nlines = d.K[1] * d.K[2], Kxh = d.Kxh;
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(256) void k_fft_y(const SysDev* __restrict__ S, int
   using R = typename std::conditional<MIXED, float, double>::type;
   using C = typename C2<R>::t;
   extern __shared__ char smem[];
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int lb = blockIdx.x - pre[s_];
   const int K1 = d.K[1], Kxh = d.Kxh;
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(const SysDev* __restrict__ S
   using R = typename std::conditional<MIXED, float, double>::type;
   using C = typename C2<R>::t;
   extern __shared__ char smem[];
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int lb = blockIdx.x - pre[s_];
   const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2], Kxh = d.Kxh;
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const SysDev* __restrict__ S,
   using R = typename std::conditional<MIXED, float, double>::type;
   using C = typename C2<R>::t;
   extern __shared__ char smem[];
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int lb = blockIdx.x - pre[s_];
   const int K0 = d.K[0], nlines = d.K[1] * d.K[2], Kxh = d.Kxh;
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const SysDev* __restrict__ S,
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_pme_gather(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using R = typename std::conditional<MIXED, float, double>::type;
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
   if (sl >= *d.n_clusters * CLUSTER) return;

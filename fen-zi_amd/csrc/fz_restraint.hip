@@ -15,7 +15,7 @@
 namespace fz {
 
 __global__ __launch_bounds__(256) void k_com_sum(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   if (!d.com_on) return;  // uniform over the block
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void k_com_sum(const SysDev* __restrict__ S, i
 
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_com_force(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
-  const int s_ = locate(pre, nsys, blockIdx.x);
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   if (!d.com_on) return;
   const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
