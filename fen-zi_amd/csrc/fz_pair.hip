@@ -24,22 +24,9 @@ namespace fz {
 
 // ---------------------------------------------------------------------------
 // Slot-ordered copies of positions, charges and LJ parameters for the pair
-// kernel.  Block 0 also lays out the pair kernel's work items: per system,
-// the number of PAIR_CHUNK-entry chunks of its list (none after an overflow).
+// kernel (k_step_first and the list build keep them current in dynamics).
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
-                                                       int* __restrict__ chunk_pre) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // pair-kernel work items per system, and the rebuild flags are spent
-    int run = 0;
-    chunk_pre[0] = 0;
-    for (int s = 0; s < nsys; s++) {
-      const long long np = S[s].flags[F_OVERFLOW] ? 0 : *S[s].n_pairs;
-      run += (int)((np + PAIR_CHUNK - 1) / PAIR_CHUNK);
-      chunk_pre[s + 1] = run;
-      S[s].flags[F_REBUILD] = 0;
-    }
-  }
+__global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
@@ -47,32 +34,48 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict_
   store_slot(d, sl, d.slot_atom[sl]);
 }
 
-// One thread: the pair kernel's work items of a batch (per system, the chunks
-// of its list; none after an overflow), and the rebuild decisions are spent.
-// The step graph of a batch runs this instead of the slot gather.
-__global__ void k_chunk_prefix(const SysDev* __restrict__ S, int nsys, int* __restrict__ chunk_pre) {
-  if (threadIdx.x != 0) return;
-  int run = 0;
-  chunk_pre[0] = 0;
-  for (int s = 0; s < nsys; s++) {
-    const long long np = S[s].flags[F_OVERFLOW] ? 0 : *S[s].n_pairs;
-    run += (int)((np + PAIR_CHUNK - 1) / PAIR_CHUNK);
-    chunk_pre[s + 1] = run;
-    S[s].flags[F_REBUILD] = 0;
+// One block: the pair kernel's work items (per system, the PAIR_CHUNK-entry
+// chunks of its list; none after an overflow) as a prefix over the systems,
+// and the rebuild decisions are spent.  Systems are taken 1024 at a time, one
+// per thread, with a block scan and a running carry.
+__global__ __launch_bounds__(1024) void k_chunk_prefix(const SysDev* __restrict__ S, int nsys, int* __restrict__ chunk_pre) {
+  __shared__ int part[1024];
+  __shared__ int carry;
+  const int t = threadIdx.x;
+  if (t == 0) { carry = 0; chunk_pre[0] = 0; }
+  for (int base = 0; base < nsys; base += 1024) {
+    const int s = base + t;
+    int c = 0;
+    if (s < nsys) {
+      const long long np = S[s].flags[F_OVERFLOW] ? 0 : *S[s].n_pairs;
+      c = (int)((np + PAIR_CHUNK - 1) / PAIR_CHUNK);
+      S[s].flags[F_REBUILD] = 0;
+    }
+    part[t] = c;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+      const int v = t >= o ? part[t - o] : 0;
+      __syncthreads();
+      part[t] += v;
+      __syncthreads();
+    }
+    if (s < nsys) chunk_pre[s + 1] = carry + part[t];
+    __syncthreads();
+    if (t == 1023) carry += part[1023];
+    __syncthreads();
   }
 }
 
 void launch_chunk_prefix(const Launch& L) {
-  hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(64), 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+  hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
 }
 
 void launch_gather_sorted(const Launch& L) {
   if (L.mixed)
-    hipLaunchKernelGGL(k_gather_sorted<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre,
-                       L.chunk_pre);
+    hipLaunchKernelGGL(k_gather_sorted<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
   else
-    hipLaunchKernelGGL(k_gather_sorted<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre,
-                       L.chunk_pre);
+    hipLaunchKernelGGL(k_gather_sorted<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+  launch_chunk_prefix(L);
 }
 
 // ---------------------------------------------------------------------------
