@@ -837,9 +837,11 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   else if (L_.nsys > 1) launch_chunk_prefix(L_);
   (void)hipEventRecord(ev_fork_, L_.stream);
   for (int k = 0; k < 2; k++) (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
+  // the pair kernel is recorded first: the graph submits nodes in recording
+  // order, and the critical path runs through it
+  launch_pair(L_, ev ? (lite ? 1 : 2) : 0);
   launch_bonded(side[0], ev);
   launch_pme_part(side[1], 2, ev);
-  launch_pair(L_, ev ? (lite ? 1 : 2) : 0);
   for (int k = 0; k < 2; k++) {
     (void)hipEventRecord(ev_join_[k], side_[k]);
     (void)hipStreamWaitEvent(L_.stream, ev_join_[k], 0);
