@@ -837,8 +837,8 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   else if (L_.nsys > 1) launch_chunk_prefix(L_);
   (void)hipEventRecord(ev_fork_, L_.stream);
   for (int k = 0; k < 2; k++) (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
-  // the pair kernel is recorded first: the graph submits nodes in recording
-  // order, and the critical path runs through it
+  // the pair kernel, on the critical path, is recorded before the side
+  // branches' kernels (measured: the replayed step is shorter that way)
   launch_pair(L_, ev ? (lite ? 1 : 2) : 0);
   launch_bonded(side[0], ev);
   launch_pme_part(side[1], 2, ev);
