@@ -480,9 +480,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       for (int h = 0; h < 4; h++) {
         if (Jg[h] < 0) break;
         const int Jk = Jg[h], b = bg[h];
-        bool on = ai >= 0 && b >= 0 && (I != Jk || il < jl);
-        if (on) on = !excluded_w(dv, wa, ai, b);
-        if (on) on = r2_list32(dv, aix, aiy, aiz, xg[h], yg[h], zg[h]) < dv.rl2f;
+        bool on;
+        if (!dv.use_excl_list) {
+          // branch free: the window bit is read for every lane (the shift is
+          // masked) and the distance computed for every lane
+          const unsigned sh = (unsigned)(b - ai + 32);
+          const bool ex = (sh < 64u) & (bool)((wa >> (sh & 63u)) & 1ull);
+          const bool near = r2_list32(dv, aix, aiy, aiz, xg[h], yg[h], zg[h]) < dv.rl2f;
+          on = (ai >= 0) & (b >= 0) & ((I != Jk) | (il < jl)) & !ex & near;
+        } else {
+          on = ai >= 0 && b >= 0 && (I != Jk || il < jl);
+          if (on) on = !excluded_w(dv, wa, ai, b);
+          if (on) on = r2_list32(dv, aix, aiy, aiz, xg[h], yg[h], zg[h]) < dv.rl2f;
+        }
         const unsigned long long m = __ballot(on);
         if (m) {
           if (count < pcap) {
