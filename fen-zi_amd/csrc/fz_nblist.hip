@@ -391,8 +391,8 @@ __device__ __forceinline__ float r2_list32(const SysDev& d, unsigned ax, unsigne
 //  contiguous share of its candidate columns and writing its own buffer part;
 //  the fill concatenates the parts in order, which is the order one wave
 //  would have produced.
-// (six waves per SIMD: the search waits on memory for every hit)
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_pair_search(
+// (five waves per SIMD: the search waits on memory for every group of hits)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_pair_search(
     const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre, int split_cap) {
   SYS_FROM(pre);
   if (skip_build(d)) return;
@@ -443,6 +443,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   int count = 0;
   bool over = false;
   int cand = 0, fill = 0;
+  // kept entries are buffered one per lane (lane k: the k-th of the current
+  // 64) and stored 64 at a time with coalesced stores
+  int bufJ = 0, bufMl = 0, bufMh = 0;
 
   auto test_hits = [&]() {
     bool hit = false;
@@ -496,7 +499,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         const unsigned long long m = __ballot(on);
         if (m) {
           if (count < pcap) {
-            if (lane == 0) { out[count] = Jk; mout[count] = m; }
+            const int k = count & 63;
+            const bool mine = lane == k;
+            bufJ = mine ? Jk : bufJ;
+            bufMl = mine ? (int)(unsigned)m : bufMl;
+            bufMh = mine ? (int)(unsigned)(m >> 32) : bufMh;
+            if (k == 63) {
+              out[count - 63 + lane] = bufJ;
+              mout[count - 63 + lane] = ((unsigned long long)(unsigned)bufMh << 32) | (unsigned)bufMl;
+            }
           } else {
             over = true;
           }
@@ -569,6 +580,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     }
   }
   if (fill) test_hits();
+  {  // the buffered tail (pcap is a multiple of 64, so a full buffer was stored)
+    const int kept = count < pcap ? count : pcap;
+    const int tail = kept & 63, base = kept - tail;
+    if (lane < tail) {
+      out[base + lane] = bufJ;
+      mout[base + lane] = ((unsigned long long)(unsigned)bufMh << 32) | (unsigned)bufMl;
+    }
+  }
   if (lane == 0) {
     const int kept = count < pcap ? count : pcap;
     dv.search_cnt[I * split + part] = kept;
