@@ -99,7 +99,7 @@ def md(ctx, topo, x, p, prec, steps, warmup, seed=1):
     done = run_steps(s, warmup, 0, every, T, seed)
     # launch shapes (column width, FFT lines per workgroup, pair-kernel grid) by
     # the library's autotuner; it times trial steps and restores the state
-    tuned = s.autotune(10)
+    tuned = s.autotune(40)
     # fz_step returns once the device has finished (it reads the status flags)
     t0 = time.perf_counter()
     run_steps(s, steps, done, every, T, seed)
@@ -216,7 +216,7 @@ def batched(ctx, members, steps, warmup, alone_steps):
         s.reassign_velocities(300.0, 1000 + k, 0)
     log("batch: %d systems, %d atoms set up + relaxed in %.1fs" % (len(systems), sum(s.n for s in systems),
                                                                    time.perf_counter() - t0))
-    tune = b.autotune(6)
+    tune = b.autotune(16)
     log("batch autotune: %s" % tune)
     b.step(warmup)
     t0 = time.perf_counter()

@@ -793,13 +793,13 @@ int Engine::prepare() {
   FZ_TRY(make_work(L_.w_ffty, fy_));
   FZ_TRY(make_work(L_.w_fftz, fz_));
   FZ_TRY(make_work(L_.w_bricks, bricks));
-  // Default persistent pair grid: about 16 list chunks per block (a list holds
-  // ~19 cluster-pair entries per atom at water density, 32 entries per chunk),
-  // so that small systems leave compute units to the bonded and PME branches
-  // running beside the pair kernel.
+  // Default persistent pair grid: about ten list chunks per block (a list holds
+  // ~19 cluster-pair entries per atom at water density, 32 entries per chunk);
+  // measured on the 17K slab, 1,024 blocks beat 640 and 2,048 (the bonded and
+  // PME branches run beside the pair kernel).
   long long atoms_total = 0;
   for (auto* s : sys_) atoms_total += s->dv.n_atoms;
-  const int grid0 = (int)std::min<long long>(4096, std::max<long long>(256, (atoms_total / 27 + 63) / 64 * 64));
+  const int grid0 = (int)std::min<long long>(4096, std::max<long long>(256, (atoms_total / 16 + 63) / 64 * 64));
   L_.pair_grid = tuned_grid_ > 0 ? tuned_grid_ : grid0;
   // Spread by bricks from 64K atoms per launch; below, the single atomic kernel
   // is cheaper than the brick pipeline's five launches.  Same grid either way.
