@@ -412,6 +412,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   const FZ_GLOBAL unsigned* su32 = (const FZ_GLOBAL unsigned*)dv.su;  // high words at odd indices
   const FZ_GLOBAL int* slot_atom = (const FZ_GLOBAL int*)dv.slot_atom;
   const FZ_GLOBAL unsigned long long* bbg = (const FZ_GLOBAL unsigned long long*)dv.bb;
+  // slot j of cluster J: atom at byte 4 (8 J + j) of slot_atom, high words of
+  // x, y, z at bytes 8 (8 J + j) + 4 of the x, y, z blocks of su
+  const __amdgpu_buffer_rsrc_t sa_r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<int*>(dv.slot_atom), 0, (int)(nsc * 4u), 0x00020000);
+  const __amdgpu_buffer_rsrc_t su_r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<unsigned long long*>(dv.su), 0, (int)(3u * nsc * 8u), 0x00020000);
+  const unsigned vo_a = 4u * jl, vo_x = 8u * jl + 4u, vo_y = vo_x + 8u * nsc, vo_z = vo_x + 16u * nsc;
   const int ai = slot_atom[I * CLUSTER + il];
   const unsigned aix = su32[2 * (I * CLUSTER + il) + 1], aiy = su32[2 * (nsc + I * CLUSTER + il) + 1],
                  aiz = su32[2 * (2 * nsc + I * CLUSTER + il) + 1];
@@ -474,9 +481,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
       for (int h = 0; h < 4; h++) {
         bg[h] = -1; xg[h] = yg[h] = zg[h] = 0u;
         if (Jg[h] >= 0) {
-          const int sl = Jg[h] * CLUSTER + jl;
-          bg[h] = slot_atom[sl];
-          xg[h] = su32[2 * sl + 1]; yg[h] = su32[2 * (nsc + sl) + 1]; zg[h] = su32[2 * (2 * nsc + sl) + 1];
+          // buffer loads: the cluster part of the offset is scalar, the lane
+          // part a per-lane constant, so no vector address arithmetic per hit
+          const unsigned so_a = (unsigned)Jg[h] * (CLUSTER * 4u), so_u = (unsigned)Jg[h] * (CLUSTER * 8u);
+          bg[h] = (int)__builtin_amdgcn_raw_buffer_load_b32(sa_r, vo_a, so_a, 0);
+          xg[h] = __builtin_amdgcn_raw_buffer_load_b32(su_r, vo_x, so_u, 0);
+          yg[h] = __builtin_amdgcn_raw_buffer_load_b32(su_r, vo_y, so_u, 0);
+          zg[h] = __builtin_amdgcn_raw_buffer_load_b32(su_r, vo_z, so_u, 0);
         }
       }
 #pragma unroll
