@@ -175,8 +175,10 @@ int fz_minimize_ex(fz_system *sys, int nsteps, const fz_min_params *p);
 int fz_system_info(fz_system *sys, int64_t *n_atoms, int64_t *device_bytes, int32_t *precision);
 /* Per-phase device time: runs nsteps steps without graph capture, with HIP
  * events between phases on the system's stream; mean ms per step per phase in
- * ms[FZ_NPHASE], phases: 0 first kick + drift + SHAKE, 1 neighbour check and
- * rebuild, 2 sorted-coordinate gather, 3 pair kernel, 4 bonded + exclusion
+ * ms[FZ_NPHASE], phases: 0 first kick + drift + SHAKE + rebuild test, 1
+ * neighbour rebuild (early exits when none is due), 2 sorted-coordinate gather
+ * (run here every step; replayed dynamics steps keep the copies in phase 0 and
+ * the rebuild), 3 pair kernel, 4 bonded + exclusion
  * terms, 5 PME spread, 6 PME FFT + convolution, 7 PME force gather,
  * 8 second kick + RATTLE.  Replaces the reference's PROFILING clock() buckets
  * (src/globals.h:796-823). */
