@@ -849,11 +849,19 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   if (L_.com_any) launch_restraint(L_);
 }
 
+// A run of n steps is: first half (eager), then n - 1 replays of [forces,
+// second half of this step fused with the first half of the next], then one
+// replay of [forces with energies, second half, kinetic energy].  Every
+// replay thus holds one integrator kernel between force evaluations; the
+// arithmetic is that of separate halves.
 void Engine::record_step(bool ev) {
-  launch_step_first(L_);
   record_forces(ev, false, false);
-  launch_step_second(L_);
-  if (ev) launch_kinetic(L_);
+  if (!ev) {
+    launch_step_first(L_, true);
+  } else {
+    launch_step_second(L_);
+    launch_kinetic(L_);
+  }
 }
 
 int Engine::capture(bool ev, hipGraphExec_t* out) {
@@ -970,6 +978,7 @@ int Engine::steps(int n) {
     // to another engine over the same systems, so lay them out for this one
     // first.
     launch_gather_sorted(L_);
+    launch_step_first(L_);
     for (int k = 0; k < n; k++) FZ_HIP(hipGraphLaunch(k == n - 1 ? g_ev_ : g_plain_, L_.stream));
     FZ_TRY(read_status());
     bool again = false;

@@ -28,15 +28,21 @@ __device__ __forceinline__ double take_force(const SysDev& d, int sl, int comp, 
   }
 }
 
+// Kicks are explicit fused multiply-adds, so a kick rounds the same whether it
+// runs alone or next to another kick with the same product (a separate
+// product could otherwise be shared and rounded once for both).
 // first half kick + drift of atom i (an atom of no constraint cluster); clears
-// its force accumulators
-template <bool MIXED>
+// its force accumulators.  FINISH: the previous step's second half kick (same
+// forces) comes first, as a separate rounded addition.
+template <bool MIXED, bool FINISH = false>
 __device__ __forceinline__ void kick_drift_atom(const SysDev& d, int i) {
   const int sl = d.atom_slot[i];
   const double h = 0.5 * d.dt * d.mdyn[i];
-  const double vx = d.vx[i] + h * take_force<MIXED>(d, sl, 0, true);
-  const double vy = d.vy[i] + h * take_force<MIXED>(d, sl, 1, true);
-  const double vz = d.vz[i] + h * take_force<MIXED>(d, sl, 2, true);
+  const double fx = take_force<MIXED>(d, sl, 0, true), fy = take_force<MIXED>(d, sl, 1, true),
+               fz = take_force<MIXED>(d, sl, 2, true);
+  double vx = d.vx[i], vy = d.vy[i], vz = d.vz[i];
+  if (FINISH) { vx = fma(h, fx, vx); vy = fma(h, fy, vy); vz = fma(h, fz, vz); }
+  vx = fma(h, fx, vx); vy = fma(h, fy, vy); vz = fma(h, fz, vz);
   d.vx[i] = vx; d.vy[i] = vy; d.vz[i] = vz;
   const unsigned long long ux = d.ux[i] + (unsigned long long)__double2ll_rn(d.dt * vx * d.isc[0]);
   const unsigned long long uy = d.uy[i] + (unsigned long long)__double2ll_rn(d.dt * vy * d.isc[1]);
@@ -50,9 +56,9 @@ template <bool MIXED>
 __device__ __forceinline__ void kick_atom(const SysDev& d, int i) {
   const int sl = d.atom_slot[i];
   const double h = 0.5 * d.dt * d.mdyn[i];
-  d.vx[i] += h * take_force<MIXED>(d, sl, 0, false);
-  d.vy[i] += h * take_force<MIXED>(d, sl, 1, false);
-  d.vz[i] += h * take_force<MIXED>(d, sl, 2, false);
+  d.vx[i] = fma(h, take_force<MIXED>(d, sl, 0, false), d.vx[i]);
+  d.vy[i] = fma(h, take_force<MIXED>(d, sl, 1, false), d.vy[i]);
+  d.vz[i] = fma(h, take_force<MIXED>(d, sl, 2, false), d.vz[i]);
 }
 
 // ---- constraint clusters ------------------------------------------------------
@@ -229,16 +235,10 @@ __global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int
   if (c < d.n_ccl) shake_ccl(d, load_ccl(d, c));
 }
 
-__device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k) {
-  V3 r[4], v[4];
-  double mi[4];
-  load_rel(d, k, d.ux, d.uy, d.uz, r);
-#pragma unroll
-  for (int m = 0; m < 4; m++) {
-    const int a = k.at[m] >= 0 ? k.at[m] : k.at[0];
-    v[m] = V3{d.vx[a], d.vy[a], d.vz[a]};
-    mi[m] = k.at[m] >= 0 ? d.mdyn[a] : 0.0;
-  }
+// RATTLE on one cluster's velocities v (in place), r the positions relative to
+// the cluster's first atom, mi the inverse masses (0 for padding); v is left as
+// it is if the constraint system is singular
+__device__ __forceinline__ void rattle_core(const CCl& k, const V3 (&r)[4], V3 (&v)[4], const double (&mi)[4]) {
   V3 rc[3];
   double A[3][3], b[3], B[3][4], w[3][3];
   incidence(k, B);
@@ -255,15 +255,32 @@ __device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k) {
   if (!solve3(A, b, 3)) return;
 #pragma unroll
   for (int m = 0; m < 4; m++) {
-    const int a = k.at[m];
-    if (a < 0) continue;
     V3 s = v[m];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       const double l = j < k.nc ? b[j] * B[j][m] * mi[m] : 0.0;
       s.x += l * rc[j].x; s.y += l * rc[j].y; s.z += l * rc[j].z;
     }
-    d.vx[a] = s.x; d.vy[a] = s.y; d.vz[a] = s.z;
+    v[m] = s;
+  }
+}
+
+__device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k) {
+  V3 r[4], v[4];
+  double mi[4];
+  load_rel(d, k, d.ux, d.uy, d.uz, r);
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int a = k.at[m] >= 0 ? k.at[m] : k.at[0];
+    v[m] = V3{d.vx[a], d.vy[a], d.vz[a]};
+    mi[m] = k.at[m] >= 0 ? d.mdyn[a] : 0.0;
+  }
+  rattle_core(k, r, v, mi);
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int a = k.at[m];
+    if (a < 0) continue;
+    d.vx[a] = v[m].x; d.vy[a] = v[m].y; d.vz[a] = v[m].z;
   }
 }
 
@@ -290,9 +307,10 @@ __device__ __forceinline__ bool moved_atom(const SysDev& d, int i) {
 // Kick, drift and SHAKE of one constraint cluster with its state in registers:
 // every load is issued before the first store, so a thread waits on memory
 // about twice instead of once per atom and pass.  The arithmetic is that of
-// kick_drift_atom followed by shake_ccl.  Returns the list-rebuild test of the
-// cluster's atoms.
-template <bool MIXED>
+// kick_drift_atom followed by shake_ccl; FINISH puts the previous step's second
+// half kick and RATTLE (same forces, k_step_second's arithmetic) in front.
+// Returns the list-rebuild test of the cluster's atoms.
+template <bool MIXED, bool FINISH = false>
 __device__ __forceinline__ bool step_first_ccl(const SysDev& d, const CCl& k) {
   unsigned long long* U[3] = {d.ux, d.uy, d.uz};
   const unsigned long long* R[3] = {d.rx, d.ry, d.rz};
@@ -313,12 +331,26 @@ __device__ __forceinline__ bool step_first_ccl(const SysDev& d, const CCl& k) {
 #pragma unroll
     for (int c = 0; c < 3; c++) f[m][c] = take_force<MIXED>(d, sl[m], c, false);
   const double h0 = 0.5 * d.dt;
+  if (FINISH) {
+    V3 r[4], w[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const double h = h0 * mi[m];
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[m][c] = fma(h, f[m][c], v[m][c]);
+      r[m] = V3{fx_d(u[0][0], u[m][0], d.sc[0]), fx_d(u[0][1], u[m][1], d.sc[1]), fx_d(u[0][2], u[m][2], d.sc[2])};
+      w[m] = V3{v[m][0], v[m][1], v[m][2]};
+    }
+    rattle_core(k, r, w, mi);
+#pragma unroll
+    for (int m = 0; m < 4; m++) { v[m][0] = w[m].x; v[m][1] = w[m].y; v[m][2] = w[m].z; }
+  }
 #pragma unroll
   for (int m = 0; m < 4; m++) {
     const double h = h0 * mi[m];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      v[m][c] = v[m][c] + h * f[m][c];
+      v[m][c] = fma(h, f[m][c], v[m][c]);
       o[m][c] = u[m][c];
       u[m][c] = u[m][c] + (unsigned long long)__double2ll_rn(d.dt * v[m][c] * d.isc[c]);
     }
@@ -362,8 +394,9 @@ __device__ __forceinline__ bool step_first_ccl(const SysDev& d, const CCl& k) {
 }
 
 // first half kick, drift, SHAKE, and the list-rebuild test; clears the force
-// and energy accumulators
-template <bool MIXED>
+// and energy accumulators.  FINISH: the previous step's second half kick and
+// RATTLE first (one kernel between two force evaluations).
+template <bool MIXED, bool FINISH>
 __global__ __launch_bounds__(128) void k_step_first(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
@@ -372,10 +405,10 @@ __global__ __launch_bounds__(128) void k_step_first(const SysDev* __restrict__ S
   const int t = lb * 128 + threadIdx.x;
   bool move = false;
   if (t < d.n_ccl) {
-    move = step_first_ccl<MIXED>(d, load_ccl(d, t));
+    move = step_first_ccl<MIXED, FINISH>(d, load_ccl(d, t));
   } else if (t < d.n_ccl + d.n_free) {
     const int i = d.free_atoms[t - d.n_ccl];
-    kick_drift_atom<MIXED>(d, i);
+    kick_drift_atom<MIXED, FINISH>(d, i);
     move = moved_atom(d, i);
   }
   if (__any(move) && (threadIdx.x & 63) == 0) d.flags[F_REBUILD] = 1;
@@ -398,9 +431,15 @@ __global__ __launch_bounds__(128) void k_step_second(const SysDev* __restrict__ 
   }
 }
 
-void launch_step_first(const Launch& L) {
-  if (L.mixed) hipLaunchKernelGGL(k_step_first<true>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
-  else hipLaunchKernelGGL(k_step_first<false>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+void launch_step_first(const Launch& L, bool finish) {
+  const dim3 g(L.w_step.total), b(128);
+  if (L.mixed) {
+    if (finish) hipLaunchKernelGGL((k_step_first<true, true>), g, b, 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+    else hipLaunchKernelGGL((k_step_first<true, false>), g, b, 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+  } else {
+    if (finish) hipLaunchKernelGGL((k_step_first<false, true>), g, b, 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+    else hipLaunchKernelGGL((k_step_first<false, false>), g, b, 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+  }
 }
 void launch_step_second(const Launch& L) {
   if (L.mixed) hipLaunchKernelGGL(k_step_second<true>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);

@@ -51,7 +51,8 @@ void launch_bonded(const Launch& L, bool ev);
 void launch_pme(const Launch& L, bool ev, int conv_mode);
 void launch_pme_fft_only(const Launch& L, int conv_mode);
 void launch_pme_part(const Launch& L, int part, bool ev);  // 0 spread, 1 FFT + convolution, 2 gather
-void launch_step_first(const Launch& L);   // kick, drift, SHAKE, rebuild test
+// kick, drift, SHAKE, rebuild test; finish: the previous step's kick + RATTLE first
+void launch_step_first(const Launch& L, bool finish = false);
 void launch_step_second(const Launch& L);  // kick, RATTLE
 void launch_shake(const Launch& L);
 void launch_rattle(const Launch& L);
