@@ -119,7 +119,7 @@ template <class P, bool EV>
 __global__ __launch_bounds__(256) void k_bonded(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  int t = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  int t = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   // atom window of the block
   __shared__ typename P::acc_t lacc[3 * BOND_WIN];
   __shared__ int wlo, whi;

@@ -75,7 +75,10 @@ __device__ __forceinline__ T wave_sum(T v) {
 // Largest s in [0, n) with pre[s] <= b (pre[0] = 0, non-decreasing).
 // System of block b of a launch over a work table: the table holds the n + 1
 // block prefixes followed by one system index per block (a single load).
-__device__ __forceinline__ int block_sys(const int* __restrict__ pre, int n, int b) { return pre[n + 1 + b]; }
+// A single-system launch (n == 1) needs no table load at all.
+__device__ __forceinline__ int block_sys(const int* __restrict__ pre, int n, int b) { return n == 1 ? 0 : pre[n + 1 + b]; }
+// first block of system s
+__device__ __forceinline__ int block_base(const int* __restrict__ pre, int n, int s) { return n == 1 ? 0 : pre[s]; }
 
 // index of the prefix interval holding b (binary search over n + 1 prefixes)
 __device__ __forceinline__ int locate(const int* __restrict__ pre, int n, int b) {

@@ -231,7 +231,7 @@ __device__ __forceinline__ void shake_ccl(const SysDev& d, const CCl& k) {
 __global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
+  const int c = (blockIdx.x - block_base(pre, nsys, s_)) * 128 + threadIdx.x;
   if (c < d.n_ccl) shake_ccl(d, load_ccl(d, c));
 }
 
@@ -287,7 +287,7 @@ __device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k) {
 __global__ __launch_bounds__(128) void k_rattle(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int c = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
+  const int c = (blockIdx.x - block_base(pre, nsys, s_)) * 128 + threadIdx.x;
   if (c < d.n_ccl) rattle_ccl(d, load_ccl(d, c));
 }
 
@@ -400,7 +400,7 @@ template <bool MIXED, bool FINISH>
 __global__ __launch_bounds__(128) void k_step_first(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int lb = blockIdx.x - pre[s_];
+  const int lb = blockIdx.x - block_base(pre, nsys, s_);
   if (lb == 0 && threadIdx.x < A_NSLOT) d.acc[threadIdx.x] = 0ull;
   const int t = lb * 128 + threadIdx.x;
   bool move = false;
@@ -419,7 +419,7 @@ template <bool MIXED>
 __global__ __launch_bounds__(128) void k_step_second(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int t = (blockIdx.x - pre[s_]) * 128 + threadIdx.x;
+  const int t = (blockIdx.x - block_base(pre, nsys, s_)) * 128 + threadIdx.x;
   if (t < d.n_ccl) {
     const CCl k = load_ccl(d, t);
 #pragma unroll
@@ -451,7 +451,7 @@ void launch_step_second(const Launch& L) {
 __global__ __launch_bounds__(256) void k_kinetic(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int i = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   double ke[1] = {0.0};
   if (i < d.n_atoms) ke[0] = 0.5 / d.minv[i] * (d.vx[i] * d.vx[i] + d.vy[i] * d.vy[i] + d.vz[i] * d.vz[i]);
   const int slots[1] = {A_KIN};
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(256) void k_maxwell(const SysDev* __restrict__ S, i
                                                  double T, unsigned long long seed, unsigned long long counter) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int i = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   double p[3] = {0.0, 0.0, 0.0};
   if (i < d.n_atoms) {
     unsigned c[4] = {(unsigned)i, 0u, (unsigned)counter, (unsigned)(counter >> 32)};
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(256) void k_maxwell(const SysDev* __restrict__ S, i
 __global__ __launch_bounds__(256) void k_momentum(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int i = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   if (i >= d.n_atoms) return;
   const double f = d.n_mobile > 0 ? d.mdyn[i] / d.n_mobile : 0.0;
   d.vx[i] -= from_fixed_e(d.acc[A_MOM]) * f;
@@ -515,7 +515,7 @@ template <bool MIXED>
 __global__ __launch_bounds__(256) void k_fmax(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int i = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   double f2 = 0.0;
   if (i < d.n_atoms && d.mdyn[i] != 0.0) {
     const int sl = d.atom_slot[i];
@@ -534,7 +534,7 @@ template <bool MIXED>
 __global__ __launch_bounds__(256) void k_min_move(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int i = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   if (i >= d.n_atoms) return;
   const double f2max = __longlong_as_double((long long)d.acc[A_FMAX]);
   const double a = f2max > 0.0 ? *d.mstep / sqrt(f2max) : 0.0;
@@ -555,7 +555,7 @@ template <bool MIXED>
 __global__ __launch_bounds__(256) void k_min_ref_move(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int i = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   const int n = d.n_atoms;
   if (i >= n) return;
   const unsigned long long ux = d.ux[i], uy = d.uy[i], uz = d.uz[i];
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(256) void k_min_restore(const SysDev* __restrict__ 
                                                      const int* __restrict__ reject) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int i = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   if (i >= d.n_atoms || !reject[s_]) return;
   d.ux[i] = d.ox[i]; d.uy[i] = d.oy[i]; d.uz[i] = d.oz[i];
 }
@@ -606,7 +606,7 @@ __global__ __launch_bounds__(256) void k_min_restore(const SysDev* __restrict__ 
 __global__ __launch_bounds__(256) void k_zero_vel(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int i = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   if (i >= d.n_atoms) return;
   d.vx[i] = 0.0; d.vy[i] = 0.0; d.vz[i] = 0.0;
 }
@@ -637,7 +637,7 @@ __global__ void k_zero_acc(const SysDev* __restrict__ S, int nsys) {
 __global__ __launch_bounds__(256) void k_zero_forces(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int sl = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   if (sl >= d.n_slots_cap) return;
   unsigned long long* F = (unsigned long long*)d.Fs;
   F[sl] = 0ull;

@@ -24,7 +24,7 @@ namespace fz {
 #define SYS_FROM(pre)                                  \
   const int s_ = block_sys(pre, nsys, blockIdx.x);        \
   const SysDev& d = S[s_];                             \
-  const int lb = blockIdx.x - pre[s_];                 \
+  const int lb = blockIdx.x - block_base(pre, nsys, s_);                 \
   (void)lb;
 
 // ---------------------------------------------------------------------------

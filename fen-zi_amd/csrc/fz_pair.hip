@@ -29,7 +29,7 @@ template <bool MIXED>
 __global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int sl = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   if (sl >= *d.n_clusters * CLUSTER) return;
   store_slot(d, sl, d.slot_atom[sl]);
 }

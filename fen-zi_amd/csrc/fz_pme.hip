@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_spread_atomic(const SysDev* __restrict_
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int lane = threadIdx.x & 63;
-  const int t0 = ((blockIdx.x - pre[s_]) * 4 + (threadIdx.x >> 6)) * 64;
+  const int t0 = ((blockIdx.x - block_base(pre, nsys, s_)) * 4 + (threadIdx.x >> 6)) * 64;
   const int na = d.n_atoms;
   const unsigned long long mult = d.spread_mult;
   const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_brick_count(const SysDev* __restrict__ 
   using R = typename std::conditional<MIXED, float, double>::type;
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int a = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int a = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   const bool inside = a < d.n_atoms;
   int h = -1;
   if (inside) {
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_brick_scatter(const SysDev* __restrict_
   using R = typename std::conditional<MIXED, float, double>::type;
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int a = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int a = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   if (a >= d.n_atoms) return;
   const int2 r = d.br_atom[a];
   if (r.x < 0) return;
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_brick_spread(const SysDev* __restrict__
   __shared__ R rec[64 * 16];
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int b = blockIdx.x - pre[s_];
+  const int b = blockIdx.x - block_base(pre, nsys, s_);
   const int a0 = d.br_off[b], a1 = d.br_off[b + 1];
   if (a0 == a1) return;  // no block: the reduce skips empty bricks
   for (int i = threadIdx.x; i < BRV; i += 256) blk[i] = A(0);
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_brick_reduce(const SysDev* __restrict__
   const SysDev& d = S[s_];
   const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
   const int nb0 = d.nbr[0], nb1 = d.nbr[1];
-  const int b = blockIdx.x - pre[s_];
+  const int b = blockIdx.x - block_base(pre, nsys, s_);
   const int bx = b % nb0, byz = b / nb0, by = byz % nb1, bz = byz / nb1;
   const A* part = (const A*)d.br_part;
   for (int t = threadIdx.x; t < BRK * BRK * BRK; t += 256) {
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(256) void k_fft_x_r2c(const SysDev* __restrict__ S,
   extern __shared__ char smem[];
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int lb = blockIdx.x - pre[s_];
+  const int lb = blockIdx.x - block_base(pre, nsys, s_);
   const int K0 = d.K[0], nlines = d.K[1] * d.K[2], Kxh = d.Kxh;
   C* b0 = (C*)smem;
   C* b1 = b0 + NL * K0;
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(256) void k_fft_y(const SysDev* __restrict__ S, int
   extern __shared__ char smem[];
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int lb = blockIdx.x - pre[s_];
+  const int lb = blockIdx.x - block_base(pre, nsys, s_);
   const int K1 = d.K[1], Kxh = d.Kxh;
   const int ng = (Kxh + NL - 1) / NL;
   const int z = lb / ng, kx0 = (lb - z * ng) * NL;
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(const SysDev* __restrict__ S
   extern __shared__ char smem[];
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int lb = blockIdx.x - pre[s_];
+  const int lb = blockIdx.x - block_base(pre, nsys, s_);
   const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2], Kxh = d.Kxh;
   const int ng = (Kxh + NL - 1) / NL;
   const int y = lb / ng, kx0 = (lb - y * ng) * NL;
@@ -579,7 +579,7 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const SysDev* __restrict__ S,
   extern __shared__ char smem[];
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int lb = blockIdx.x - pre[s_];
+  const int lb = blockIdx.x - block_base(pre, nsys, s_);
   const int K0 = d.K[0], nlines = d.K[1] * d.K[2], Kxh = d.Kxh;
   C* b0 = (C*)smem;
   C* b1 = b0 + NL * K0;
@@ -609,7 +609,7 @@ __global__ __launch_bounds__(256) void k_pme_gather(const SysDev* __restrict__ S
   using R = typename std::conditional<MIXED, float, double>::type;
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int sl = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int sl = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   if (sl >= *d.n_clusters * CLUSTER) return;
   const int a = d.slot_atom[sl];
   if (a < 0) return;

@@ -18,7 +18,7 @@ __global__ __launch_bounds__(256) void k_com_sum(const SysDev* __restrict__ S, i
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   if (!d.com_on) return;  // uniform over the block
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int i = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < d.n_atoms) {
     const int g = d.com_grp[i];
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_com_force(const SysDev* __restrict__ S,
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   if (!d.com_on) return;
-  const int i = (blockIdx.x - pre[s_]) * 256 + threadIdx.x;
+  const int i = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   if (i >= d.n_atoms || d.com_grp[i] != 1) return;
   double dr[3];
 #pragma unroll
