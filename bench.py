@@ -96,10 +96,12 @@ def md(ctx, topo, x, p, prec, steps, warmup, seed=1):
     s = fenzi.System(ctx, topo, p, prec, xyz=x)
     T, every = p["temperature"], int(p.get("reassign_every", 0))
     s.reassign_velocities(T, seed, 0)
-    done = run_steps(s, warmup, 0, every, T, seed)
     # launch shapes (column width, FFT lines per workgroup, pair-kernel grid) by
-    # the library's autotuner; it times trial steps and restores the state
+    # the library's autotuner; it times trial steps and restores the state.  It
+    # runs before the warm-up, so that the warm-up captures the step graphs of
+    # the chosen shapes and the timed steps replay them.
     tuned = s.autotune(40)
+    done = run_steps(s, max(2, warmup), 0, every, T, seed)
     # fz_step returns once the device has finished (it reads the status flags)
     t0 = time.perf_counter()
     run_steps(s, steps, done, every, T, seed)
@@ -126,8 +128,16 @@ def nve_drift(s, n_atoms, dt_ps, segments=10, seg_steps=50):
             "e_total_first": e[0], "e_total_last": e[-1]}
 
 
+E_KEYS = {"vdw": "vdw", "elec": "elec", "pme": "pme", "ewald_excl": "ewexcl", "bond": "bond", "angle": "angle",
+          "dihedral": "dihed"}
+
+
 def compare_forms(ctx, topo, x, p):
-    """Mixed-precision form against the FP64 form at the same positions."""
+    """Both GPU forms against the FP64 oracle at the same positions (one frame),
+    and the mixed form against the FP64 form."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.set_threads(os.cpu_count() or 1)
     res = {}
     a = fenzi.System(ctx, topo, p, fenzi.FP64, xyz=x)
     f64, e64 = a.forces()
@@ -137,10 +147,24 @@ def compare_forms(ctx, topo, x, p):
     b.set_fixed(u)
     f32, e32 = b.forces()
     b.close()
-    res["force_rel_rms"] = float(np.sqrt(((f32 - f64) ** 2).sum() / (f64 ** 2).sum()))
-    res["energy_rel_err"] = {k: float(abs(e32[k] - e64[k]) / max(abs(e64[k]), 1e-12))
-                             for k in ("vdw", "elec", "pme", "ewald_excl", "bond", "angle", "dihedral", "potential")}
-    res["virial_rel_err"] = float(np.abs(e32["virial"] - e64["virial"]).max() / np.abs(e64["virial"]).max())
+    o = oracle.Oracle(topo, p)
+    fo, eo, wo = o.forces(u)
+    eo_pot = sum(eo.values())
+
+    def rel(a_, b_):
+        return float(np.sqrt(((a_ - b_) ** 2).sum() / (b_ ** 2).sum()))
+
+    for name, f, e in (("mixed", f32, e32), ("fp64", f64, e64)):
+        res[name + "_vs_oracle"] = {
+            "force_rel_rms": rel(f, fo),
+            "energy_rel_err": {k: float(abs(e[k] - eo[v]) / max(abs(eo[v]), 1e-12)) for k, v in E_KEYS.items()},
+            "potential_rel_err": float(abs(e["potential"] - eo_pot) / abs(eo_pot)),
+            "virial_rel_err": float(np.abs(e["virial"] - wo).max() / np.abs(wo).max())}
+    res["mixed_vs_fp64"] = {
+        "force_rel_rms": rel(f32, f64),
+        "energy_rel_err": {k: float(abs(e32[k] - e64[k]) / max(abs(e64[k]), 1e-12))
+                           for k in ("vdw", "elec", "pme", "ewald_excl", "bond", "angle", "dihedral", "potential")},
+        "virial_rel_err": float(np.abs(e32["virial"] - e64["virial"]).max() / np.abs(e64["virial"]).max())}
     return res
 
 
@@ -161,9 +185,9 @@ def roofline(run, prec, workload):
     peak = FP32_PEAK if prec == fenzi.MIXED else FP64_PEAK
     tr = traffic_per_launch() if workload == "slab230k" else None
     return {"kernel": "k_pair_mixed (LJ + real-space Ewald over cluster-pair tiles)", "workload": workload,
-            "bound": "mfma",
-            "note": "compute-bound on the FP32 vector ALU (no dense contraction to put on the matrix cores); "
-                    "MI355X's FP32 peak is 157.3 TFLOP/s for both",
+            "bound": "valu",
+            "note": "bound by FP32 vector-ALU issue (no dense contraction, no MFMA); priced at MI355X's published "
+                    "FP32 vector peak of 157.3 TFLOP/s (packed FP32 counted; without it the reachable peak is half)",
             "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s", "frac": ach / peak,
             "traffic": None if tr is None else tr["bytes_per_launch"],
             "traffic_source": None if tr is None else tr["source"],
@@ -185,37 +209,60 @@ def pme_hbm(run, n_atoms, K):
     return out
 
 
-def cpu_baseline(topo, x, p, budget_s=20.0):
-    """FP64 oracle (C + OpenMP) on the host cores: velocity Verlet + SHAKE/RATTLE + PME."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(topo, x, p, budget_s=15.0, max_steps=200):
+    """FP64 oracle (C + OpenMP) on all host cores: velocity Verlet + SHAKE/RATTLE
+    + PME, timed as the mean wall time per step over a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = oracle.set_threads(min(16, os.cpu_count() or 1))
+    threads = oracle.set_threads(os.cpu_count() or 1)
     o = oracle.Oracle(topo, p)
     u = o.to_fixed(x)
     v = o.maxwell(u, p["temperature"], 1, 0)
     t0 = time.perf_counter()
     u, v, _ = o.step(u, v, 1)
     t1 = max(time.perf_counter() - t0, 1e-3)
-    n = max(2, min(200, int(budget_s / t1)))
+    n = max(2, min(max_steps, int(budget_s / t1)))
     t0 = time.perf_counter()
     o.step(u, v, n)
     ms = (time.perf_counter() - t0) * 1e3 / n
     return {"value": ns_day(ms, p["dt_ps"]), "unit": "ns/day", "cores": threads, "kind": "port",
-            "sample": "%d steps of the %d-atom slab (FP64 oracle, C/OpenMP: switched LJ + erfc real space, "
-                      "order-4 PME with own FFT, bonded, SHAKE/RATTLE); %.1f ms/step" % (n, topo["n_atoms"], ms)}
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "ms_per_step": ms,
+            "sample": "%d steps of the %d-atom slab after 1 warm-up step (FP64 oracle, C/OpenMP on %d threads: "
+                      "switched LJ + erfc real space, order-4 PME with own FFT, bonded, SHAKE/RATTLE); %.1f ms/step"
+                      % (n, topo["n_atoms"], threads, ms)}
 
 
 # ---- batched ensemble ----------------------------------------------------------
 
 def batched(ctx, members, steps, warmup, alone_steps):
+    """The ensemble as one batch (one launch per phase per step, graph-replayed)
+    and the same systems one at a time (a batch of one each), on equal terms:
+    the same number of steps per call in both legs."""
     t0 = time.perf_counter()
     systems = [fenzi.System(ctx, t, p, fenzi.MIXED, xyz=x) for t, x, p in members]
     b = fenzi.Batch(ctx, systems)
-    b.minimize(100, 0.05)
+    b.minimize(150, 0.05)
+    # untimed equilibration at 300 K (velocities redrawn every 100 steps), so the
+    # timed NVE steps start from relaxed members
+    for r in range(4):
+        for k, s in enumerate(systems):
+            s.reassign_velocities(300.0, 1000 + k, r)
+        b.step(100)
     for k, s in enumerate(systems):
-        s.reassign_velocities(300.0, 1000 + k, 0)
-    log("batch: %d systems, %d atoms set up + relaxed in %.1fs" % (len(systems), sum(s.n for s in systems),
-                                                                   time.perf_counter() - t0))
+        s.reassign_velocities(300.0, 1000 + k, 9)
+    log("batch: %d systems, %d atoms set up, relaxed and equilibrated in %.1fs" % (
+        len(systems), sum(s.n for s in systems), time.perf_counter() - t0))
     tune = b.autotune(16)
     log("batch autotune: %s" % tune)
     b.step(warmup)
@@ -224,27 +271,30 @@ def batched(ctx, members, steps, warmup, alone_steps):
     t_batch = (time.perf_counter() - t0) / steps
     dt_ps = members[0][2]["dt_ps"]
     agg_batch = ns_day(t_batch * 1e3, dt_ps) * len(systems)
-    log("batch: %.3f ms per batch step, aggregate %.1f ns/day" % (t_batch * 1e3, agg_batch))
-    # one at a time: the same systems, each through its own engine (batch of one)
+    e = [s.energies()["temperature"] for s in systems]
+    log("batch: %.3f ms per batch step, aggregate %.1f ns/day, T mean %.0f K" % (t_batch * 1e3, agg_batch, np.mean(e)))
+    b.close()
+    # one at a time: the same systems, each through its own engine (batch of
+    # one), each timed over one call of alone_steps steps
     t_alone = 0.0
     for s in systems:
-        s.step(3)
+        s.step(max(2, warmup))
         t0 = time.perf_counter()
         s.step(alone_steps)
         t_alone += (time.perf_counter() - t0) / alone_steps
     agg_alone = ns_day(t_alone * 1e3, dt_ps) * len(systems)
     log("one at a time: %.3f ms per round of single steps, aggregate %.1f ns/day" % (t_alone * 1e3, agg_alone))
-    e = [s.energies()["temperature"] for s in systems]
-    b.close()
     for s in systems:
         s.close()
     return {"systems": len(members), "atoms": int(sum(m[0]["n_atoms"] for m in members)),
             "steps": steps, "ms_per_batch_step": t_batch * 1e3, "aggregate_ns_per_day": agg_batch,
             "one_at_a_time_ms_per_round": t_alone * 1e3, "one_at_a_time_aggregate_ns_per_day": agg_alone,
-            "one_at_a_time_steps_per_system": alone_steps, "speedup": agg_batch / agg_alone,
-            "autotune": tune, "temperature_mean": float(np.mean(e)),
+            "one_at_a_time_steps_per_call": alone_steps, "batch_steps_per_call": steps,
+            "speedup": agg_batch / agg_alone,
+            "autotune": {k: float(v) for k, v in tune.items()}, "temperature_mean": float(np.mean(e)),
             "workload": "256 systems, N log-uniform in [1500, 24000] (seed 3), chain fraction U[0, 0.3], "
-                        "cubic boxes at ~1 g/cm^3, cutoff 10 A, PME K in [16, 48], dt 2 fs, NVE, mixed"}
+                        "cubic boxes at ~1 g/cm^3, cutoff 10 A, PME K in [16, 48], dt 2 fs, NVE after an untimed "
+                        "300 K equilibration, mixed"}
 
 
 # ---- serving ---------------------------------------------------------------------
@@ -331,7 +381,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-water", action="store_true", help="skip the 2,400-water FP64 box")
     ap.add_argument("--batch-systems", type=int, default=256, help="0 skips the batched ensemble")
-    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--serve-jobs", type=int, default=100, help="0 skips the serving loop")
     ap.add_argument("--serve-chunk", type=int, default=100)
     ap.add_argument("--min-steps", type=int, default=300)
@@ -364,13 +414,14 @@ def main():
     ctx = fenzi.Context(0)
     topo1, x1, p1 = prepare(ctx, 1, a.min_steps)
     r1m, s = md(ctx, topo1, x1, p1, fenzi.MIXED, a.steps, a.warmup)
-    s.close()
     log("17K mixed: %.3f ms/step %.1f ns/day" % (r1m["ms_per_step"], r1m["ns_per_day"]))
+    r1m["nve_drift"] = nve_drift(s, topo1["n_atoms"], p1["dt_ps"])
+    s.close()
     r1d, s = md(ctx, topo1, x1, p1, fenzi.FP64, a.steps, a.warmup)
     s.close()
     log("17K fp64 : %.3f ms/step %.1f ns/day" % (r1d["ms_per_step"], r1d["ns_per_day"]))
     err1 = compare_forms(ctx, topo1, x1, p1)
-    extra = {"slab17k_mixed": r1m, "slab17k_fp64": r1d, "slab17k_mixed_vs_fp64": err1,
+    extra = {"slab17k_mixed": r1m, "slab17k_fp64": r1d, "slab17k_errors": err1,
              "slab17k_mixed_over_fp64": r1d["ms_per_step"] / r1m["ms_per_step"],
              "slab17k_mixed_pme_hbm": pme_hbm(r1m, topo1["n_atoms"], p1["K"])}
     r2m = None
@@ -387,7 +438,7 @@ def main():
         s.close()
         log("230K fp64 : %.3f ms/step %.1f ns/day" % (r2d["ms_per_step"], r2d["ns_per_day"]))
         extra.update({"slab230k_mixed": r2m, "slab230k_fp64": r2d,
-                      "slab230k_mixed_vs_fp64": compare_forms(ctx, topo2, x2, p2),
+                      "slab230k_errors": compare_forms(ctx, topo2, x2, p2),
                       "slab230k_mixed_over_fp64": r2d["ms_per_step"] / r2m["ms_per_step"],
                       "slab230k_mixed_pme_hbm": pme_hbm(r2m, topo2["n_atoms"], p2["K"])})
     if not a.no_water:
@@ -398,7 +449,7 @@ def main():
         log("2400 waters fp64: %.3f ms/step %.1f ns/day" % (r0["ms_per_step"], r0["ns_per_day"]))
         extra["water2400_fp64"] = r0
     if members:
-        extra["batch256_mixed"] = batched(ctx, members, a.batch_steps, 20, 30)
+        extra["batch256_mixed"] = batched(ctx, members, a.batch_steps, 20, a.batch_steps)
     if jobs:
         extra["serving_mixed"] = serving(ctx, jobs, arrival, md_steps, 200e9, a.serve_chunk)
         sv = extra["serving_mixed"]
@@ -408,6 +459,9 @@ def main():
     if not a.no_cpu:
         cpu = cpu_baseline(topo1, x1, p1)
         log("cpu oracle: %.3f ns/day on %d threads" % (cpu["value"], cpu["cores"]))
+        if not a.no_large:
+            extra["cpu_baseline_slab230k"] = cpu_baseline(topo2, x2, p2, budget_s=10.0, max_steps=10)
+            log("cpu oracle 230K: %.3f ns/day" % extra["cpu_baseline_slab230k"]["value"])
     line = {
         "metric": METRIC, "value": r1m["ns_per_day"], "unit": "ns/day", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": r1m["ms_per_step"], "higher_is_better": True, "scaling": "weak",

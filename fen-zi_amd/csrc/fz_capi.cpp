@@ -168,6 +168,14 @@ int fz_step(fz_system* s, int nsteps) {
   FZ_GUARD_END
 }
 
+int fz_set_neighbor_capacity(fz_system* s, int64_t pairs_cap, int32_t clusters_cap) {
+  FZ_GUARD_BEGIN
+  if (!s) return set_error(FZ_ERR_ARG, "null argument");
+  FZ_HIP(hipSetDevice(s->ctx->device));
+  return s->eng->set_capacity(0, pairs_cap, clusters_cap);
+  FZ_GUARD_END
+}
+
 int fz_phase_times(fz_system* s, int nsteps, double* ms) {
   FZ_GUARD_BEGIN
   if (!s || nsteps <= 0 || !ms) return set_error(FZ_ERR_ARG, "invalid argument");

@@ -231,15 +231,36 @@ static int process_topology(const fz_topology* t, HostTopo& h) {
 System::~System() { free_device(); }
 
 void System::free_device() {
-  if (!allocs.empty()) {
+  if (!allocs.empty() || !nb_allocs.empty()) {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
-    for (void* p : allocs) (void)hipFree(p);
+    for (auto& a : allocs) (void)hipFree(a.first);
+    for (auto& a : nb_allocs) (void)hipFree(a.first);
     (void)hipSetDevice(cur);
   }
   allocs.clear();
+  nb_allocs.clear();
   bytes = 0;
+}
+
+void System::free_neighbor() {
+  for (auto& a : nb_allocs) {
+    (void)hipFree(a.first);
+    bytes -= a.second;
+  }
+  nb_allocs.clear();
+}
+
+void System::dfree(void* p) {
+  if (!p) return;
+  for (size_t k = 0; k < allocs.size(); k++)
+    if (allocs[k].first == p) {
+      (void)hipFree(p);
+      bytes -= allocs[k].second;
+      allocs.erase(allocs.begin() + (long)k);
+      return;
+    }
 }
 
 template <class T>
@@ -249,7 +270,7 @@ int System::dalloc(T** p, size_t count, bool zero) {
   void* q = nullptr;
   hipError_t e = hipMalloc(&q, b);
   if (e != hipSuccess) return set_error(FZ_ERR_NOMEM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
-  allocs.push_back(q);
+  (alloc_nb ? nb_allocs : allocs).emplace_back(q, b);
   bytes += b;
   if (zero) FZ_HIP(hipMemset(q, 0, b));
   *p = (T*)q;
@@ -440,46 +461,76 @@ int System::alloc_device() {
   FZ_TRY(dalloc(&d.mstep, 2));
   FZ_TRY(dalloc(&d.mH, (size_t)3 * n));
   FZ_TRY(dalloc(&d.mF, (size_t)3 * n));
-  FZ_TRY(dalloc(&snap, (size_t)9 * n));
+  FZ_TRY(dalloc(&snap, (size_t)9 * n + (F_NSLOT * sizeof(int) + 7) / 8));
   return set_column_grid(par.column_grid[0], par.column_grid[1], 1.0);
 }
 
-// Column grid (binning) and the neighbour structures sized from it.  Columns
+// Column grid (binning) and the neighbour capacities sized from it.  Columns
 // are about f * cbrt(8 / rho) wide, so that a cluster of 8 atoms is roughly
 // cubic at f = 1; (ncx, ncy) > 0 override.  Replaces the compile-time
 // CELL_ATOMS / MAXNB presets of the reference (src/configure:89-137).
-int System::set_column_grid(int ncx, int ncy, double f) {
-  SysDev& d = dv;
-  const int n = topo.n;
+NbShape nb_shape(const fz_params& par, int n, int ncx, int ncy, double f) {
+  NbShape s;
   const double V = par.box[0] * par.box[1] * par.box[2];
   const double rho = n / V;
   const double w = f * std::cbrt(CLUSTER / rho);
   if (ncx <= 0) ncx = std::max(1, (int)std::floor(par.box[0] / w));
   if (ncy <= 0) ncy = std::max(1, (int)std::floor(par.box[1] / w));
   while ((double)n / (ncx * ncy) > 0.5 * MAX_COLUMN_ATOMS) { ncx++; ncy++; }
-  if (nb_alloc0 > 0) {  // release the previous neighbour block
-    for (size_t k = nb_alloc0; k < allocs.size(); k++) (void)hipFree(allocs[k]);
-    allocs.resize(nb_alloc0);
-  }
-  d.ncx = ncx; d.ncy = ncy; d.ncol = ncx * ncy;
-  d.kx_range = (int)std::ceil(par.pairlist / (par.box[0] / ncx)) + 1;
-  d.ky_range = (int)std::ceil(par.pairlist / (par.box[1] / ncy)) + 1;
-  d.colw_x = (unsigned long long)(kTwo64 / ncx);
-  d.colw_y = (unsigned long long)(kTwo64 / ncy);
-  d.nzb = std::min(256, std::max(4, (int)std::ceil(par.box[2] / 2.0)));  // ~2 A z bins
-  const int clusters_cap = n / CLUSTER + d.ncol + 64;
+  s.ncx = ncx; s.ncy = ncy; s.ncol = ncx * ncy;
+  s.nzb = std::min(256, std::max(4, (int)std::ceil(par.box[2] / 2.0)));  // ~2 A z bins
+  s.clusters_cap = n / CLUSTER + s.ncol + 64;
   const double cw = std::sqrt(par.box[0] / ncx * par.box[1] / ncy);
   const double est = 4.0 / 3.0 * kPi * std::pow(par.pairlist + cw + 1.0, 3) * rho / CLUSTER * 0.5 + 2.0;
-  long long per = std::min<long long>(clusters_cap, (long long)std::ceil(1.6 * est));
-  FZ_TRY(alloc_neighbor((long long)clusters_cap * per + 4096, clusters_cap));
+  const long long per = std::min<long long>(s.clusters_cap, (long long)std::ceil(1.6 * est));
+  s.pairs_cap = (long long)s.clusters_cap * per + 4096;
+  return s;
+}
+
+static size_t rnd256(size_t b) { return (std::max<size_t>(1, b) + 255) & ~(size_t)255; }
+
+// search waves per I cluster the search buffers are sized for (small systems
+// search each I cluster with several waves: more waves in flight)
+static int search_parts_of(int clusters_cap) { return clusters_cap < 8192 ? 4 : clusters_cap < 16384 ? 2 : 1; }
+
+// Device bytes of one neighbour block (the allocations of alloc_neighbor_block).
+size_t nb_bytes(int ncol, int nzb, int mixed, long long pairs_cap, int clusters_cap) {
+  const size_t ns = (size_t)clusters_cap * CLUSTER, cc = clusters_cap;
+  const size_t parts = search_parts_of(clusters_cap);
+  size_t b = 2 * rnd256(4 * (size_t)ncol) + rnd256(4 * ((size_t)ncol + 1)) + 2 * rnd256(4 * (size_t)ncol * (nzb + 1));
+  b += rnd256(4 * ns) + rnd256(ns * (mixed ? 16 : 32)) + rnd256(4 * ns) + rnd256(8 * ns) + rnd256(24 * ns) +
+       rnd256(4 * ns) + rnd256(4 * cc) + rnd256(48 * cc) + rnd256(24 * ns);
+  b += rnd256(4 * (cc * parts + 1)) + rnd256(4 * cc * parts * SEARCH_TMP_CAP) + rnd256(8 * cc * parts * SEARCH_TMP_CAP);
+  b += rnd256(4 * ((cc + 63) / 64));
+  b += 2 * rnd256(4 * (size_t)pairs_cap) + rnd256(8 * (size_t)pairs_cap);
+  return b;
+}
+
+int System::set_column_grid(int ncx, int ncy, double f) {
+  SysDev& d = dv;
+  const NbShape sh = nb_shape(par, topo.n, ncx, ncy, f);
+  free_neighbor();  // the previous neighbour block, if any
+  d.ncx = sh.ncx; d.ncy = sh.ncy; d.ncol = sh.ncol;
+  d.kx_range = (int)std::ceil(par.pairlist / (par.box[0] / sh.ncx)) + 1;
+  d.ky_range = (int)std::ceil(par.pairlist / (par.box[1] / sh.ncy)) + 1;
+  d.colw_x = (unsigned long long)(kTwo64 / sh.ncx);
+  d.colw_y = (unsigned long long)(kTwo64 / sh.ncy);
+  d.nzb = sh.nzb;
+  FZ_TRY(alloc_neighbor(sh.pairs_cap, sh.clusters_cap));
   gen++;
   need_rebuild = true;
   return FZ_OK;
 }
 
 int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
+  alloc_nb = true;
+  const int rc = alloc_neighbor_block(pairs_cap, clusters_cap);
+  alloc_nb = false;
+  return rc;
+}
+
+int System::alloc_neighbor_block(long long pairs_cap, int clusters_cap) {
   SysDev& d = dv;
-  nb_alloc0 = allocs.size();
   FZ_TRY(dalloc(&d.col_count, d.ncol)); FZ_TRY(dalloc(&d.col_fill, d.ncol)); FZ_TRY(dalloc(&d.col_clu0, d.ncol + 1));
   FZ_TRY(dalloc(&d.zt_hi, (size_t)d.ncol * (d.nzb + 1)));
   FZ_TRY(dalloc(&d.zt_lo, (size_t)d.ncol * (d.nzb + 1)));
@@ -496,8 +547,7 @@ int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   FZ_TRY(dalloc(&d.clu_col, clusters_cap));
   FZ_TRY(dalloc(&d.bb, (size_t)6 * clusters_cap));
   FZ_TRY(dalloc(&d.su, (size_t)3 * ns));
-  // small systems search each I cluster with several waves (more waves in flight)
-  d.search_parts = clusters_cap < 8192 ? 4 : clusters_cap < 16384 ? 2 : 1;
+  d.search_parts = search_parts_of(clusters_cap);
   FZ_TRY(dalloc(&d.search_cnt, (size_t)clusters_cap * d.search_parts + 1));
   FZ_TRY(dalloc(&d.search_tmp, (size_t)clusters_cap * d.search_parts * SEARCH_TMP_CAP, false));
   FZ_TRY(dalloc(&d.search_mtmp, (size_t)clusters_cap * d.search_parts * SEARCH_TMP_CAP, false));
@@ -506,6 +556,40 @@ int System::alloc_neighbor(long long pairs_cap, int clusters_cap) {
   FZ_TRY(dalloc(&d.pl_j, pairs_cap, false));
   FZ_TRY(dalloc(&d.pl_mask, pairs_cap, false));
   return FZ_OK;
+}
+
+// Upper bound of a system's device bytes: the allocations of alloc_device
+// (exclusion and constraint counts bounded by the term counts) plus the
+// neighbour block after one growth of both capacities (the block is replaced,
+// not added to, when it grows).
+int64_t footprint_bytes(const fz_topology* t, const fz_params* p, int precision) {
+  const int mixed = precision == FZ_MIXED ? 1 : 0;
+  const size_t n = (size_t)t->n_atoms, T2 = (size_t)t->n_types * t->n_types;
+  const size_t nb = t->n_bonds, na = t->n_angles, nd = t->n_dihedrals, ni = t->n_impropers, nc = t->n_constraints;
+  const size_t nexcl = 2 * (nb + na + nd) + 1;
+  size_t b = 0;
+  b += 12 * rnd256(8 * n);                                   // u, build and SHAKE positions, velocities
+  b += 3 * rnd256(8 * n) + rnd256(4 * n) + rnd256(4);         // q, minv, mdyn, type, tick
+  b += 3 * rnd256(4 * n) + 2 * rnd256(16);                   // com groups, atom_slot, atom_col, counters
+  b += rnd256(8 * n) + rnd256(4 * (n + 1)) + rnd256(4 * nexcl);
+  b += rnd256(8 * nb) + rnd256(16 * nb) + rnd256(16 * na) + rnd256(32 * na);
+  b += rnd256(16 * nd) + rnd256(16 * nd) + rnd256(4 * nd) + rnd256(16 * ni) + rnd256(8 * ni);
+  b += rnd256(8 * (nb + na)) + rnd256(8 * nd);               // Ewald-corrected and 1-4 pairs
+  b += rnd256(8 * T2) + 2 * rnd256(16 * T2) + rnd256(8 * (size_t)t->n_types);
+  b += 3 * rnd256(32 * (nc + 1)) + rnd256(4 * n) + rnd256(8 * n);  // clusters, free atoms, brick atoms
+  const size_t K3 = (size_t)p->pme_grid[0] * p->pme_grid[1] * p->pme_grid[2];
+  const size_t KH = (size_t)(p->pme_grid[0] / 2 + 1) * p->pme_grid[1] * p->pme_grid[2];
+  const size_t KS = (size_t)p->pme_grid[0] + p->pme_grid[1] + p->pme_grid[2];
+  size_t nbr = 1;
+  for (int k = 0; k < 3; k++) nbr *= (size_t)(p->pme_grid[k] + 7) / 8;
+  b += rnd256(8 * K3) + rnd256(KH * (mixed ? 8 : 16)) + rnd256(K3 * (mixed ? 4 : 8));
+  b += rnd256(4 * nbr) + rnd256(4 * (nbr + 1)) + rnd256(n * 16 * (mixed ? 4 : 8)) + rnd256(nbr * 1331 * 8);
+  b += 3 * rnd256(8 * KS) + rnd256(KS * (mixed ? 8 : 16));
+  b += rnd256(8 * A_NSLOT) + rnd256(4 * F_NSLOT) + rnd256(16) + 2 * rnd256(24 * n);
+  b += rnd256(72 * n + 16);                                  // snapshot
+  const NbShape sh = nb_shape(*p, t->n_atoms, p->column_grid[0], p->column_grid[1], 1.0);
+  b += nb_bytes(sh.ncol, sh.nzb, mixed, (long long)(sh.pairs_cap * 1.6) + 4096, (int)(sh.clusters_cap * 1.3) + 64);
+  return (int64_t)b;
 }
 
 // inverse masses of the integrator: 0 for fixed atoms
@@ -530,12 +614,14 @@ int System::set_fixed(const uint8_t* fixed) {
   FZ_TRY(build_ccl(topo));
   SysDev& d = dv;
   d.n_ccl = (int)topo.ccl_atoms.size();
-  if (d.n_ccl > 0) {
-    // new (smaller or equal) arrays; the previous ones are released with the system
-    FZ_TRY(dalloc(&d.ccl_atoms, topo.ccl_atoms.size())); FZ_TRY(up(d.ccl_atoms, topo.ccl_atoms));
-    FZ_TRY(dalloc(&d.ccl_cons, topo.ccl_cons.size())); FZ_TRY(up(d.ccl_cons, topo.ccl_cons));
-    FZ_TRY(dalloc(&d.ccl_d2, topo.ccl_d2.size())); FZ_TRY(up(d.ccl_d2, topo.ccl_d2));
-  }
+  // the cluster tables are replaced (no kernel is in flight on this system:
+  // calls on one handle do not overlap and every call ends synchronised)
+  FZ_HIP(hipDeviceSynchronize());
+  dfree(d.ccl_atoms); dfree(d.ccl_cons); dfree(d.ccl_d2); dfree(d.free_atoms);
+  d.ccl_atoms = nullptr; d.ccl_cons = nullptr; d.ccl_d2 = nullptr; d.free_atoms = nullptr;
+  FZ_TRY(dalloc(&d.ccl_atoms, topo.ccl_atoms.size())); FZ_TRY(up(d.ccl_atoms, topo.ccl_atoms));
+  FZ_TRY(dalloc(&d.ccl_cons, topo.ccl_cons.size())); FZ_TRY(up(d.ccl_cons, topo.ccl_cons));
+  FZ_TRY(dalloc(&d.ccl_d2, topo.ccl_d2.size())); FZ_TRY(up(d.ccl_d2, topo.ccl_d2));
   d.n_free = (int)topo.free_atoms.size();
   FZ_TRY(dalloc(&d.free_atoms, topo.free_atoms.size())); FZ_TRY(up(d.free_atoms, topo.free_atoms));
   FZ_TRY(upload_mdyn());
@@ -889,28 +975,103 @@ int Engine::sync() {
   return FZ_OK;
 }
 
-int Engine::grow_and_reset(int si, int bits) {
+// Neighbour arrays of member si at new capacities; the list they held is gone
+// (the member rebuilds at its next evaluation).
+int Engine::realloc_neighbor(int si, long long pairs, int clus) {
   System* s = sys_[si];
   SysDev& d = s->dv;
-  long long pairs = d.n_pairs_cap;
-  int clus = d.n_clusters_cap;
-  if (bits & OV_PAIRS) pairs = (long long)(pairs * 1.6) + 4096;
-  if (bits & OV_SLOTS) clus = (int)(clus * 1.3) + 64;
-  if (bits & (OV_COLUMN | OV_SEARCH))
-    return set_error(FZ_ERR_CAPACITY, "neighbour search capacity exceeded (column occupancy or more than %d J clusters "
-                     "in a search part); use a finer column grid", SEARCH_TMP_CAP);
-  // the neighbour arrays are the last allocations of the system: free them
-  const size_t keep = s->nb_alloc0;
-  for (size_t k = keep; k < s->allocs.size(); k++) (void)hipFree(s->allocs[k]);
-  s->allocs.resize(keep);
+  FZ_HIP(hipStreamSynchronize(L_.stream));
+  s->free_neighbor();
   FZ_TRY(s->alloc_neighbor(pairs, clus));
   s->gen++;
-  FZ_HIP(hipMemset(d.flags, 0, F_NSLOT * sizeof(int)));
+  FZ_HIP(hipMemset(d.flags + F_OVERFLOW, 0, sizeof(int)));
   FZ_HIP(hipMemset(d.col_count, 0, d.ncol * sizeof(int)));
   FZ_HIP(hipMemset(d.col_fill, 0, d.ncol * sizeof(int)));
   FZ_HIP(hipMemset(d.tick, 0, sizeof(unsigned)));
   s->need_rebuild = true;
   return FZ_OK;
+}
+
+int Engine::grow_and_reset(int si, int bits) {
+  const SysDev& d = sys_[si]->dv;
+  long long pairs = d.n_pairs_cap;
+  int clus = d.n_clusters_cap;
+  if (bits & (OV_COLUMN | OV_SEARCH))
+    return set_error(FZ_ERR_CAPACITY, "neighbour search capacity exceeded (column occupancy or more than %d J clusters "
+                     "in a search part); use a finer column grid", SEARCH_TMP_CAP);
+  if (bits & OV_PAIRS) pairs = (long long)(pairs * 1.6) + 4096;
+  if (bits & OV_SLOTS) clus = (int)(clus * 1.3) + 64;
+  return realloc_neighbor(si, pairs, clus);
+}
+
+static unsigned long long* snap_part(System* s, int k) { return s->snap + (size_t)k * s->topo.n; }
+
+int Engine::snapshot() {
+  for (auto* s : sys_) {
+    const size_t nb = (size_t)s->topo.n * 8;
+    void* src[9] = {s->dv.ux, s->dv.uy, s->dv.uz, s->dv.vx, s->dv.vy, s->dv.vz, s->dv.rx, s->dv.ry, s->dv.rz};
+    for (int k = 0; k < 9; k++)
+      FZ_HIP(hipMemcpyAsync(snap_part(s, k), src[k], nb, hipMemcpyDeviceToDevice, L_.stream));
+    FZ_HIP(hipMemcpyAsync(snap_part(s, 9), s->dv.flags, F_NSLOT * sizeof(int), hipMemcpyDeviceToDevice, L_.stream));
+    s->snap_step = s->step;
+    s->snap_min_step = s->min_step;
+  }
+  return FZ_OK;
+}
+
+// Every member returns to its snapshot.  Its list is rebuilt at the snapshot's
+// build positions, which reproduces the list it held (the build depends only
+// on those positions and the column grid, not on the capacities), then the
+// positions go back to the snapshot's and the forces are evaluated there (no
+// rebuild: no atom was past half the skin from the build positions, or the
+// list would have been rebuilt then).  Velocities, flags (list-build count
+// included) and the host counters are the snapshot's, so a replay from here
+// repeats what the interrupted call computed.
+int Engine::restore(const std::vector<int>* grow) {
+  for (size_t k = 0; k < sys_.size(); k++) {
+    System* s = sys_[k];
+    const size_t nb = (size_t)s->topo.n * 8;
+    void* v[3] = {s->dv.vx, s->dv.vy, s->dv.vz};
+    void* u[3] = {s->dv.ux, s->dv.uy, s->dv.uz};
+    for (int c = 0; c < 3; c++) {
+      FZ_HIP(hipMemcpyAsync(v[c], snap_part(s, 3 + c), nb, hipMemcpyDeviceToDevice, L_.stream));
+      FZ_HIP(hipMemcpyAsync(u[c], snap_part(s, 6 + c), nb, hipMemcpyDeviceToDevice, L_.stream));
+    }
+    FZ_HIP(hipMemcpyAsync(s->dv.flags, snap_part(s, 9), F_NSLOT * sizeof(int), hipMemcpyDeviceToDevice, L_.stream));
+    if (grow && (*grow)[k]) FZ_TRY(grow_and_reset((int)k, (*grow)[k]));
+    s->need_rebuild = true;
+    s->step = s->snap_step;
+    s->min_step = s->snap_min_step;
+  }
+  FZ_TRY(ensure());
+  FZ_TRY(forces(true));
+  for (auto* s : sys_) {
+    const size_t nb = (size_t)s->topo.n * 8;
+    void* u[3] = {s->dv.ux, s->dv.uy, s->dv.uz};
+    for (int c = 0; c < 3; c++)
+      FZ_HIP(hipMemcpyAsync(u[c], snap_part(s, c), nb, hipMemcpyDeviceToDevice, L_.stream));
+  }
+  FZ_TRY(forces(true));
+  // the list-build count as at the snapshot (the two evaluations above rebuilt
+  // a list the trajectory already had); the other flags are clear there too
+  for (auto* s : sys_)
+    FZ_HIP(hipMemcpyAsync(s->dv.flags, snap_part(s, 9), F_NSLOT * sizeof(int), hipMemcpyDeviceToDevice, L_.stream));
+  return read_status();
+}
+
+int Engine::set_capacity(int si, long long pairs_cap, int clusters_cap) {
+  if (si < 0 || si >= (int)sys_.size()) return set_error(FZ_ERR_ARG, "no member %d", si);
+  System* s = sys_[si];
+  if (pairs_cap < 0 || clusters_cap < 0) return set_error(FZ_ERR_ARG, "capacities must not be negative");
+  if (clusters_cap > 0 && clusters_cap < s->topo.n / CLUSTER + 1)
+    return set_error(FZ_ERR_ARG, "cluster capacity %d cannot hold %d atoms", clusters_cap, s->topo.n);
+  FZ_TRY(ensure());
+  for (auto* m : sys_)
+    if (m->need_rebuild) FZ_TRY(forces(true));
+  FZ_TRY(snapshot());
+  FZ_TRY(realloc_neighbor(si, pairs_cap > 0 ? pairs_cap : s->dv.n_pairs_cap,
+                          clusters_cap > 0 ? clusters_cap : s->dv.n_clusters_cap));
+  return restore();
 }
 
 int Engine::forces(bool ev, bool lite) {
@@ -950,27 +1111,9 @@ int Engine::steps(int n) {
   FZ_TRY(ensure());
   for (auto* s : sys_)
     if (s->need_rebuild) FZ_TRY(forces(true));
-  auto copy3 = [&](unsigned long long* const* dst, unsigned long long* const* src, size_t nb) -> int {
-    for (int c = 0; c < 3; c++) FZ_HIP(hipMemcpyAsync(dst[c], src[c], nb, hipMemcpyDeviceToDevice, L_.stream));
-    return FZ_OK;
-  };
   for (int attempt = 0; attempt < 4; attempt++) {
     launch_clear_shake_flag(L_);  // the flag reports this attempt's steps only
-    // snapshot positions, velocities and the positions of the last list build
-    // for replay after a capacity overflow
-    for (auto* s : sys_) {
-      const size_t n3 = s->topo.n;
-      unsigned long long* sn[3] = {s->snap, s->snap + n3, s->snap + 2 * n3};
-      unsigned long long* sv[3] = {s->snap + 3 * n3, s->snap + 4 * n3, s->snap + 5 * n3};
-      unsigned long long* sr[3] = {s->snap + 6 * n3, s->snap + 7 * n3, s->snap + 8 * n3};
-      unsigned long long* u[3] = {s->dv.ux, s->dv.uy, s->dv.uz};
-      unsigned long long* v[3] = {(unsigned long long*)s->dv.vx, (unsigned long long*)s->dv.vy,
-                                  (unsigned long long*)s->dv.vz};
-      unsigned long long* r[3] = {s->dv.rx, s->dv.ry, s->dv.rz};
-      FZ_TRY(copy3(sn, u, n3 * 8));
-      FZ_TRY(copy3(sv, v, n3 * 8));
-      FZ_TRY(copy3(sr, r, n3 * 8));
-    }
+    FZ_TRY(snapshot());
     if (!g_plain_) FZ_TRY(capture(false, &g_plain_));
     if (!g_ev_) FZ_TRY(capture(true, &g_ev_));
     // The step graph has no slot gather: it relies on the slot copies and
@@ -981,50 +1124,35 @@ int Engine::steps(int n) {
     launch_step_first(L_);
     for (int k = 0; k < n; k++) FZ_HIP(hipGraphLaunch(k == n - 1 ? g_ev_ : g_plain_, L_.stream));
     FZ_TRY(read_status());
-    bool again = false;
-    for (auto* s : sys_) again = again || s->flags_host[F_OVERFLOW];
-    if (!again) {
-      for (size_t k = 0; k < sys_.size(); k++)
-        if (sys_[k]->flags_host[F_SHAKE_FAIL])
-          return set_error(FZ_ERR_SHAKE, "SHAKE did not converge (system %d of the set)", (int)k);
+    // classify before touching any state
+    std::vector<int> grow(sys_.size(), 0);
+    int overflow = 0, fatal = -1, shake = -1;
+    for (size_t k = 0; k < sys_.size(); k++) {
+      const int ov = sys_[k]->flags_host[F_OVERFLOW];
+      grow[k] = ov;
+      overflow |= ov;
+      if ((ov & (OV_COLUMN | OV_SEARCH)) && fatal < 0) fatal = (int)k;
+      if (sys_[k]->flags_host[F_SHAKE_FAIL] && shake < 0) shake = (int)k;
+    }
+    if (!overflow && shake < 0) {
       for (auto* s : sys_) s->step += n;
       return FZ_OK;
     }
-    // Every member returns to the snapshot.  Members that overflowed grow and
-    // rebuild at the snapshot positions; the others rebuild their list at the
-    // snapshot's build positions, which restores the exact list they held, so
-    // their replay repeats what they would have computed without the overflow.
-    std::vector<int> grown(sys_.size(), 0);
-    for (size_t k = 0; k < sys_.size(); k++) {
-      System* s = sys_[k];
-      const size_t n3 = s->topo.n;
-      unsigned long long* sn[3] = {s->snap, s->snap + n3, s->snap + 2 * n3};
-      unsigned long long* sv[3] = {s->snap + 3 * n3, s->snap + 4 * n3, s->snap + 5 * n3};
-      unsigned long long* sr[3] = {s->snap + 6 * n3, s->snap + 7 * n3, s->snap + 8 * n3};
-      unsigned long long* u[3] = {s->dv.ux, s->dv.uy, s->dv.uz};
-      unsigned long long* v[3] = {(unsigned long long*)s->dv.vx, (unsigned long long*)s->dv.vy,
-                                  (unsigned long long*)s->dv.vz};
-      FZ_TRY(copy3(v, sv, n3 * 8));
-      FZ_TRY(copy3(u, s->flags_host[F_OVERFLOW] ? sn : sr, n3 * 8));
-      FZ_HIP(hipStreamSynchronize(L_.stream));
-      if (s->flags_host[F_OVERFLOW]) {
-        FZ_TRY(grow_and_reset((int)k, s->flags_host[F_OVERFLOW]));
-        grown[k] = 1;
-      } else {
-        s->need_rebuild = true;
-      }
+    if (fatal >= 0 || (shake >= 0 && !overflow)) {
+      // an error the replay cannot cure: every member back to the state before
+      // the call, then report it
+      FZ_TRY(restore());
+      if (fatal >= 0)
+        return set_error(FZ_ERR_CAPACITY, "neighbour search capacity exceeded in system %d of the set (column "
+                         "occupancy or more than %d J clusters in a search part); use a finer column grid",
+                         fatal, SEARCH_TMP_CAP);
+      return set_error(FZ_ERR_SHAKE, "SHAKE did not converge (system %d of the set)", shake);
     }
-    FZ_TRY(prepare());
-    FZ_TRY(forces(true));  // builds every list (grown members at their positions)
-    for (size_t k = 0; k < sys_.size(); k++) {
-      if (grown[k]) continue;
-      System* s = sys_[k];
-      const size_t n3 = s->topo.n;
-      unsigned long long* sn[3] = {s->snap, s->snap + n3, s->snap + 2 * n3};
-      unsigned long long* u[3] = {s->dv.ux, s->dv.uy, s->dv.uz};
-      FZ_TRY(copy3(u, sn, n3 * 8));
-    }
-    FZ_TRY(forces(true));  // forces at the snapshot positions with the restored lists
+    // A capacity overflow (a SHAKE failure after it may be its consequence:
+    // the steps ran without the pairs that did not fit): every member returns
+    // to the snapshot, the overflowed ones with larger capacities, and the
+    // steps are replayed.
+    FZ_TRY(restore(&grow));
   }
   return set_error(FZ_ERR_CAPACITY, "neighbour capacity could not be grown");
 }
@@ -1038,8 +1166,7 @@ void launch_zero_vel(const Launch& L);
 // moves by the current step; an energy decrease is accepted and grows the step
 // by 1.2, an increase is undone and halves it (the reference reverts and
 // switches method on an increase, src/dynamics.cu:597-618).
-int Engine::minimize(int nsteps, double max_step) {
-  FZ_TRY(ensure());
+int Engine::minimize_sd(int nsteps, double max_step) {
   const int ns = (int)sys_.size();
   std::vector<double> step(ns, max_step), e_old(ns);
   for (int s = 0; s < ns; s++)
@@ -1084,10 +1211,7 @@ void launch_min_ref_move(const Launch& L);
 // energy above the last accepted one (the first move is always accepted) undoes
 // the move and demotes conjugate gradient -> steepest descent -> stop; after
 // every 100 accepted moves the active factor grows by rate * 100.
-int Engine::minimize_ref(int nsteps, const fz_min_params& p) {
-  if (p.method == 0) return minimize(nsteps, p.max_step);
-  if (p.method != 1 && p.method != 2) return set_error(FZ_ERR_ARG, "unknown minimisation method %d", p.method);
-  FZ_TRY(ensure());
+int Engine::minimize_cgsd(int nsteps, const fz_min_params& p) {
   const int ns = (int)sys_.size();
   std::vector<int> method(ns, p.method), done(ns, 0), have(ns, 0), reject(ns, 0);
   std::vector<double> cg(ns, p.cgfac), sd(ns, p.sdfac), e_prev(ns, 0.0);
@@ -1142,10 +1266,44 @@ int Engine::minimize_ref(int nsteps, const fz_min_params& p) {
   return forces(true);
 }
 
+// A failed minimisation leaves every member as it was before the call (a
+// batch failure is then retried member by member from the same state).
+int Engine::minimize(int nsteps, double max_step) {
+  FZ_TRY(ensure());
+  for (auto* s : sys_)
+    if (s->need_rebuild) FZ_TRY(forces(true));
+  FZ_TRY(snapshot());
+  const int rc = minimize_sd(nsteps, max_step);
+  if (rc != FZ_OK) {
+    const std::string msg = last_error();
+    (void)restore();
+    return set_error(rc, "%s", msg.c_str());
+  }
+  return FZ_OK;
+}
+
+int Engine::minimize_ref(int nsteps, const fz_min_params& p) {
+  if (p.method == 0) return minimize(nsteps, p.max_step);
+  if (p.method != 1 && p.method != 2) return set_error(FZ_ERR_ARG, "unknown minimisation method %d", p.method);
+  FZ_TRY(ensure());
+  for (auto* s : sys_)
+    if (s->need_rebuild) FZ_TRY(forces(true));
+  FZ_TRY(snapshot());
+  const int rc = minimize_cgsd(nsteps, p);
+  if (rc != FZ_OK) {
+    const std::string msg = last_error();
+    (void)restore();
+    return set_error(rc, "%s", msg.c_str());
+  }
+  return FZ_OK;
+}
+
 int Engine::phase_times(int nsteps, double* out) {
   FZ_TRY(ensure());
   for (auto* s : sys_)
     if (s->need_rebuild) FZ_TRY(forces(true));
+  launch_clear_shake_flag(L_);
+  FZ_TRY(snapshot());
   const int NP = 9;
   hipEvent_t ev[NP + 1];
   for (int k = 0; k <= NP; k++) FZ_HIP(hipEventCreate(&ev[k]));
@@ -1182,6 +1340,15 @@ int Engine::phase_times(int nsteps, double* out) {
   FZ_TRY(rc);
   for (int k = 0; k < NP; k++) out[k] /= std::max(1, nsteps);
   FZ_TRY(read_status());
+  for (size_t k = 0; k < sys_.size(); k++) {
+    const int ov = sys_[k]->flags_host[F_OVERFLOW], sf = sys_[k]->flags_host[F_SHAKE_FAIL];
+    if (ov || sf) {
+      // eager steps do not replay: the members go back to where they were
+      FZ_TRY(restore());
+      return set_error(ov ? FZ_ERR_CAPACITY : FZ_ERR_SHAKE, "%s in system %d during the phase timing",
+                       ov ? "neighbour capacity overflow" : "SHAKE failure", (int)k);
+    }
+  }
   for (auto* s : sys_) s->step += nsteps;
   return FZ_OK;
 }
@@ -1208,55 +1375,75 @@ int Engine::time_steps(int nsteps, double* ms) {
 // compile time (CELL_BLOCKSIZE / CELL_ATOMS presets, src/configure:89-137):
 // column width (atoms per column and cluster compactness), FFT lines per
 // workgroup, persistent pair-kernel grid.  Each candidate is timed on
-// graph-replayed steps; the systems' positions, velocities and step counters
-// are restored afterwards.
+// graph-replayed steps from the same state; a candidate whose steps fail (a
+// capacity or SHAKE error) counts as infinitely slow.  Members with an explicit
+// column grid keep it.  On every exit the members' positions, velocities,
+// step counters and minimiser steps are those they had on entry.
 int Engine::autotune(int nsteps, double* out) {
   FZ_TRY(ensure());
   const int ns = (int)sys_.size();
   std::vector<std::vector<uint64_t>> u0(ns);
   std::vector<std::vector<double>> v0(ns);
   std::vector<int64_t> st0(ns);
+  std::vector<double> ms0(ns);
   for (int s = 0; s < ns; s++) {
     u0[s].resize((size_t)3 * sys_[s]->topo.n);
     v0[s].resize((size_t)3 * sys_[s]->topo.n);
     FZ_TRY(sys_[s]->download_state(nullptr, v0[s].data(), u0[s].data()));
     st0[s] = sys_[s]->step;
+    ms0[s] = sys_[s]->min_step;
   }
-  auto restore = [&]() -> int {
+  auto restore_state = [&]() -> int {
     for (int s = 0; s < ns; s++) {
       FZ_TRY(sys_[s]->upload_fixed(u0[s].data()));
       FZ_TRY(sys_[s]->upload_velocities(v0[s].data()));
       sys_[s]->step = st0[s];
+      sys_[s]->min_step = ms0[s];
     }
     return FZ_OK;
   };
-  auto trial = [&](double f, int nl, int grid, double* ms) -> int {
-    for (auto* s : sys_) FZ_TRY(s->set_column_grid(0, 0, f));
+  auto configure = [&](double f, int nl, int grid) -> int {
+    for (auto* s : sys_) FZ_TRY(s->set_column_grid(s->par.column_grid[0], s->par.column_grid[1], f));
     tuned_nl_ = nl;
     tuned_grid_ = grid;
     FZ_TRY(prepare());
-    FZ_TRY(restore());
-    return time_steps(nsteps, ms);
+    return restore_state();
   };
+  auto trial = [&](double f, int nl, int grid, double* ms) -> int {
+    FZ_TRY(configure(f, nl, grid));
+    const int rc = time_steps(nsteps, ms);
+    if (rc == FZ_ERR_SHAKE || rc == FZ_ERR_CAPACITY) *ms = 1e30;
+    else if (rc != FZ_OK) return rc;
+    return FZ_OK;
+  };
+  bool explicit_grid = true;
+  for (auto* s : sys_) explicit_grid = explicit_grid && s->par.column_grid[0] > 0 && s->par.column_grid[1] > 0;
   double best_f = 1.0, best_ms = 1e30, ms = 0;
   int best_nl = 16, best_grid = 2048;
-  for (double f : {0.8, 0.9, 1.0, 1.15, 1.3}) {
-    FZ_TRY(trial(f, best_nl, best_grid, &ms));
+  int rc = FZ_OK;
+  std::vector<double> widths = {0.8, 0.9, 1.0, 1.15, 1.3};
+  if (explicit_grid) widths = {1.0};
+  for (double f : widths) {
+    if ((rc = trial(f, best_nl, best_grid, &ms)) != FZ_OK) break;
     if (ms < best_ms) { best_ms = ms; best_f = f; }
   }
   for (int nl : {4, 8, 16}) {
-    FZ_TRY(trial(best_f, nl, best_grid, &ms));
+    if (rc != FZ_OK || (rc = trial(best_f, nl, best_grid, &ms)) != FZ_OK) break;
     if (ms < best_ms) { best_ms = ms; best_nl = nl; }
   }
   for (int grid : {512, 1024, 4096, 8192}) {
-    FZ_TRY(trial(best_f, best_nl, grid, &ms));
+    if (rc != FZ_OK || (rc = trial(best_f, best_nl, grid, &ms)) != FZ_OK) break;
     if (ms < best_ms) { best_ms = ms; best_grid = grid; }
   }
-  for (auto* s : sys_) FZ_TRY(s->set_column_grid(0, 0, best_f));
-  tuned_nl_ = best_nl;
-  tuned_grid_ = best_grid;
-  FZ_TRY(prepare());
-  FZ_TRY(restore());
+  if (rc != FZ_OK) {
+    // an unexpected error: back to the defaults and the entry state
+    const std::string msg = last_error();
+    tuned_nl_ = tuned_grid_ = 0;
+    (void)configure(1.0, 0, 0);
+    (void)forces(true);
+    return set_error(rc, "%s", msg.c_str());
+  }
+  FZ_TRY(configure(best_f, best_nl, best_grid));
   FZ_TRY(forces(true));
   out[0] = best_f; out[1] = best_nl; out[2] = best_grid; out[3] = best_ms;
   return FZ_OK;

@@ -58,16 +58,22 @@ struct System {
   fz_params par{};
   HostTopo topo;
   SysDev dv{};
-  std::vector<void*> allocs;
-  size_t nb_alloc0 = 0;  // index in allocs of the first neighbour-list buffer
-  size_t bytes = 0;
+  // device allocations with their sizes: the neighbour-list block (replaced
+  // whenever the column grid or the capacities change) apart from the rest
+  std::vector<std::pair<void*, size_t>> allocs, nb_allocs;
+  bool alloc_nb = false;  // dalloc adds to nb_allocs (inside alloc_neighbor)
+  size_t bytes = 0;       // sum of the live allocations
   int64_t step = 0;
   int gen = 0;  // bumped whenever device buffers are reallocated
   double self_energy = 0.0;
   double min_step = 0.0;  // adaptive minimiser step carried between calls (0 = start at the maximum)
   bool need_rebuild = true;
-  // snapshot buffers for capacity-overflow recovery
-  unsigned long long* snap = nullptr;  // 3n u + 3n v (as bits)
+  // Snapshot for restoring a call that failed or overflowed: 3n positions,
+  // 3n velocities (as bits), 3n positions of the last list build, then the
+  // status flags (F_NSLOT ints); step and minimiser step on the host.
+  unsigned long long* snap = nullptr;
+  int64_t snap_step = 0;
+  double snap_min_step = 0.0;
   // host mirror of the last read accumulators
   unsigned long long acc_host[A_NSLOT] = {0};
   int flags_host[F_NSLOT] = {0};
@@ -75,8 +81,11 @@ struct System {
   int init(const fz_topology* t, const fz_params* p, int precision, int device);
   int alloc_device();
   int alloc_neighbor(long long pairs_cap, int clusters_cap);
+  int alloc_neighbor_block(long long pairs_cap, int clusters_cap);
   int set_column_grid(int ncx, int ncy, double width_factor);
   void free_device();
+  void free_neighbor();
+  void dfree(void* p);  // one allocation of allocs (no-op for null)
   template <class T> int dalloc(T** p, size_t count, bool zero = true);
   int upload_positions(const double* xyz);
   int upload_fixed(const uint64_t* u);
@@ -97,10 +106,18 @@ class Engine {
   // stand-alone evaluation at current positions; lite: pair energies summed in FP32
   // (the minimisers' accept/reject tests), otherwise in FP64
   int forces(bool ev, bool lite = false);
-  int steps(int n);                    // n MD steps (energies on the last)
+  // n MD steps (energies on the last).  On any error every member is back in
+  // the state it had before the call (positions, velocities, list, counters).
+  int steps(int n);
   int maxwell(double T, uint64_t seed, uint64_t counter);
-  int minimize(int nsteps, double max_step);  // adaptive steepest descent, velocities zeroed
-  int minimize_ref(int nsteps, const fz_min_params& p);  // the reference's CG -> SD schedule
+  // adaptive steepest descent, velocities zeroed; restores the members on error
+  int minimize(int nsteps, double max_step);
+  // the reference's CG -> SD schedule; restores the members on error
+  int minimize_ref(int nsteps, const fz_min_params& p);
+  // Neighbour capacities of member s (pair-list entries, clusters; 0 keeps
+  // one): its list is rebuilt at the positions of its last build, so the state
+  // and the list it holds are unchanged.  Overflow grows them again.
+  int set_capacity(int s, long long pairs_cap, int clusters_cap);
   // nsteps eager (non-graph) steps with HIP events between phases; mean ms per
   // phase per step into out[FZ_NPHASE] (order: see fz_phase_times in fenzi.h)
   int phase_times(int nsteps, double* out);
@@ -141,9 +158,30 @@ class Engine {
   void record_step(bool ev);
   void record_forces(bool ev, bool lite = false, bool check = true);
   int grow_and_reset(int s, int bits);
+  int realloc_neighbor(int s, long long pairs, int clus);
+  // device copies of every member's positions, velocities, build positions and
+  // flags (stream-ordered), host counters
+  int snapshot();
+  // every member back to its snapshot; grow[k] != 0 enlarges member k's
+  // neighbour capacity (overflow bits) first
+  int restore(const std::vector<int>* grow = nullptr);
+  int minimize_sd(int nsteps, double max_step);
+  int minimize_cgsd(int nsteps, const fz_min_params& p);
   void drop_graphs();
   int upload_desc();
 };
+
+// Column grid and initial neighbour capacities for n atoms under par
+// (explicit ncx, ncy > 0 kept); width factor f scales the column width.
+struct NbShape {
+  int ncx, ncy, ncol, nzb, clusters_cap;
+  long long pairs_cap;
+};
+NbShape nb_shape(const fz_params& par, int n, int ncx, int ncy, double f);
+// device bytes of one neighbour block
+size_t nb_bytes(int ncol, int nzb, int mixed, long long pairs_cap, int clusters_cap);
+// upper bound of one system's device bytes (the serving admission figure)
+int64_t footprint_bytes(const fz_topology* t, const fz_params* p, int precision);
 
 int pme_convolve(int device, const int32_t K[3], const double box[3], double beta, int mixed, int mode,
                  const double* q, double* phi, double* energy);

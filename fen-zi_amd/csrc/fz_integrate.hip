@@ -237,8 +237,10 @@ __global__ __launch_bounds__(128) void k_shake(const SysDev* __restrict__ S, int
 
 // RATTLE on one cluster's velocities v (in place), r the positions relative to
 // the cluster's first atom, mi the inverse masses (0 for padding); v is left as
-// it is if the constraint system is singular
-__device__ __forceinline__ void rattle_core(const CCl& k, const V3 (&r)[4], V3 (&v)[4], const double (&mi)[4]) {
+// it is if the constraint system is singular.  lam receives the multipliers b_j
+// of v_m += mi_m sum_j b_j B_jm r_j (zero when singular).
+__device__ __forceinline__ void rattle_core(const CCl& k, const V3 (&r)[4], V3 (&v)[4], const double (&mi)[4],
+                                            double (&lam)[3]) {
   V3 rc[3];
   double A[3][3], b[3], B[3][4], w[3][3];
   incidence(k, B);
@@ -252,7 +254,10 @@ __device__ __forceinline__ void rattle_core(const CCl& k, const V3 (&r)[4], V3 (
     for (int e = 0; e < 3; e++)
       A[j][e] = (j < k.nc && e < k.nc) ? v3dot(rc[j], rc[e]) * w[j][e] : (j == e ? 1.0 : 0.0);
   }
+  lam[0] = lam[1] = lam[2] = 0.0;
   if (!solve3(A, b, 3)) return;
+#pragma unroll
+  for (int j = 0; j < 3; j++) lam[j] = j < k.nc ? b[j] : 0.0;
 #pragma unroll
   for (int m = 0; m < 4; m++) {
     V3 s = v[m];
@@ -265,9 +270,13 @@ __device__ __forceinline__ void rattle_core(const CCl& k, const V3 (&r)[4], V3 (
   }
 }
 
-__device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k) {
+// RATTLE of one cluster.  wv (nullable) receives the virial of its constraint
+// forces xx yy zz xy xz yz: the projection is the half kick dt/(2 m_a) G_a of
+// G_a = (2/dt) sum_j b_j B_ja r_j, so sum_a r_a (x) G_a = (2/dt) sum_j b_j r_j (x) r_j
+// (the constraint part of the reference's PCONSTANT virial, src/shake.cu:300-306).
+__device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k, double* wv = nullptr) {
   V3 r[4], v[4];
-  double mi[4];
+  double mi[4], lam[3];
   load_rel(d, k, d.ux, d.uy, d.uz, r);
 #pragma unroll
   for (int m = 0; m < 4; m++) {
@@ -275,7 +284,19 @@ __device__ __forceinline__ void rattle_ccl(const SysDev& d, const CCl& k) {
     v[m] = V3{d.vx[a], d.vy[a], d.vz[a]};
     mi[m] = k.at[m] >= 0 ? d.mdyn[a] : 0.0;
   }
-  rattle_core(k, r, v, mi);
+  rattle_core(k, r, v, mi, lam);
+  if (wv) {
+    double B[3][4];
+    incidence(k, B);
+    const double s2 = 2.0 / d.dt;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const V3 c = bsum(B[j], r);
+      const double l = s2 * lam[j];
+      wv[0] += l * c.x * c.x; wv[1] += l * c.y * c.y; wv[2] += l * c.z * c.z;
+      wv[3] += l * c.x * c.y; wv[4] += l * c.x * c.z; wv[5] += l * c.y * c.z;
+    }
+  }
 #pragma unroll
   for (int m = 0; m < 4; m++) {
     const int a = k.at[m];
@@ -341,7 +362,8 @@ __device__ __forceinline__ bool step_first_ccl(const SysDev& d, const CCl& k) {
       r[m] = V3{fx_d(u[0][0], u[m][0], d.sc[0]), fx_d(u[0][1], u[m][1], d.sc[1]), fx_d(u[0][2], u[m][2], d.sc[2])};
       w[m] = V3{v[m][0], v[m][1], v[m][2]};
     }
-    rattle_core(k, r, w, mi);
+    double lam[3];
+    rattle_core(k, r, w, mi, lam);
 #pragma unroll
     for (int m = 0; m < 4; m++) { v[m][0] = w[m].x; v[m][1] = w[m].y; v[m][2] = w[m].z; }
   }
@@ -414,21 +436,27 @@ __global__ __launch_bounds__(128) void k_step_first(const SysDev* __restrict__ S
   if (__any(move) && (threadIdx.x & 63) == 0) d.flags[F_REBUILD] = 1;
 }
 
-// second half kick and RATTLE
+// second half kick and RATTLE; the constraint virial of the RATTLE projection
+// is added to the step's virial (the last step of a run, whose energies and
+// virial are reported)
 template <bool MIXED>
 __global__ __launch_bounds__(128) void k_step_second(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
   const int t = (blockIdx.x - block_base(pre, nsys, s_)) * 128 + threadIdx.x;
+  double wv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (t < d.n_ccl) {
     const CCl k = load_ccl(d, t);
 #pragma unroll
     for (int m = 0; m < 4; m++)
       if (k.at[m] >= 0) kick_atom<MIXED>(d, k.at[m]);
-    rattle_ccl(d, k);
+    rattle_ccl(d, k, wv);
   } else if (t < d.n_ccl + d.n_free) {
     kick_atom<MIXED>(d, d.free_atoms[t - d.n_ccl]);
   }
+  double w9[9] = {wv[0], wv[3], wv[4], wv[3], wv[1], wv[5], wv[4], wv[5], wv[2]};
+  const int slots[9] = {A_VIR + 0, A_VIR + 1, A_VIR + 2, A_VIR + 3, A_VIR + 4, A_VIR + 5, A_VIR + 6, A_VIR + 7, A_VIR + 8};
+  block_acc_fixed<9>(w9, d.acc, slots);
 }
 
 void launch_step_first(const Launch& L, bool finish) {

@@ -79,24 +79,7 @@ struct Job {
 };
 
 int64_t estimate_bytes(const fz_topology* t, const fz_params* p, int precision) {
-  const double n = t->n_atoms;
-  const double V = p->box[0] * p->box[1] * p->box[2];
-  const double rho = n / V;
-  const double w = std::cbrt(CLUSTER / rho);
-  const double ncol = std::max(1.0, std::floor(p->box[0] / w)) * std::max(1.0, std::floor(p->box[1] / w));
-  const double clusters = n / CLUSTER + ncol + 64;
-  const double est = 4.0 / 3.0 * 3.14159265358979 * std::pow(p->pairlist + w + 1.0, 3) * rho / CLUSTER * 0.5 + 2.0;
-  const double pairs = clusters * std::min(clusters, std::ceil(1.6 * est)) + 4096;
-  const double K3 = (double)p->pme_grid[0] * p->pme_grid[1] * p->pme_grid[2];
-  const double terms = (double)t->n_bonds + t->n_angles + t->n_dihedrals + t->n_impropers;
-  double b = n * 200.0;                                       // per-atom state, exclusions, snapshot
-  b += clusters * CLUSTER * (precision == FZ_MIXED ? 60.0 : 76.0);  // slots, forces, binning
-  const double parts = clusters < 8192 ? 4.0 : clusters < 16384 ? 2.0 : 1.0;  // search parts per I cluster
-  b += clusters * (parts * SEARCH_TMP_CAP * 12.0 + 80.0);     // search buffers, boxes, offsets
-  b += pairs * 16.0;                                          // pair list
-  b += K3 * (precision == FZ_MIXED ? 16.0 : 32.0);             // PME grids
-  b += terms * 40.0 + 3 * n * 8.0;                            // bonded terms, exclusion pairs
-  return (int64_t)(b * 1.05) + (1 << 20);
+  return footprint_bytes(t, p, precision);
 }
 
 }  // namespace

@@ -80,7 +80,7 @@ EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create
            "fz_server_release", "fz_server_estimate_bytes", "fz_server_destroy",
            "fz_minimize_ex", "fz_batch_minimize_ex", "fz_restart_write", "fz_restart_read", "fz_dcd_open",
            "fz_dcd_write_frame", "fz_dcd_close", "fz_set_fixed_atoms", "fz_set_com_restraint",
-           "fz_deck_com_restraint", "fz_deck_fixed_atoms"]
+           "fz_deck_com_restraint", "fz_deck_fixed_atoms", "fz_set_neighbor_capacity"]
 
 MIN_ADAPTIVE_SD, MIN_REF_CG, MIN_REF_SD = 0, 1, 2
 
@@ -114,7 +114,12 @@ _lib = None
 
 
 class FenziError(RuntimeError):
-    pass
+    """A negative status of the C ABI; .code is the FZ_ERR_* value."""
+    code = 0
+
+
+# status codes of include/fenzi.h
+ERR_ARG, ERR_HIP, ERR_CAPACITY, ERR_PARSE, ERR_NOMEM, ERR_SHAKE, ERR_STATE, ERR_BUDGET = -1, -2, -3, -4, -5, -6, -7, -8
 
 
 def lib():
@@ -139,6 +144,7 @@ def lib():
         L.fz_get_energies.argtypes = [vp, C.POINTER(Energies)]
         L.fz_minimize.argtypes = [vp, C.c_int, C.c_double]
         L.fz_system_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+        L.fz_set_neighbor_capacity.argtypes = [vp, C.c_int64, C.c_int32]
         L.fz_phase_times.argtypes = [vp, C.c_int, _dp]
         L.fz_autotune.argtypes = [vp, C.c_int, _dp]
         L.fz_batch_autotune.argtypes = [vp, C.c_int, _dp]
@@ -191,7 +197,9 @@ def lib():
 
 def _check(rc, what):
     if rc < 0:
-        raise FenziError("%s: %s (code %d)" % (what, lib().fz_last_error().decode(), rc))
+        err = FenziError("%s: %s (code %d)" % (what, lib().fz_last_error().decode(), rc))
+        err.code = int(rc)
+        raise err
     return rc
 
 
@@ -363,6 +371,10 @@ class System:
         out = np.zeros(4)
         _check(lib().fz_autotune(self.h, int(nsteps), _p(out, C.c_double)), "fz_autotune")
         return dict(column_width=out[0], fft_lines=int(out[1]), pair_grid=int(out[2]), ms_per_step=out[3])
+
+    def set_neighbor_capacity(self, pairs_cap=0, clusters_cap=0):
+        """Neighbour capacities (0 keeps one); the list held stays the same."""
+        _check(lib().fz_set_neighbor_capacity(self.h, int(pairs_cap), int(clusters_cap)), "fz_set_neighbor_capacity")
 
     def info(self):
         n, b, p = C.c_int64(), C.c_int64(), C.c_int32()

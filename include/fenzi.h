@@ -95,7 +95,9 @@ typedef struct fz_energies {
   double ewald_self;      /* -CC beta / sqrt(pi) sum q^2 */
   double potential;       /* sum of the terms above */
   double kinetic;
-  double virial[9];       /* sum r_ij (x) F_ij + reciprocal tensor (no constraint part) */
+  double virial[9];       /* sum r_ij (x) F_ij + reciprocal tensor; after fz_step also the constraint
+                             virial of the last step's RATTLE, (2/dt) sum_c lambda_c r_c (x) r_c
+                             (src/shake.cu:300-306, there halved and diagonal only) */
   int64_t pairs_within_cutoff; /* non-excluded pairs inside the cutoff, this evaluation */
   int64_t list_builds;    /* neighbour-list builds so far */
 } fz_energies;
@@ -145,7 +147,10 @@ int fz_set_com_restraint(fz_system *sys, const int32_t *group, const double dist
  * (src/evalprops.cu:17). */
 int fz_compute_forces(fz_system *sys, double *forces, fz_energies *e);
 /* nsteps of velocity Verlet (+ SHAKE/RATTLE); neighbour rebuilds decided on
- * the device.  Replaces SingleStep (src/dynamics.cu:623). */
+ * the device.  A capacity overflow is recovered inside the call (restore to the
+ * call's start, grow, replay).  On an error (SHAKE failure, capacity that cannot
+ * grow) the system is back in the state it had before the call.  Replaces
+ * SingleStep (src/dynamics.cu:623). */
 int fz_step(fz_system *sys, int nsteps);
 /* Energies of the last evaluated step (temperature from current velocities). */
 int fz_get_energies(fz_system *sys, fz_energies *e);
@@ -173,6 +178,14 @@ typedef struct fz_min_params {
 } fz_min_params;
 int fz_minimize_ex(fz_system *sys, int nsteps, const fz_min_params *p);
 int fz_system_info(fz_system *sys, int64_t *n_atoms, int64_t *device_bytes, int32_t *precision);
+/* Neighbour-list capacities: pair-list entries and clusters (0 keeps the
+ * current value).  The list is rebuilt at the positions of its last build, so
+ * the state and the list the system holds do not change; a later build that
+ * does not fit grows them again (restore to the start of the fz_step call and
+ * replay).  Replaces the compile-time MAXNB / CELL_ATOMS capacities of the
+ * reference (src/_config/fenzi_conf_options:20, src/configure:89-137) and its
+ * exit on overflow (Check_Static_Parameters, src/dynamics.cu:52-234). */
+int fz_set_neighbor_capacity(fz_system *sys, int64_t pairs_cap, int32_t clusters_cap);
 /* Per-phase device time: runs nsteps steps without graph capture, with HIP
  * events between phases on the system's stream; mean ms per step per phase in
  * ms[FZ_NPHASE], phases: 0 first kick + drift + SHAKE + rebuild test, 1
@@ -210,7 +223,9 @@ int fz_pme_convolve(fz_ctx *ctx, const int32_t K[3], const double box[3], double
 
 /* ---- batch: many systems per launch ---------------------------------------------- */
 /* Systems stay owned by their handles; the batch advances them in place with
- * one launch per phase per step (hipGraph-captured). */
+ * one launch per phase per step (hipGraph-captured).  fz_batch_step,
+ * fz_batch_minimize and fz_batch_minimize_ex leave every member in the state
+ * it had before the call when they fail. */
 int fz_batch_create(fz_ctx *ctx, fz_system *const *systems, int n, fz_batch **out);
 int fz_batch_step(fz_batch *b, int nsteps);
 /* Steepest descent on every member (per-system step control), one launch per phase. */

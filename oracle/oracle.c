@@ -433,19 +433,49 @@ static void real_space(or_sys *s, const uint64_t *u, double *f, double *e, doubl
   free(fth); free(eth); free(vth); free(head); free(next); free(cellof);
 }
 
+/* Non-excluded pairs with r < rcut, over a cell grid of edge >= rcut whose
+ * stencil visits each neighbouring cell once (fewer than three cells on an axis
+ * visit every cell of that axis once), so each pair is tested exactly once. */
 int64_t or_count_pairs_within(or_sys *s, const uint64_t *u, double rcut) {
   int n = s->t.n_atoms;
   double sc[3] = {s->p.box[0] * TWO_POW_M64, s->p.box[1] * TWO_POW_M64, s->p.box[2] * TWO_POW_M64};
   double r2c = rcut * rcut;
+  int nc[3];
+  for (int d = 0; d < 3; d++) { nc[d] = (int)floor(s->p.box[d] / rcut); if (nc[d] < 1) nc[d] = 1; }
+  int ncell = nc[0] * nc[1] * nc[2];
+  int *head = (int *)malloc(sizeof(int) * (size_t)ncell), *next = (int *)malloc(sizeof(int) * (size_t)n);
+  int *cellof = (int *)malloc(sizeof(int) * (size_t)n);
+  for (int c = 0; c < ncell; c++) head[c] = -1;
+  for (int i = n - 1; i >= 0; i--) {
+    int c[3];
+    for (int d = 0; d < 3; d++) c[d] = (int)(((u128)u[3 * i + d] * (u128)nc[d]) >> 64);
+    int id = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
+    cellof[i] = id; next[i] = head[id]; head[id] = i;
+  }
   int64_t cnt = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : cnt)
-  for (int i = 0; i < n; i++)
-    for (int j = i + 1; j < n; j++) {
-      double d[3];
-      fx_disp(u, i, j, sc, d);
-      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-      if (r2 < r2c && !excluded(s, i, j)) cnt++;
+  for (int i = 0; i < n; i++) {
+    int ci = cellof[i];
+    int c0[3] = {ci % nc[0], (ci / nc[0]) % nc[1], ci / (nc[0] * nc[1])};
+    int rng[3][3], nr[3];
+    for (int d = 0; d < 3; d++) {
+      if (nc[d] < 3) { nr[d] = nc[d]; for (int k = 0; k < nc[d]; k++) rng[d][k] = k; }
+      else { nr[d] = 3; for (int k = 0; k < 3; k++) rng[d][k] = (c0[d] + k - 1 + nc[d]) % nc[d]; }
     }
+    for (int a = 0; a < nr[2]; a++)
+      for (int b = 0; b < nr[1]; b++)
+        for (int c = 0; c < nr[0]; c++) {
+          int cj = (rng[2][a] * nc[1] + rng[1][b]) * nc[0] + rng[0][c];
+          for (int j = head[cj]; j >= 0; j = next[j]) {
+            if (j <= i) continue;
+            double d[3];
+            fx_disp(u, i, j, sc, d);
+            double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (r2 < r2c && !excluded(s, i, j)) cnt++;
+          }
+        }
+  }
+  free(head); free(next); free(cellof);
   return cnt;
 }
 
@@ -1067,8 +1097,15 @@ static int shake_cluster(const or_sys *s, const or_cluster *cl, const uint64_t *
 }
 
 /* RATTLE velocity projection of one cluster (src/shake.cu:330-615): exact solve
- * of r_c . (v_c + dv_c) = 0 for all constraints of the cluster. */
-static void rattle_cluster(const or_sys *s, const or_cluster *cl, const uint64_t *u, const double sc[3], double *v) {
+ * of r_c . (v_c + dv_c) = 0 for all constraints of the cluster.  With wc, adds
+ * the constraint virial of the projection: the velocity change of atom a is
+ * minv_a sum_c b_c B_ca r_c, i.e. the half-kick dt/(2 m_a) G_a of the
+ * constraint force G_a = (2/dt) sum_c b_c B_ca r_c, whose virial
+ * sum_a r_a (x) G_a is (2/dt) sum_c b_c r_c (x) r_c (the constraint-force part
+ * of the reference's PCONSTANT virial, src/shake.cu:300-306, there halved and
+ * diagonal only). */
+static void rattle_cluster(const or_sys *s, const or_cluster *cl, const uint64_t *u, const double sc[3], double *v,
+                           double dt, double *wc) {
   int na = cl->natom, nc = cl->ncons;
   double r[8][3], minv[8];
   for (int k = 0; k < na; k++) { fx_disp(u, cl->atom[0], cl->atom[k], sc, r[k]); minv[k] = dyn_minv(s, cl->atom[k]); }
@@ -1093,6 +1130,9 @@ static void rattle_cluster(const or_sys *s, const or_cluster *cl, const uint64_t
       v[3 * cl->atom[i] + d] += b[c] * rc_[c][d] * minv[i];
       v[3 * cl->atom[j] + d] -= b[c] * rc_[c][d] * minv[j];
     }
+    if (wc)
+      for (int a = 0; a < 3; a++)
+        for (int d = 0; d < 3; d++) wc[3 * a + d] += 2.0 / dt * b[c] * rc_[c][a] * rc_[c][d];
   }
 }
 
@@ -1130,6 +1170,10 @@ double or_temperature(or_sys *s, const double *v) {
  * (SingleStep, src/dynamics.cu:623-820; HalfKickGPU src/mdcuda.cu:27,
  * UpdateCoords :354). */
 int or_step(or_sys *s, uint64_t *u, double *v, int nsteps) {
+  return or_step_ex(s, u, v, nsteps, NULL, NULL);
+}
+
+int or_step_ex(or_sys *s, uint64_t *u, double *v, int nsteps, double *e_out, double *vir_out) {
   int n = s->t.n_atoms;
   double dt = OR_TU * s->p.dt_ps;
   double sc[3] = {s->p.box[0] * TWO_POW_M64, s->p.box[1] * TWO_POW_M64, s->p.box[2] * TWO_POW_M64};
@@ -1162,8 +1206,10 @@ int or_step(or_sys *s, uint64_t *u, double *v, int nsteps) {
       double hm = 0.5 * dt * dyn_minv(s, i);
       for (int d = 0; d < 3; d++) v[3 * i + d] += hm * f[3 * i + d];
     }
-    for (int c = 0; c < s->n_clusters; c++) rattle_cluster(s, &s->clusters[c], u, sc, v);
+    for (int c = 0; c < s->n_clusters; c++) rattle_cluster(s, &s->clusters[c], u, sc, v, dt, vir);
   }
+  if (e_out) memcpy(e_out, e, sizeof(e));
+  if (vir_out) memcpy(vir_out, vir, sizeof(vir));
   free(f); free(uold);
   return rc;
 }
@@ -1222,5 +1268,5 @@ void or_maxwell(or_sys *s, const uint64_t *u, double T, uint64_t seed, uint64_t 
     if (!s->mobile || s->mobile[i])
       for (int d = 0; d < 3; d++) v[3 * i + d] -= P[d] / nm / s->t.mass[i];
   double sc[3] = {s->p.box[0] * TWO_POW_M64, s->p.box[1] * TWO_POW_M64, s->p.box[2] * TWO_POW_M64};
-  for (int c = 0; c < s->n_clusters; c++) rattle_cluster(s, &s->clusters[c], u, sc, v);
+  for (int c = 0; c < s->n_clusters; c++) rattle_cluster(s, &s->clusters[c], u, sc, v, 1.0, NULL);
 }

@@ -102,6 +102,10 @@ int or_set_com_restraint(or_sys *s, const int32_t *group, const double dist[3], 
 
 /* Velocity Verlet with SHAKE/RATTLE: nsteps steps; u, v updated in place. */
 int or_step(or_sys *s, uint64_t *u, double *v, int nsteps);
+/* or_step that also returns the energies [OR_NE] and the virial [9] of the last
+ * step (nullable): the force virial at the final positions plus the
+ * constraint virial of the final RATTLE projection, (2/dt) sum_c b_c r_c (x) r_c. */
+int or_step_ex(or_sys *s, uint64_t *u, double *v, int nsteps, double *e_out, double *vir_out);
 /* SHAKE of positions u against reference positions uold (minimiser moves). */
 int or_shake_positions(or_sys *s, const uint64_t *uold, uint64_t *u);
 /* Maxwell velocities (Philox4x32-10 + Box-Muller), momentum removal, RATTLE projection. */

@@ -75,6 +75,8 @@ def lib():
         L.or_cluster_pairs.restype = C.c_int64
         L.or_step.argtypes = [C.c_void_p, _up, _dp, C.c_int]
         L.or_step.restype = C.c_int
+        L.or_step_ex.argtypes = [C.c_void_p, _up, _dp, C.c_int, _dp, _dp]
+        L.or_step_ex.restype = C.c_int
         L.or_shake_positions.argtypes = [C.c_void_p, _up, _up]
         L.or_shake_positions.restype = C.c_int
         L.or_maxwell.argtypes = [C.c_void_p, _up, C.c_double, C.c_uint64, C.c_uint64, _dp]
@@ -213,6 +215,17 @@ class Oracle:
         rc = lib().or_step(self.h, _p(u, C.c_uint64), _p(v, C.c_double), int(nsteps))
         return u, v, rc
 
+    def step_ex(self, u, v, nsteps):
+        """step() that also returns the last step's energies and virial (forces
+        at the final positions plus the final RATTLE's constraint virial)."""
+        u = np.ascontiguousarray(u, dtype=np.uint64).copy()
+        v = np.ascontiguousarray(v, dtype=np.float64).copy()
+        e = np.zeros(10)
+        w = np.zeros(9)
+        rc = lib().or_step_ex(self.h, _p(u, C.c_uint64), _p(v, C.c_double), int(nsteps), _p(e, C.c_double),
+                              _p(w, C.c_double))
+        return u, v, dict(zip(E_NAMES, e)), w.reshape(3, 3), rc
+
     def shake_positions(self, uold, u):
         u = np.ascontiguousarray(u, dtype=np.uint64).copy()
         uold = np.ascontiguousarray(uold, dtype=np.uint64)
@@ -269,6 +282,40 @@ class Oracle:
                 else:
                     sdfac *= sdrate * 100.0
         return u, method, done
+
+    def minimize_sd(self, u, nsteps, max_step, step=0.0):
+        """The library's adaptive steepest descent (fz_minimize; the reference's
+        MinimizeStep SD branch, src/dynamics.cu:491-620 and
+        src/mdcuda.cu:245-330, with its own step rule): the mobile atom with the
+        largest force moves `step` A along it and every mobile atom moves by
+        the same multiple of its force, then SHAKE against the previous
+        positions; an energy below the last accepted one is accepted and the
+        step grows by 1.2 (capped at max_step), otherwise the move is undone and
+        the step halves.  step <= 0 starts at max_step.  Returns (u, step)."""
+        sc = np.array(self.box) / 2.0 ** 64
+        u = np.ascontiguousarray(u, dtype=np.uint64).copy()
+        step = max_step if step <= 0.0 else min(step, max_step)
+        f, e, _ = self.forces(u)
+        e_old = sum(e.values())
+        mobile = ~self.fixed
+        for _ in range(nsteps):
+            f2 = (f * f).sum(1)
+            f2max = f2[mobile].max() if mobile.any() else 0.0
+            a = step / np.sqrt(f2max) if f2max > 0 else 0.0
+            uold = u.copy()
+            h = np.where(mobile[:, None], a * f, 0.0)
+            u = u + np.rint(h / sc).astype(np.int64).view(np.uint64)
+            u, _ = self.shake_positions(uold, u)
+            fn, e, _ = self.forces(u)
+            pot = sum(e.values())
+            if pot < e_old:
+                e_old = pot
+                step = min(step * 1.2, max_step)
+                f = fn
+            else:
+                u = uold
+                step *= 0.5
+        return u, step
 
     def maxwell(self, u, T, seed, counter=0):
         u = np.ascontiguousarray(u, dtype=np.uint64)

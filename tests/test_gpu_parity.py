@@ -340,3 +340,25 @@ def test_fenzi_driver_checkpoints_restarts_and_writes_a_trajectory(tmp_path):
     assert [int(r[0]) for r in rows] == [40, 60]
     r60 = fenzi.restart_read(prefix + ".rst", n, binary=True)
     assert r60["step"] == 60 and not np.array_equal(r60["xyz"], r40["xyz"])
+
+
+@pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
+def test_step_virial_with_constraint_part_vs_oracle(ctx, prec):
+    """The virial reported after fz_step: force virial at the final positions
+    plus the constraint virial of the last RATTLE projection, against the
+    oracle's or_step_ex (rigid waters and SHAKE'd chain-free slab)."""
+    topo, xyz, params = systems.slab(seed=14)
+    orc = oracle.Oracle(topo, params)
+    u = orc.to_fixed(xyz)
+    v0 = orc.maxwell(u, 300.0, seed=5, counter=0)
+    u_ref, _, e_ref, w_ref, rc = orc.step_ex(u, v0, 6)
+    assert rc == 0
+    s = _system(topo, params, prec, ctx, xyz, orc)
+    s.reassign_velocities(300.0, 5, 0)
+    s.step(6)
+    e = s.energies()
+    _, _, w_forces = orc.forces(u_ref)
+    wc = w_ref - w_forces
+    assert np.abs(wc).max() > 1e-2 * np.abs(w_ref).max()   # the constraint part matters here
+    tol = 1e-7 if prec == fenzi.FP64 else 2e-5
+    assert np.abs(e["virial"] - w_ref).max() < tol * np.abs(w_ref).max()

@@ -134,3 +134,46 @@ def test_server_isolates_a_failed_job(ctx):
         srv.result(badid)
     srv.release(badid)
     srv.close()
+
+
+def test_server_isolates_a_job_failing_mid_run(ctx):
+    """A job whose SHAKE cannot converge (tolerance 1e-30) fails in its first
+    dynamics round, inside the batch of every dynamics job: the batch call
+    restores every member to the round's start, the members are retried one by
+    one, the failing job is marked failed, and every other job ends bitwise as
+    it does alone, with its full step count."""
+    jobs = _jobs()[:3]
+    alone = [_run_server(ctx, [j], chunk_steps=25)[0] for j in jobs]
+    t, x, p = systems.water_box(n_waters=200, L=19.0, seed=6)
+    srv = fenzi.Server(ctx, 8e9, precision=fenzi.MIXED, chunk_steps=25)
+    ids = [srv.submit(tt, pp, xx, m, n, 300.0, r, s) for (tt, xx, pp), m, n, r, s in jobs]
+    bad = srv.submit(t, dict(p, shake_tol=1e-30), x, 10, 60, 300.0, 0, 1)
+    srv.run_until_done()
+    assert srv.info(bad)["state"] == fenzi.JOB_FAILED
+    for jid, a, j in zip(ids, alone, jobs):
+        info = srv.info(jid)
+        assert info["state"] == fenzi.JOB_DONE and info["steps_done"] == j[2]
+        r = srv.result(jid)
+        assert np.array_equal(r["fixed"], a["fixed"])
+        assert np.array_equal(r["vel"], a["vel"])
+        assert r["energies"]["step"] == j[2]
+    srv.close()
+
+
+def test_admission_estimate_bounds_the_device_footprint(ctx):
+    """fz_server_estimate_bytes (the admission figure) is at least what a
+    system of that job actually holds on the device, from small to large."""
+    import fz_synth
+    cases = [systems.water_box(seed=2), systems.slab(seed=1)]
+    for n, f, seed in ((1000, 0.0, 1), (6000, 0.3, 2), (24000, 0.15, 3), (60000, 0.1, 4)):
+        topo, xyz, params = fz_synth.ensemble_member(n, f, seed)
+        cases.append((topo, xyz, params))
+    for topo, xyz, params in cases:
+        est = fenzi.estimate_bytes(topo, params, fenzi.MIXED)
+        s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+        s.reassign_velocities(300.0, 1, 0)
+        s.step(20)
+        used = s.info()["device_bytes"]
+        assert used <= est, (topo["n_atoms"], used, est)
+        assert est < 3 * used, (topo["n_atoms"], used, est)
+        s.close()

@@ -494,3 +494,48 @@ def test_fixed_atoms_stay_put_and_constraints_between_them_are_dropped():
     # 3 N_mobile - 6 - N_constraints: the 10 fixed waters' 30 constraints are gone
     dof = 3 * (topo["n_atoms"] - 31) - 6 - (topo["n_constraints"] - 30)
     assert abs(o.temperature(v) - 2 * o.kinetic(v) / (oracle.RC * dof)) < 1e-12 * o.temperature(v)
+
+
+def test_adaptive_minimiser_lowers_energy_and_keeps_constraints():
+    """The restated adaptive steepest descent: the largest force moves max_step
+    on the first move, the energy falls, SHAKE holds, and a split run (step
+    size carried over) follows one run."""
+    topo, xyz, params = systems.slab(n_chains=2, n_waters=120, L=20.0, seed=12, K=(20, 20, 20))
+    o = oracle.Oracle(topo, params)
+    u0 = o.to_fixed(xyz)
+    f0, e0, _ = o.forces(u0)
+    u1, s1 = o.minimize_sd(u0, 1, 0.05)
+    d = (u1 - u0).view(np.int64).astype(np.float64) * (params["box"][0] / 2.0 ** 64)
+    imax = int(np.argmax((f0 * f0).sum(1)))
+    if s1 > 0.05 - 1e-12:     # accepted: the largest-force atom moved by the step (before SHAKE of its cluster)
+        assert np.linalg.norm(d[imax]) > 0.0
+    ua, sa = o.minimize_sd(u0, 30, 0.05)
+    ub, sb = o.minimize_sd(u0, 12, 0.05)
+    ub, sb = o.minimize_sd(ub, 18, 0.05, step=sb)
+    # the oracle's threaded force sums differ from run to run in the last bit
+    assert np.abs((ua - ub).view(np.int64)).max() * params["box"][0] / 2.0 ** 64 < 1e-9 and sa == sb
+    _, e1, _ = o.forces(ua)
+    assert sum(e1.values()) < sum(e0.values())
+    res, _ = o.constraint_residual(ua)
+    assert res < params["shake_tol"] * 10
+
+
+def test_adaptive_minimiser_on_one_bond_follows_the_scalar_recurrence():
+    """On one harmonic bond (E = k x^2, both atoms feel |F| = 2 k |x|) every
+    accepted move shifts each atom by the step toward the other, so the stretch
+    follows x -> x - 2 s sign(x) when |x - 2 s sign(x)| < |x| (then s -> min(1.2 s,
+    s_max)), else x stays and s -> s / 2."""
+    k, r0, x0, smax = 100.0, 1.5, 0.37, 0.05
+    o = oracle.Oracle(_two_atom_bond(k, r0), dict(box=(30.0,) * 3, cutoff=9.0, cuton=8.0, pairlist=10.0,
+                                                  beta=0.3, K=(16, 16, 16), dt_ps=0.001))
+    u = o.to_fixed(np.array([[0.0, 0.0, 0.0], [r0 + x0, 0.0, 0.0]]))
+    u2, s2 = o.minimize_sd(u, 25, smax)
+    x2 = o.from_fixed(u2)
+    x, s = x0, smax
+    for _ in range(25):
+        xn = x - 2.0 * s * np.sign(x)
+        if abs(xn) < abs(x):
+            x, s = xn, min(1.2 * s, smax)
+        else:
+            s *= 0.5
+    assert abs((x2[1, 0] - x2[0, 0] - r0) - x) < 1e-12 and abs(s2 - s) < 1e-15

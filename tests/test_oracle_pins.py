@@ -358,3 +358,42 @@ def test_dmpc_small_pair_tables_follow_the_prm_conversions(tmp_path):
     e, f = _lj_pair(eps[ot, ot], sig2[ot, ot], 2 * 1.7765722)
     assert abs(e + 0.1554253) < 1e-12 and abs(f) < 1e-11
     deck.close()
+
+
+def test_constraint_virial_satisfies_the_rigid_molecule_identity():
+    """For a rigid molecule, d^2/dt^2 sum_i m_i |r_i - R|^2 = 0 gives, at every
+    instant, sum_i (r_i - R) . (F_i + G_i) = -sum_i m_i |v_i - V|^2 (G the
+    constraint forces).  The trace of the constraint virial that the final
+    RATTLE projection yields (the step's virial minus the force virial) meets
+    it to first order in the time step: RATTLE's multipliers see the relative
+    velocities half a step earlier (src/shake.cu:300-306 computes the same
+    quantity from SHAKE)."""
+    topo, xyz, params = systems.water_box(150, 17.0, seed=4)
+    p0 = dict(params, cutoff=8.0, cuton=6.0, pairlist=9.0, K=(20, 20, 20))
+    o = oracle.Oracle(topo, dict(p0, dt_ps=0.0005))
+    u = o.to_fixed(xyz)
+    v = o.maxwell(u, 300.0, 1, 0)
+    u, v, _ = o.step(u, v, 40)
+    m = np.asarray(topo["mass"])
+    L = np.array(p0["box"])
+    errs = []
+    for dt, n in ((0.0005, 4), (0.00025, 8), (0.000125, 16)):
+        od = oracle.Oracle(topo, dict(p0, dt_ps=dt))
+        u2, v2, _, W, rc = od.step_ex(u, v, n)
+        assert rc == 0
+        f, _, Wf = od.forces(u2)
+        Wc = W - Wf
+        assert np.abs(Wc - Wc.T).max() < 1e-9 * np.abs(Wc).max()
+        x = od.from_fixed(u2)
+        rhs = 0.0
+        for w in range(0, topo["n_atoms"], 3):
+            idx = [w, w + 1, w + 2]
+            xr = x[idx] - x[w]
+            xr -= L * np.round(xr / L)
+            M = m[idx].sum()
+            R = (m[idx, None] * xr).sum(0) / M
+            V = (m[idx, None] * v2[idx]).sum(0) / M
+            rhs += -(m[idx] * ((v2[idx] - V) ** 2).sum(1)).sum() - ((xr - R) * f[idx]).sum()
+        errs.append(abs(np.trace(Wc) / rhs - 1.0))
+    assert errs[-1] < 1.5e-3
+    assert 1.6 < errs[0] / errs[1] < 2.6 and 1.6 < errs[1] / errs[2] < 2.6, errs
