@@ -462,6 +462,7 @@ int System::alloc_device() {
   FZ_TRY(dalloc(&d.mH, (size_t)3 * n));
   FZ_TRY(dalloc(&d.mF, (size_t)3 * n));
   FZ_TRY(dalloc(&snap, (size_t)9 * n + (F_NSLOT * sizeof(int) + 7) / 8));
+  d.snap = snap;
   return set_column_grid(par.column_grid[0], par.column_grid[1], 1.0);
 }
 
@@ -587,8 +588,17 @@ int64_t footprint_bytes(const fz_topology* t, const fz_params* p, int precision)
   b += 3 * rnd256(8 * KS) + rnd256(KS * (mixed ? 8 : 16));
   b += rnd256(8 * A_NSLOT) + rnd256(4 * F_NSLOT) + rnd256(16) + 2 * rnd256(24 * n);
   b += rnd256(72 * n + 16);                                  // snapshot
+  // the neighbour block after two growth steps of the overflow path (pair-list
+  // entries x1.6 + 4096, clusters x1.3 + 64 each): clustered chains can need
+  // about twice the pair entries the uniform-density estimate gives
   const NbShape sh = nb_shape(*p, t->n_atoms, p->column_grid[0], p->column_grid[1], 1.0);
-  b += nb_bytes(sh.ncol, sh.nzb, mixed, (long long)(sh.pairs_cap * 1.6) + 4096, (int)(sh.clusters_cap * 1.3) + 64);
+  long long pairs = sh.pairs_cap;
+  int clus = sh.clusters_cap;
+  for (int g = 0; g < 2; g++) {
+    pairs = (long long)(pairs * 1.6) + 4096;
+    clus = (int)(clus * 1.3) + 64;
+  }
+  b += nb_bytes(sh.ncol, sh.nzb, mixed, pairs, clus);
   return (int64_t)b;
 }
 
@@ -899,7 +909,19 @@ int Engine::prepare() {
 int Engine::ensure() {
   bool stale = !prepared_ || gens_.size() != sys_.size();
   for (size_t k = 0; !stale && k < sys_.size(); k++) stale = gens_[k] != sys_[k]->gen;
-  return stale ? prepare() : FZ_OK;
+  if (stale) FZ_TRY(prepare());
+  // The atomic spread adds into a zero grid, which its x pass leaves behind;
+  // the brick spread overwrites the grid and leaves its last spread there.  A
+  // member whose grid was last spread by bricks (in another engine: a batch
+  // over 64K atoms) is cleared before this engine spreads atomically.
+  for (auto* s : sys_) {
+    if (!L_.spread_bricks && s->grid_mode != 0 && s->dv.Qacc) {
+      const size_t K3 = (size_t)s->dv.K[0] * s->dv.K[1] * s->dv.K[2];
+      FZ_HIP(hipMemsetAsync(s->dv.Qacc, 0, K3 * 8, L_.stream));
+    }
+    s->grid_mode = L_.spread_bricks;
+  }
+  return FZ_OK;
 }
 
 void Engine::record_forces(bool ev, bool lite, bool check) {
@@ -939,9 +961,12 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
 // second half of this step fused with the first half of the next], then one
 // replay of [forces with energies, second half, kinetic energy].  Every
 // replay thus holds one integrator kernel between force evaluations; the
-// arithmetic is that of separate halves.
+// arithmetic is that of separate halves.  The step's pair energies are summed
+// in FP64 from the FP32 pair terms (the mixed form's arithmetic; a stand-alone
+// evaluation recomputes them in FP64, which costs the 17K slab's pair kernel
+// seven times its force-only time).
 void Engine::record_step(bool ev) {
-  record_forces(ev, false, false);
+  record_forces(ev, true, false);
   if (!ev) {
     launch_step_first(L_, true);
   } else {
@@ -1006,13 +1031,13 @@ int Engine::grow_and_reset(int si, int bits) {
 
 static unsigned long long* snap_part(System* s, int k) { return s->snap + (size_t)k * s->topo.n; }
 
-int Engine::snapshot() {
+int Engine::snapshot(bool clear_shake) {
+  // one launch over every member (positions, velocities, build positions,
+  // flags); clear_shake also clears the SHAKE-failure flag first, so that it
+  // reports the steps after the snapshot only
+  launch_snapshot(L_, clear_shake);
+  FZ_HIP(hipGetLastError());
   for (auto* s : sys_) {
-    const size_t nb = (size_t)s->topo.n * 8;
-    void* src[9] = {s->dv.ux, s->dv.uy, s->dv.uz, s->dv.vx, s->dv.vy, s->dv.vz, s->dv.rx, s->dv.ry, s->dv.rz};
-    for (int k = 0; k < 9; k++)
-      FZ_HIP(hipMemcpyAsync(snap_part(s, k), src[k], nb, hipMemcpyDeviceToDevice, L_.stream));
-    FZ_HIP(hipMemcpyAsync(snap_part(s, 9), s->dv.flags, F_NSLOT * sizeof(int), hipMemcpyDeviceToDevice, L_.stream));
     s->snap_step = s->step;
     s->snap_min_step = s->min_step;
   }
@@ -1112,8 +1137,7 @@ int Engine::steps(int n) {
   for (auto* s : sys_)
     if (s->need_rebuild) FZ_TRY(forces(true));
   for (int attempt = 0; attempt < 4; attempt++) {
-    launch_clear_shake_flag(L_);  // the flag reports this attempt's steps only
-    FZ_TRY(snapshot());
+    FZ_TRY(snapshot(true));  // the SHAKE flag reports this attempt's steps only
     if (!g_plain_) FZ_TRY(capture(false, &g_plain_));
     if (!g_ev_) FZ_TRY(capture(true, &g_ev_));
     // The step graph has no slot gather: it relies on the slot copies and
@@ -1302,8 +1326,7 @@ int Engine::phase_times(int nsteps, double* out) {
   FZ_TRY(ensure());
   for (auto* s : sys_)
     if (s->need_rebuild) FZ_TRY(forces(true));
-  launch_clear_shake_flag(L_);
-  FZ_TRY(snapshot());
+  FZ_TRY(snapshot(true));
   const int NP = 9;
   hipEvent_t ev[NP + 1];
   for (int k = 0; k <= NP; k++) FZ_HIP(hipEventCreate(&ev[k]));

@@ -68,6 +68,9 @@ struct System {
   double self_energy = 0.0;
   double min_step = 0.0;  // adaptive minimiser step carried between calls (0 = start at the maximum)
   bool need_rebuild = true;
+  // spread path that last wrote the PME accumulation grid: 0 atomic (the grid is
+  // left zero), 1 bricks (the grid holds that spread); allocated zero
+  int grid_mode = 0;
   // Snapshot for restoring a call that failed or overflowed: 3n positions,
   // 3n velocities (as bits), 3n positions of the last list build, then the
   // status flags (F_NSLOT ints); step and minimiser step on the host.
@@ -161,7 +164,7 @@ class Engine {
   int realloc_neighbor(int s, long long pairs, int clus);
   // device copies of every member's positions, velocities, build positions and
   // flags (stream-ordered), host counters
-  int snapshot();
+  int snapshot(bool clear_shake = false);
   // every member back to its snapshot; grow[k] != 0 enlarges member k's
   // neighbour capacity (overflow bits) first
   int restore(const std::vector<int>* grow = nullptr);

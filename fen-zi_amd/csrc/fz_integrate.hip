@@ -692,6 +692,34 @@ __global__ void k_clear_shake_flag(const SysDev* __restrict__ S, int nsys) {
 void launch_clear_shake_flag(const Launch& L) {
   hipLaunchKernelGGL(k_clear_shake_flag, dim3(L.nsys), dim3(64), 0, L.stream, L.d_sys, L.nsys);
 }
+__global__ __launch_bounds__(256) void k_snapshot(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
+                                                 int clear_shake) {
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int lb = blockIdx.x - block_base(pre, nsys, s_);
+  const int n = d.n_atoms;
+  unsigned long long* sn = d.snap;
+  if (lb == 0 && threadIdx.x < F_NSLOT) {
+    int f = d.flags[threadIdx.x];
+    if (clear_shake && threadIdx.x == F_SHAKE_FAIL) {
+      f = 0;
+      d.flags[F_SHAKE_FAIL] = 0;
+    }
+    ((int*)(sn + (size_t)9 * n))[threadIdx.x] = f;
+  }
+  const int i = lb * 256 + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long* src[9] = {d.ux, d.uy, d.uz, (const unsigned long long*)d.vx, (const unsigned long long*)d.vy,
+                                      (const unsigned long long*)d.vz, d.rx, d.ry, d.rz};
+#pragma unroll
+  for (int k = 0; k < 9; k++) sn[(size_t)k * n + i] = src[k][i];
+}
+
+void launch_snapshot(const Launch& L, bool clear_shake) {
+  hipLaunchKernelGGL(k_snapshot, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre,
+                     clear_shake ? 1 : 0);
+}
+
 void launch_zero_forces(const Launch& L) {
   hipLaunchKernelGGL(k_zero_forces, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
 }

@@ -60,6 +60,9 @@ void launch_kinetic(const Launch& L);
 void launch_maxwell(const Launch& L, double T, unsigned long long seed, unsigned long long counter);
 void launch_zero_acc(const Launch& L);
 void launch_clear_shake_flag(const Launch& L);
+// every member's positions, velocities, build positions and flags into its
+// snapshot (clear_shake: the SHAKE-failure flag cleared first)
+void launch_snapshot(const Launch& L, bool clear_shake);
 void launch_zero_forces(const Launch& L);
 void launch_restraint(const Launch& L);  // COM restraint forces (systems with com_on)
 

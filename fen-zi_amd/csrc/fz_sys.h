@@ -157,6 +157,9 @@ struct SysDev {
   double *mstep;            // minimiser: [0] largest atom displacement of the next move (A) or the
                             // conjugate-gradient / steepest-descent factor, [1] schedule method
   double *mH, *mF;          // [3 * n_atoms] conjugate-gradient direction and previous force (x, y, z blocks)
+  // snapshot of a call's starting state: blocks of n_atoms words ux, uy, uz,
+  // vx, vy, vz (bits), rx, ry, rz, then the F_NSLOT status flags
+  unsigned long long *snap;
 };
 
 }  // namespace fz

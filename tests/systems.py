@@ -18,6 +18,17 @@ def slab(n_chains=6, n_waters=400, L=26.0, seed=5, K=(27, 27, 27)):
     return topo, xyz, params
 
 
+def pair_count_ok(count, orc, u, rc, mixed, delta=4e-5):
+    """The device's count of non-excluded pairs inside the cutoff against the
+    oracle's FP64 count: equal in the FP64 form; in the mixed form, which tests
+    r^2 < rc^2 in FP32 arithmetic (relative rounding of a few 1e-7 at the
+    cutoff, well inside delta / rc), between the FP64 counts within rc - delta
+    and rc + delta."""
+    if not mixed:
+        return count == orc.count_pairs_within(u, rc)
+    return orc.count_pairs_within(u, rc - delta) <= count <= orc.count_pairs_within(u, rc + delta)
+
+
 def rel_rms(a, b):
     return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
 

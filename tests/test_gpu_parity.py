@@ -281,7 +281,7 @@ def test_reference_deck_forces_vs_oracle(ctx, prec, tmp_path):
     assert systems.rel_rms(f, f_ref) < (1e-10 if prec == fenzi.FP64 else 1e-5)
     epot_ref = sum(e_ref.values())
     assert abs(e["potential"] - epot_ref) < (1e-8 if prec == fenzi.FP64 else 1e-5) * sum(abs(v) for v in e_ref.values())
-    assert e["pairs_within_cutoff"] == orc.count_pairs_within(u, params["cutoff"])
+    assert systems.pair_count_ok(e["pairs_within_cutoff"], orc, u, params["cutoff"], prec == fenzi.MIXED)
 
 
 @pytest.mark.parametrize("cid,prec", [(1, fenzi.FP64), (1, fenzi.MIXED), (2, fenzi.MIXED)])
@@ -362,3 +362,8 @@ def test_step_virial_with_constraint_part_vs_oracle(ctx, prec):
     assert np.abs(wc).max() > 1e-2 * np.abs(w_ref).max()   # the constraint part matters here
     tol = 1e-7 if prec == fenzi.FP64 else 2e-5
     assert np.abs(e["virial"] - w_ref).max() < tol * np.abs(w_ref).max()
+    # the step's energies (mixed form: pair terms in FP32, summed in FP64)
+    scale = sum(abs(v) for v in e_ref.values())
+    assert abs(e["potential"] - sum(e_ref.values())) < (1e-9 if prec == fenzi.FP64 else 1e-5) * scale
+    for k, name in (("vdw", "vdw"), ("elec", "elec"), ("pme", "pme")):
+        assert abs(e[name] - e_ref[k]) < (1e-9 if prec == fenzi.FP64 else 1e-5) * scale, k
