@@ -44,6 +44,7 @@ int fz_ctx_create(int device, fz_ctx** out) {
   FZ_HIP(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return set_error(FZ_ERR_ARG, "device %d not present (%d devices)", device, n);
   FZ_HIP(hipSetDevice(device));
+  FZ_TRY(noalloc_init(device));
   *out = new fz_ctx{device};
   return FZ_OK;
 }
@@ -258,6 +259,17 @@ int fz_pme_convolve(fz_ctx* ctx, const int32_t K[3], const double box[3], double
   return pme_convolve(ctx->device, K, box, beta, precision == FZ_MIXED ? 1 : 0, mode, q, phi, energy);
   FZ_GUARD_END
 }
+
+int fz_pme_convolve_noalloc(fz_ctx* ctx, const int32_t K[3], const double box[3], double beta, int precision,
+                            int mode, const double* q_dev, double* phi_dev, double* energy_dev, void* stream) {
+  FZ_GUARD_BEGIN
+  if (!ctx || !K || !box || !q_dev || !phi_dev) return set_error(FZ_ERR_ARG, "null argument");
+  return pme_convolve_noalloc(ctx->device, K, box, beta, precision == FZ_MIXED ? 1 : 0, mode, q_dev, phi_dev,
+                              energy_dev, (hipStream_t)stream);
+  FZ_GUARD_END
+}
+
+int64_t fz_pme_convolve_noalloc_bytes(void) { return (int64_t)noalloc_bytes(); }
 
 int fz_batch_create(fz_ctx* ctx, fz_system* const* systems, int n, fz_batch** out) {
   FZ_GUARD_BEGIN

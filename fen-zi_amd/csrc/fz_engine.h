@@ -189,6 +189,16 @@ int64_t footprint_bytes(const fz_topology* t, const fz_params* p, int precision)
 int pme_convolve(int device, const int32_t K[3], const double box[3], double beta, int mixed, int mode,
                  const double* q, double* phi, double* energy);
 
+// The convolution over buffers carved out of a file-scope device array
+// (fz_noalloc.hip): noalloc_init takes that array's device address on
+// `device` (once per device, that device current); pme_convolve_noalloc
+// enqueues the whole call on `stream` (caller's device grids q, phi and the
+// nullable device scalar energy) and allocates nothing.
+int noalloc_init(int device);
+size_t noalloc_bytes();
+int pme_convolve_noalloc(int device, const int32_t K[3], const double box[3], double beta, int mixed, int mode,
+                         const double* q, double* phi, double* energy, hipStream_t stream);
+
 }  // namespace fz
 
 // device context handle of the C ABI

@@ -80,7 +80,8 @@ EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create
            "fz_server_release", "fz_server_estimate_bytes", "fz_server_destroy",
            "fz_minimize_ex", "fz_batch_minimize_ex", "fz_restart_write", "fz_restart_read", "fz_dcd_open",
            "fz_dcd_write_frame", "fz_dcd_close", "fz_set_fixed_atoms", "fz_set_com_restraint",
-           "fz_deck_com_restraint", "fz_deck_fixed_atoms", "fz_set_neighbor_capacity"]
+           "fz_deck_com_restraint", "fz_deck_fixed_atoms", "fz_set_neighbor_capacity",
+           "fz_pme_convolve_noalloc", "fz_pme_convolve_noalloc_bytes"]
 
 MIN_ADAPTIVE_SD, MIN_REF_CG, MIN_REF_SD = 0, 1, 2
 
@@ -152,6 +153,8 @@ def lib():
         L.fz_get_cluster_pairs.argtypes = [vp, _ip, _up, C.c_int64]
         L.fz_get_cluster_pairs.restype = C.c_int64
         L.fz_pme_convolve.argtypes = [vp, _ip, _dp, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.fz_pme_convolve_noalloc.argtypes = [vp, _ip, _dp, C.c_double, C.c_int, C.c_int, vp, vp, vp, vp]
+        L.fz_pme_convolve_noalloc_bytes.restype = C.c_int64
         L.fz_batch_create.argtypes = [vp, C.POINTER(vp), C.c_int, C.POINTER(vp)]
         L.fz_batch_step.argtypes = [vp, C.c_int]
         L.fz_batch_minimize.argtypes = [vp, C.c_int, C.c_double]
@@ -268,6 +271,23 @@ class Context:
         _check(lib().fz_pme_convolve(self.h, _p(K, C.c_int32), _p(b, C.c_double), float(beta), precision, mode,
                                      _p(q, C.c_double), _p(phi, C.c_double), C.byref(e)), "fz_pme_convolve")
         return phi, e.value
+
+    def pme_convolve_noalloc(self, K, box, beta, q_dev, phi_dev, energy_dev=None, precision=FP64, mode=0,
+                             stream=None):
+        """fz_pme_convolve_noalloc on device addresses (ints): q_dev, phi_dev
+        hold K0*K1*K2 doubles, energy_dev one double (or None); enqueued on
+        stream (None = the null stream), not synchronised."""
+        Kc = np.asarray(K, np.int32)
+        b = np.asarray(box, np.float64)
+        _check(lib().fz_pme_convolve_noalloc(self.h, _p(Kc, C.c_int32), _p(b, C.c_double), float(beta), precision,
+                                             mode, C.c_void_p(q_dev), C.c_void_p(phi_dev),
+                                             C.c_void_p(energy_dev) if energy_dev else None,
+                                             C.c_void_p(stream) if stream else None), "fz_pme_convolve_noalloc")
+
+
+def noalloc_scratch_bytes():
+    """Bytes of the static device array behind fz_pme_convolve_noalloc (per GPU)."""
+    return int(lib().fz_pme_convolve_noalloc_bytes())
 
 
 class System:

@@ -221,6 +221,20 @@ int64_t fz_get_cluster_pairs(fz_system *sys, int32_t *pairs, uint64_t *masks, in
 int fz_pme_convolve(fz_ctx *ctx, const int32_t K[3], const double box[3], double beta, int precision,
                     int mode, const double *q, double *phi, double *energy);
 
+/* fz_pme_convolve over device memory, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  q_dev and phi_dev are the caller's
+ * device grids of K0*K1*K2 doubles, energy_dev a nullable device double; the
+ * whole call is enqueued on `stream` (a hipStream_t; NULL = the null stream)
+ * and nothing is synchronised.  Every buffer beyond them lives in one static
+ * device array of fz_pme_convolve_noalloc_bytes() bytes per GPU, held until
+ * the process ends; grids above 2^18 points (64^3) are rejected (FZ_ERR_ARG).
+ * Calls that overlap on one GPU share that array and race.  The GPU's context
+ * (fz_ctx_create) must exist. */
+int fz_pme_convolve_noalloc(fz_ctx *ctx, const int32_t K[3], const double box[3], double beta, int precision,
+                            int mode, const double *q_dev, double *phi_dev, double *energy_dev, void *stream);
+int64_t fz_pme_convolve_noalloc_bytes(void);
+
 /* ---- batch: many systems per launch ---------------------------------------------- */
 /* Systems stay owned by their handles; the batch advances them in place with
  * one launch per phase per step (hipGraph-captured).  fz_batch_step,
