@@ -47,8 +47,8 @@ int main(int argc, char** argv) {
   fz_ctx* ctx = nullptr;
   if (fz_ctx_create(device, &ctx)) return die("device");
   fz_system* sys = nullptr;
-  if (fz_system_create(ctx, fz_deck_topology(deck), fz_deck_params(deck), prec, &sys)) return die("system setup");
   const fz_params* par = fz_deck_params(deck);
+  fz_params run_par = *par;
   const int64_t n_atoms = fz_deck_topology(deck)->n_atoms;
   // restart from <prefix>.rst when it exists (src/fenzi.cu:161-181)
   const std::string rst = std::string(rs.file_prefix) + ".rst";
@@ -62,7 +62,11 @@ int main(int argc, char** argv) {
       return die("reading the restart file");
     rs.startstep = rstep;
     restart = true;
+    // a barostat run resumes at the box it had reached
+    if (rs.pmass > 0)
+      for (int k = 0; k < 3; k++) run_par.box[k] = rbox[k];
   }
+  if (fz_system_create(ctx, fz_deck_topology(deck), &run_par, prec, &sys)) return die("system setup");
   if (fz_set_state(sys, restart ? rx.data() : fz_deck_coordinates(deck), restart ? rv.data() : nullptr))
     return die("initial state");
   {  // consfix segments and the consharm restraint (src/prmread.cu:674-722)
@@ -83,6 +87,9 @@ int main(int argc, char** argv) {
     mp.cgfac = rs.cgfac; mp.sdfac = rs.sdfac; mp.cgrate = rs.cgrate; mp.sdrate = rs.sdrate;
     if (fz_minimize_ex(sys, (int)rs.minimizesteps, &mp)) return die("minimization");
   }
+  // Andersen barostat from the deck's pmass / pref (the box of the restart file,
+  // when resuming, is the one the state was read at)
+  if (rs.pmass > 0 && fz_set_barostat(sys, rs.pref, rs.pmass)) return die("barostat");
   // velocity draws are counted from the run's first step, so a restarted run
   // continues the same sequence
   uint64_t counter = rs.tempreset_every > 0 ? (uint64_t)(rs.startstep / rs.tempreset_every) + 1 : 1;
@@ -132,10 +139,12 @@ int main(int argc, char** argv) {
     const bool frame = dcd && step % rs.traj_every == 0;
     if (ckpt || frame) {
       if (fz_get_state(sys, rx.data(), rv.data())) return die("state");
-      if (ckpt && fz_restart_write(rst.c_str(), rs.checkpoint_binary, n_atoms, step, (double)rs.seed, par->box,
+      double box_now[3];  // the barostat's current box (the deck's box otherwise)
+      if (fz_get_barostat(sys, box_now, nullptr)) return die("box");
+      if (ckpt && fz_restart_write(rst.c_str(), rs.checkpoint_binary, n_atoms, step, (double)rs.seed, box_now,
                                    rx.data(), rv.data()))
         return die("checkpoint");
-      if (frame && fz_dcd_write_frame(dcd, rx.data(), par->box)) return die("trajectory");
+      if (frame && fz_dcd_write_frame(dcd, rx.data(), box_now)) return die("trajectory");
     }
     if (rs.tempreset_every > 0 && step % rs.tempreset_every == 0 && step < end)
       if (fz_reassign_velocities(sys, rs.temperature, rs.seed, counter++)) return die("velocities");

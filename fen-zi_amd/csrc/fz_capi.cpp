@@ -80,7 +80,8 @@ int fz_set_state(fz_system* s, const double* xyz, const double* vel) {
   FZ_HIP(hipSetDevice(s->ctx->device));
   FZ_TRY(s->sys.upload_positions(xyz));
   FZ_TRY(s->sys.upload_velocities(vel));
-  return s->eng->forces(true);
+  FZ_TRY(s->eng->forces(true));
+  return s->eng->npt_reset_virial();
   FZ_GUARD_END
 }
 
@@ -98,7 +99,34 @@ int fz_set_fixed_positions(fz_system* s, const uint64_t* u) {
   if (!s || !u) return set_error(FZ_ERR_ARG, "null argument");
   FZ_HIP(hipSetDevice(s->ctx->device));
   FZ_TRY(s->sys.upload_fixed(u));
-  return s->eng->forces(true);
+  FZ_TRY(s->eng->forces(true));
+  return s->eng->npt_reset_virial();
+  FZ_GUARD_END
+}
+
+int fz_set_barostat(fz_system* s, double pref_atm, double piston_mass) {
+  FZ_GUARD_BEGIN
+  if (!s) return set_error(FZ_ERR_ARG, "null argument");
+  FZ_HIP(hipSetDevice(s->ctx->device));
+  FZ_TRY(s->eng->sync());
+  FZ_TRY(s->sys.set_barostat(pref_atm, piston_mass));
+  FZ_TRY(s->eng->forces(true));
+  return s->eng->npt_reset_virial();
+  FZ_GUARD_END
+}
+
+int fz_get_barostat(fz_system* s, double* box, double* state) {
+  FZ_GUARD_BEGIN
+  if (!s || !box) return set_error(FZ_ERR_ARG, "null argument");
+  for (int k = 0; k < 3; k++) box[k] = s->sys.dv.L[k];
+  if (state) {
+    const NptState& st = s->sys.npt_host;
+    const bool on = s->sys.dv.npt != 0;
+    state[0] = s->sys.dv.L[0] * s->sys.dv.L[1] * s->sys.dv.L[2];
+    state[1] = on ? st.Pi : 0.0;
+    state[2] = on ? st.Wtr : 0.0;
+  }
+  return FZ_OK;
   FZ_GUARD_END
 }
 

@@ -361,6 +361,7 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
     double cgfac = 1e-10, sdfac = 1e-4, cgrate = 0.1, sdrate = 0.01;
     long ckpt_every = 100000, traj_every = -1;
     int printlevel = 0, ckpt_binary = 1;
+    double pmass = 0.0, pref = 1.0;  // barostat off unless pmass is given (PREF 1 atm, src/defs.in.h:101)
     double box[3] = {0, 0, 0};
     int K[3] = {0, 0, 0};
     std::string crd, psf, prm, rtf, outname;
@@ -423,9 +424,12 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
         const int ns = atoi(val(1).c_str());
         for (int j = 0; j < ns; j++) D->consfix_seg.push_back(val(2 + j));
       }
+      // Andersen barostat (src/prmread.cu:664-670)
+      else if (k == "pmass") pmass = atof(val(1).c_str());
+      else if (k == "pref") pref = atof(val(1).c_str());
       // accepted and not used by the force-and-integrate path
       else if (k == "thermostat" || k == "inbfrq" || k == "gpublocksize" || k == "hfac" ||
-               k == "keepcheckpoints" || k == "restartfilename" || k == "pmass" || k == "pref") {
+               k == "keepcheckpoints" || k == "restartfilename") {
       } else {
         fprintf(stderr, "Ignoring unrecognized token, \"%s\"\n", t[0].c_str());
       }
@@ -568,6 +572,7 @@ int fz_deck_load(const char* in_path, fz_deck** out) {
     s.cgfac = cgfac; s.sdfac = sdfac; s.cgrate = cgrate; s.sdrate = sdrate;
     s.checkpoint_every = ckpt_every; s.traj_every = traj_every > 0 ? traj_every : outputTiming;
     s.print_level = printlevel; s.checkpoint_binary = ckpt_binary;
+    s.pmass = pmass; s.pref = pref;
     {  // <outputtrajectory> without the extension after its last '/' (src/fenzi.cu:150-158)
       std::string pfx = outname.empty() ? std::string("fenzi") : outname;
       const size_t dot = pfx.rfind('.'), sl = pfx.rfind('/');

@@ -34,6 +34,7 @@ const char* last_error() { return g_err.c_str(); }
 
 static const double kPi = 3.14159265358979323846;
 static const double kCC = 332.0716, kTU = 20.4548, kRC = 0.001987;
+static const double kPU = 1.4586e-5;  // atm -> kcal/mol/A^3 (PU, src/defs.in.h:105)
 static const double kTwo64 = 18446744073709551616.0;
 
 void launch_zero_forces(const Launch& L);
@@ -335,6 +336,8 @@ int System::alloc_device() {
   d.zwin = (unsigned long long)(par.pairlist / d.sc[2] * (1.0 + 1e-9)) + 64ull;
   double half_skin = 0.5 * (par.pairlist - par.cutoff);
   d.skin2 = half_skin * half_skin;
+  d.rl = par.pairlist;
+  d.rc = par.cutoff;
   d.beta = par.beta;
   d.sw_inv = 1.0 / std::pow(d.rc2 - d.ron2, 3);
   d.dt = kTU * par.dt_ps;
@@ -512,8 +515,12 @@ int System::set_column_grid(int ncx, int ncy, double f) {
   const NbShape sh = nb_shape(par, topo.n, ncx, ncy, f);
   free_neighbor();  // the previous neighbour block, if any
   d.ncx = sh.ncx; d.ncy = sh.ncy; d.ncol = sh.ncol;
-  d.kx_range = (int)std::ceil(par.pairlist / (par.box[0] / sh.ncx)) + 1;
-  d.ky_range = (int)std::ceil(par.pairlist / (par.box[1] / sh.ncy)) + 1;
+  // a barostat's box may shrink to NPT_MIN_SCALE of its reference size: the
+  // search ranges cover the pair-list distance at that scale
+  const double shrink = d.npt ? NPT_MIN_SCALE : 1.0;
+  const double bx = d.npt ? d.npt_L0[0] : par.box[0], by = d.npt ? d.npt_L0[1] : par.box[1];
+  d.kx_range = (int)std::ceil(par.pairlist / (shrink * bx / sh.ncx)) + 1;
+  d.ky_range = (int)std::ceil(par.pairlist / (shrink * by / sh.ncy)) + 1;
   d.colw_x = (unsigned long long)(kTwo64 / sh.ncx);
   d.colw_y = (unsigned long long)(kTwo64 / sh.ncy);
   d.nzb = sh.nzb;
@@ -619,6 +626,7 @@ int System::set_fixed(const uint8_t* fixed) {
   int nf = 0;
   for (int i = 0; i < n; i++) nf += fixed && fixed[i] ? 1 : 0;
   if (nf == n) return set_error(FZ_ERR_ARG, "every atom is fixed");
+  if (nf > 0 && dv.npt) return set_error(FZ_ERR_ARG, "fixed atoms (consfix) cannot join a barostat system");
   for (int i = 0; i < n; i++) topo.fixed[i] = fixed && fixed[i] ? 1 : 0;
   topo.n_fixed = nf;
   FZ_TRY(build_ccl(topo));
@@ -645,6 +653,50 @@ int System::set_fixed(const uint8_t* fixed) {
 }
 
 // Harmonic COM restraint (consharm, src/prmread.cu:674-690; src/mdcuda.cu:481-516).
+// Andersen barostat (pmass / pref, src/prmread.cu:664-670; piston of
+// reduce_virial, src/cell_based_kernels.cu:403-600): the current box becomes the
+// reference box L0 (scale 1), the piston starts at rest.  The column search
+// ranges are widened so that they hold at the smallest box scale allowed,
+// NPT_MIN_SCALE; the cutoff must stay below half of that box.
+int System::set_barostat(double pref_atm, double mass) {
+  SysDev& d = dv;
+  if (!(mass > 0.0)) {
+    d.npt = 0;
+    gen++;  // the descriptor changes (re-uploaded by the next prepare)
+    return FZ_OK;
+  }
+  if (!std::isfinite(pref_atm)) return set_error(FZ_ERR_ARG, "reference pressure must be finite");
+  if (topo.n_fixed > 0) return set_error(FZ_ERR_ARG, "the barostat does not take fixed atoms (consfix)");
+  for (int k = 0; k < 3; k++)
+    if (!(par.cutoff < 0.5 * NPT_MIN_SCALE * d.L[k]))
+      return set_error(FZ_ERR_ARG, "barostat: cutoff %.3f must stay below half the box at %.2f of its size (%.3f)",
+                       par.cutoff, NPT_MIN_SCALE, 0.5 * NPT_MIN_SCALE * d.L[k]);
+  if (!d.nst) FZ_TRY(dalloc((char**)&d.nst, 2 * sizeof(NptState)));
+  d.npt = 1;
+  d.npt_pref = pref_atm * kPU;
+  d.npt_mass = mass;
+  for (int k = 0; k < 3; k++) d.npt_L0[k] = d.L[k];
+  d.npt_V0 = d.L[0] * d.L[1] * d.L[2];
+  d.kx_range = (int)std::ceil(par.pairlist / (NPT_MIN_SCALE * d.npt_L0[0] / d.ncx)) + 1;
+  d.ky_range = (int)std::ceil(par.pairlist / (NPT_MIN_SCALE * d.npt_L0[1] / d.ncy)) + 1;
+  NptState st{};
+  st.V = d.npt_V0;
+  st.ell = st.ell_b = 1.0;
+  st.nbuild = flags_host[F_NBUILD];
+  npt_host = st;
+  FZ_HIP(hipMemcpy(d.nst, &st, sizeof st, hipMemcpyHostToDevice));
+  npt_box_fields(d, 1.0, 1.0);
+  need_rebuild = true;
+  gen++;
+  return FZ_OK;
+}
+
+void System::npt_sync_host() {
+  if (!dv.npt) return;
+  npt_box_fields(dv, npt_host.ell, npt_host.ell_b);
+  for (int k = 0; k < 3; k++) par.box[k] = dv.L[k];
+}
+
 int System::set_com_restraint(const int32_t* group, const double dist[3], double fc) {
   const int n = topo.n;
   SysDev& d = dv;
@@ -690,7 +742,7 @@ int System::upload_positions(const double* xyz) {
   std::vector<uint64_t> u((size_t)3 * n);
   for (int i = 0; i < n; i++)
     for (int k = 0; k < 3; k++) {
-      double t = xyz[3 * i + k] / par.box[k];
+      double t = xyz[3 * i + k] / dv.L[k];
       t -= std::floor(t);
       if (!(t < 1.0)) t = 0.0;
       u[3 * i + k] = (uint64_t)(t * kTwo64);
@@ -721,7 +773,7 @@ int System::download_state(double* xyz, double* v, uint64_t* u) {
         if (u) u[3 * i + k] = c[i];
         if (xyz) {
           double x = (double)c[i] * dv.sc[k];
-          if (x >= 0.5 * par.box[k]) x -= par.box[k];
+          if (x >= 0.5 * dv.L[k]) x -= dv.L[k];
           xyz[3 * i + k] = x;
         }
       }
@@ -764,6 +816,9 @@ int System::energies(fz_energies* e) {
   for (int k = 0; k < 9; k++) e->virial[k] = fx(a[A_VIR + k]);
   e->pairs_within_cutoff = (int64_t)a[A_NPAIR];
   e->list_builds = flags_host[F_NBUILD];
+  e->volume = dv.L[0] * dv.L[1] * dv.L[2];
+  // (2 KE + tr W) / 3V in atm (the reference's pressure, src/cell_based_kernels.cu:586-589)
+  e->pressure = (2.0 * e->kinetic + e->virial[0] + e->virial[4] + e->virial[8]) / (3.0 * e->volume) / kPU;
   return FZ_OK;
 }
 
@@ -789,7 +844,8 @@ Engine::~Engine() {
 void Engine::drop_graphs() {
   if (g_plain_) (void)hipGraphExecDestroy(g_plain_);
   if (g_ev_) (void)hipGraphExecDestroy(g_ev_);
-  g_plain_ = g_ev_ = nullptr;
+  if (g_npt_) (void)hipGraphExecDestroy(g_npt_);
+  g_plain_ = g_ev_ = g_npt_ = nullptr;
 }
 
 int Engine::make_work(Work& w, const std::vector<int>& blocks) {
@@ -840,6 +896,9 @@ int Engine::prepare() {
   L_.mixed = sys_[0]->mixed;
   for (auto* s : sys_)
     if (s->mixed != L_.mixed) return set_error(FZ_ERR_ARG, "a batch must use one precision");
+  L_.npt = sys_[0]->dv.npt;
+  for (auto* s : sys_)
+    if (s->dv.npt != L_.npt) return set_error(FZ_ERR_ARG, "a batch must be all barostat or all fixed-box systems");
   L_.lj_sep = 1;
   L_.com_any = 0;
   for (auto* s : sys_) {
@@ -975,6 +1034,73 @@ void Engine::record_step(bool ev) {
   }
 }
 
+// One barostat step (see the NPT kernels in fz_integrate.hip): piston and box
+// halves around the drift, SHAKE at the new box, forces with the virial every
+// step (pair energies from the FP32 terms), the clusters' internal virial and
+// the piston half kick, kick and RATTLE, kinetic energy.
+void Engine::record_npt_step() {
+  launch_npt_a(L_);
+  launch_npt_b(L_);
+  launch_npt_c(L_);
+  launch_npt_d(L_);
+  record_forces(true, true, false);
+  launch_npt_intra(L_);
+  launch_npt_g(L_);
+  launch_step_second(L_);
+  launch_kinetic(L_);
+}
+
+int Engine::capture_npt(hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  FZ_HIP(hipStreamBeginCapture(L_.stream, hipStreamCaptureModeRelaxed));
+  record_npt_step();
+  FZ_HIP(hipStreamEndCapture(L_.stream, &g));
+  hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return set_error(FZ_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  return FZ_OK;
+}
+
+// n barostat steps, one replay each; failure handling as in steps()
+int Engine::steps_npt(int n) {
+  for (auto* s : sys_)
+    if (s->need_rebuild) FZ_TRY(forces(true));
+  for (int attempt = 0; attempt < 4; attempt++) {
+    FZ_TRY(snapshot(true));
+    if (!g_npt_) FZ_TRY(capture_npt(&g_npt_));
+    launch_gather_sorted(L_);
+    for (int k = 0; k < n; k++) FZ_HIP(hipGraphLaunch(g_npt_, L_.stream));
+    FZ_TRY(read_status());
+    std::vector<int> grow(sys_.size(), 0);
+    int overflow = 0, fatal = -1, shake = -1, box = -1;
+    for (size_t k = 0; k < sys_.size(); k++) {
+      const int ov = sys_[k]->flags_host[F_OVERFLOW];
+      grow[k] = ov & ~OV_BOX;
+      overflow |= grow[k];
+      if ((ov & (OV_COLUMN | OV_SEARCH)) && fatal < 0) fatal = (int)k;
+      if ((ov & OV_BOX) && box < 0) box = (int)k;
+      if (sys_[k]->flags_host[F_SHAKE_FAIL] && shake < 0) shake = (int)k;
+    }
+    if (!overflow && shake < 0 && box < 0) {
+      for (auto* s : sys_) s->step += n;
+      return FZ_OK;
+    }
+    if (fatal >= 0 || box >= 0 || (shake >= 0 && !overflow)) {
+      FZ_TRY(restore());
+      if (fatal >= 0)
+        return set_error(FZ_ERR_CAPACITY, "neighbour search capacity exceeded in system %d of the set (column "
+                         "occupancy or more than %d J clusters in a search part); use a finer column grid",
+                         fatal, SEARCH_TMP_CAP);
+      if (box >= 0)
+        return set_error(FZ_ERR_STATE, "barostat: the box of system %d shrank below %.2f of its reference size",
+                         box, NPT_MIN_SCALE);
+      return set_error(FZ_ERR_SHAKE, "SHAKE did not converge (system %d of the set)", shake);
+    }
+    FZ_TRY(restore(&grow));
+  }
+  return set_error(FZ_ERR_CAPACITY, "neighbour capacity could not be grown");
+}
+
 int Engine::capture(bool ev, hipGraphExec_t* out) {
   hipGraph_t g = nullptr;
   FZ_HIP(hipStreamBeginCapture(L_.stream, hipStreamCaptureModeRelaxed));
@@ -990,9 +1116,19 @@ int Engine::read_status() {
   for (auto* s : sys_) {
     FZ_HIP(hipMemcpyAsync(s->flags_host, s->dv.flags, sizeof(s->flags_host), hipMemcpyDeviceToHost, L_.stream));
     FZ_HIP(hipMemcpyAsync(s->acc_host, s->dv.acc, sizeof(s->acc_host), hipMemcpyDeviceToHost, L_.stream));
+    if (s->dv.npt)
+      FZ_HIP(hipMemcpyAsync(&s->npt_host, s->dv.nst, sizeof(NptState), hipMemcpyDeviceToHost, L_.stream));
   }
   FZ_HIP(hipStreamSynchronize(L_.stream));
+  for (auto* s : sys_) s->npt_sync_host();
   return FZ_OK;
+}
+
+int Engine::npt_reset_virial() {
+  if (!L_.npt) return FZ_OK;
+  launch_npt_wtr(L_);
+  FZ_HIP(hipGetLastError());
+  return read_status();
 }
 
 int Engine::sync() {
@@ -1063,12 +1199,17 @@ int Engine::restore(const std::vector<int>* grow) {
       FZ_HIP(hipMemcpyAsync(u[c], snap_part(s, 6 + c), nb, hipMemcpyDeviceToDevice, L_.stream));
     }
     FZ_HIP(hipMemcpyAsync(s->dv.flags, snap_part(s, 9), F_NSLOT * sizeof(int), hipMemcpyDeviceToDevice, L_.stream));
+    if (s->dv.npt)
+      FZ_HIP(hipMemcpyAsync(s->dv.nst, s->dv.nst + 1, sizeof(NptState), hipMemcpyDeviceToDevice, L_.stream));
     if (grow && (*grow)[k]) FZ_TRY(grow_and_reset((int)k, (*grow)[k]));
     s->need_rebuild = true;
     s->step = s->snap_step;
     s->min_step = s->snap_min_step;
   }
   FZ_TRY(ensure());
+  // barostat: the list is rebuilt at the box of its build, the forces are then
+  // evaluated at the snapshot's box
+  if (L_.npt) launch_npt_apply(L_, true);
   FZ_TRY(forces(true));
   for (auto* s : sys_) {
     const size_t nb = (size_t)s->topo.n * 8;
@@ -1076,11 +1217,20 @@ int Engine::restore(const std::vector<int>* grow) {
     for (int c = 0; c < 3; c++)
       FZ_HIP(hipMemcpyAsync(u[c], snap_part(s, c), nb, hipMemcpyDeviceToDevice, L_.stream));
   }
+  // (the evaluation above moved the state's build mark: the snapshot's again)
+  for (auto* s : sys_)
+    if (s->dv.npt)
+      FZ_HIP(hipMemcpyAsync(s->dv.nst, s->dv.nst + 1, sizeof(NptState), hipMemcpyDeviceToDevice, L_.stream));
+  if (L_.npt) launch_npt_apply(L_, false);
   FZ_TRY(forces(true));
   // the list-build count as at the snapshot (the two evaluations above rebuilt
-  // a list the trajectory already had); the other flags are clear there too
-  for (auto* s : sys_)
+  // a list the trajectory already had); the other flags are clear there too;
+  // the barostat state as at the snapshot (the evaluations moved its build mark)
+  for (auto* s : sys_) {
     FZ_HIP(hipMemcpyAsync(s->dv.flags, snap_part(s, 9), F_NSLOT * sizeof(int), hipMemcpyDeviceToDevice, L_.stream));
+    if (s->dv.npt)
+      FZ_HIP(hipMemcpyAsync(s->dv.nst, s->dv.nst + 1, sizeof(NptState), hipMemcpyDeviceToDevice, L_.stream));
+  }
   return read_status();
 }
 
@@ -1113,6 +1263,7 @@ int Engine::forces(bool ev, bool lite) {
     launch_zero_forces(L_);
     record_forces(ev, lite);
     launch_kinetic(L_);
+    if (L_.npt) launch_npt_mark(L_);
     FZ_HIP(hipGetLastError());
     FZ_TRY(read_status());
     bool again = false;
@@ -1134,6 +1285,7 @@ int Engine::forces(bool ev, bool lite) {
 int Engine::steps(int n) {
   if (n <= 0) return FZ_OK;
   FZ_TRY(ensure());
+  if (L_.npt) return steps_npt(n);
   for (auto* s : sys_)
     if (s->need_rebuild) FZ_TRY(forces(true));
   for (int attempt = 0; attempt < 4; attempt++) {
@@ -1303,7 +1455,7 @@ int Engine::minimize(int nsteps, double max_step) {
     (void)restore();
     return set_error(rc, "%s", msg.c_str());
   }
-  return FZ_OK;
+  return npt_reset_virial();
 }
 
 int Engine::minimize_ref(int nsteps, const fz_min_params& p) {
@@ -1319,7 +1471,7 @@ int Engine::minimize_ref(int nsteps, const fz_min_params& p) {
     (void)restore();
     return set_error(rc, "%s", msg.c_str());
   }
-  return FZ_OK;
+  return npt_reset_virial();
 }
 
 int Engine::phase_times(int nsteps, double* out) {

@@ -96,6 +96,13 @@ struct System {
   int upload_mdyn();
   int set_fixed(const uint8_t* fixed);
   int set_com_restraint(const int32_t* group, const double dist[3], double fc);
+  // Andersen barostat: reference pressure (atm) and piston mass (amu/A^4);
+  // mass <= 0 turns it off (the box stays where it is).  The box at this call
+  // is the reference box L0 (scale 1).
+  int set_barostat(double pref_atm, double mass);
+  NptState npt_host{};  // host copy of the barostat state, read after every step call
+  // the host descriptor's box fields (and par.box) from the state
+  void npt_sync_host();
   int download_state(double* xyz, double* v, uint64_t* u);
   int energies(fz_energies* e);
 };
@@ -129,7 +136,10 @@ class Engine {
   // fastest, and restores the systems' state.  out: {width factor, fft lines,
   // pair grid, best ms per step}.
   int autotune(int nsteps, double* out);
-  int read_status();                   // flags + accumulators of every system -> host
+  int read_status();                   // flags + accumulators (+ barostat state) of every system -> host
+  // barostat members: the piston's virial trace from the last stand-alone
+  // evaluation (after a change of positions outside dynamics)
+  int npt_reset_virial();
   int sync();
   hipStream_t stream() const { return L_.stream; }
   Launch& launch() { return L_; }
@@ -141,7 +151,7 @@ class Engine {
   Launch L_;
   SysDev* d_sys_ = nullptr;
   std::vector<int*> d_pre_;
-  hipGraphExec_t g_plain_ = nullptr, g_ev_ = nullptr;
+  hipGraphExec_t g_plain_ = nullptr, g_ev_ = nullptr, g_npt_ = nullptr;
   // The force evaluation forks after the list build: the pair kernel on the
   // main stream, bonded terms and the PME chain on two side streams (default
   // priority), joined before the second half kick.  The PME branch forks
@@ -159,6 +169,9 @@ class Engine {
   int make_work(Work& w, const std::vector<int>& blocks);
   int capture(bool ev, hipGraphExec_t* out);
   void record_step(bool ev);
+  void record_npt_step();
+  int steps_npt(int n);
+  int capture_npt(hipGraphExec_t* out);
   void record_forces(bool ev, bool lite = false, bool check = true);
   int grow_and_reset(int s, int bits);
   int realloc_neighbor(int s, long long pairs, int clus);

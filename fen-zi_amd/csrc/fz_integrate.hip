@@ -474,6 +474,380 @@ void launch_step_second(const Launch& L) {
   else hipLaunchKernelGGL(k_step_second<false>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
 }
 
+// ---- Andersen barostat (NPT) -----------------------------------------------------
+// Isotropic piston on the volume, V'' = (P - p_ref) / M (the piston of the
+// reference's reduce_virial, src/cell_based_kernels.cu:403-600; its NPT step is
+// a stub, src/npt_kernels.cu:26-33).  Scaling is molecular for the constraint
+// clusters: a box change from scale l to l' moves each cluster's centre of mass
+// and each free atom with the box (scaled coordinates fixed) and keeps a
+// cluster's internal geometry (its fixed-point offsets from the centre of mass
+// scale by l / l'), so constraints are never strained; centre-of-mass and
+// free-atom velocities (peculiar) scale by l / l'.  The piston sees the
+// molecular pressure (2 KE_mol + W_mol) / 3V: KE_mol of the cluster centres and
+// free atoms, W_mol = tr sum r F minus sum_c sum_{i in c} (r_i - R_c) . F_i
+// (constraint forces are internal to a cluster and drop out).  One step, h = dt/2:
+//   k_npt_a  Pi += h (W_mol / 3V - p_ref); V1 = V + h Pi / M
+//   k_npt_b  kick; box l -> l1; drift at box l1; KE_mol
+//   k_npt_c  Pi += dt 2 KE_mol / 3V1; V2 = V1 + h Pi / M; box fields and PME tables at l2
+//   k_npt_d  box l1 -> l2; SHAKE against the cluster geometry before the drift;
+//            slot copies; list-rebuild test
+//   forces; k_npt_intra (W_mol's cluster part); k_npt_g (Pi += h (W_mol / 3V2 - p_ref));
+//   k_step_second (kick, RATTLE)
+// Without constraint clusters each piece is the exact flow of part of the
+// extended Hamiltonian KE + U + p_ref V + Pi^2 / 2M (symplectic, time reversible).
+
+__global__ void k_npt_a(const SysDev* __restrict__ S, int nsys) {
+  const SysDev& d = S[blockIdx.x];
+  if (threadIdx.x != 0 || !d.npt) return;
+  NptState& st = *d.nst;
+  const double h = 0.5 * d.dt;
+  const double Pi = st.Pi + h * (st.Wtr / (3.0 * st.V) - d.npt_pref);
+  const double V1 = st.V + h * Pi / d.npt_mass;
+  const double ell1 = cbrt(V1 / d.npt_V0);
+  st.Pi = Pi;
+  st.V1 = V1;
+  st.ell0 = st.ell;
+  st.ell1 = ell1;
+  st.a1 = st.ell / ell1;
+  st.ke1 = 0ull;
+  st.wintra = 0ull;
+}
+
+// A box change by f = l_old / l_new of one cluster held in registers (m: masses,
+// u: fixed-point positions, v: velocities): offsets from the centre of mass
+// scale by f (their length in A stays), the centre-of-mass velocity by f.
+// Returns the cluster's centre-of-mass kinetic energy after the change.
+__device__ __forceinline__ double npt_cluster_box(int na, const double (&m)[4], unsigned long long (&u)[4][3],
+                                                  double (&v)[4][3], double f) {
+  double M = 0.0, off[3] = {0.0, 0.0, 0.0}, P[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const double mk = k < na ? m[k] : 0.0;
+    M += mk;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      off[c] += mk * (double)(long long)(u[k][c] - u[0][c]);
+      P[c] += mk * v[k][c];
+    }
+  }
+  double ke = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    off[c] /= M;
+    const double Vc = P[c] / M;
+    double dl[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) dl[k] = (double)(long long)(u[k][c] - u[0][c]) - off[c];  // all before any moves
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      u[k][c] += (unsigned long long)__double2ll_rn((f - 1.0) * dl[k]);
+      v[k][c] += (f - 1.0) * Vc;
+    }
+    ke += 0.5 * M * (f * Vc) * (f * Vc);
+  }
+  return ke;
+}
+
+// kick, box change l0 -> l1 and drift at box l1 of one cluster or free atom;
+// positions before the step into o; clears the step's energy accumulators
+template <bool MIXED>
+__global__ __launch_bounds__(128) void k_npt_b(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int lb = blockIdx.x - block_base(pre, nsys, s_);
+  if (lb == 0 && threadIdx.x < A_NSLOT) d.acc[threadIdx.x] = 0ull;
+  const int t = lb * 128 + threadIdx.x;
+  double ke[1] = {0.0};
+  unsigned long long* U[3] = {d.ux, d.uy, d.uz};
+  unsigned long long* O[3] = {d.ox, d.oy, d.oz};
+  double* Vp[3] = {d.vx, d.vy, d.vz};
+  const double f1 = d.nst->a1, ell1 = d.nst->ell1, h0 = 0.5 * d.dt;
+  double isc1[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) isc1[c] = 18446744073709551616.0 / (ell1 * d.npt_L0[c]);
+  if (t < d.n_ccl) {
+    const CCl k = load_ccl(d, t);
+    double m[4], v[4][3];
+    unsigned long long u[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int a = k.at[j] >= 0 ? k.at[j] : k.at[0];
+      const int sl = d.atom_slot[a];
+      m[j] = 1.0 / d.minv[a];
+      const double hm = h0 * d.mdyn[a];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        u[j][c] = U[c][a];
+        v[j][c] = fma(hm, take_force<MIXED>(d, sl, c, false), Vp[c][a]);
+      }
+    }
+    ke[0] = npt_cluster_box(k.na, m, u, v, f1);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int a = k.at[j];
+      if (a < 0) continue;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        O[c][a] = U[c][a];
+        U[c][a] = u[j][c] + (unsigned long long)__double2ll_rn(d.dt * v[j][c] * isc1[c]);
+        Vp[c][a] = v[j][c];
+      }
+    }
+  } else if (t < d.n_ccl + d.n_free) {
+    const int i = d.free_atoms[t - d.n_ccl];
+    const int sl = d.atom_slot[i];
+    const double hm = h0 * d.mdyn[i];
+    double v2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double v = fma(hm, take_force<MIXED>(d, sl, c, false), Vp[c][i]) * f1;
+      const unsigned long long u = U[c][i];
+      O[c][i] = u;
+      U[c][i] = u + (unsigned long long)__double2ll_rn(d.dt * v * isc1[c]);
+      Vp[c][i] = v;
+      v2 += v * v;
+    }
+    ke[0] = 0.5 / d.minv[i] * v2;
+  }
+  const int slot[1] = {0};
+  block_acc_fixed<1>(ke, &d.nst->ke1, slot);
+}
+
+// PME per-axis tables m_d = j' / L_d and e^{-pi^2 m_d^2 / beta^2} at box scale ell
+__device__ __forceinline__ void npt_pme_tables(const SysDev& d, double ell) {
+  const int KS = d.K[0] + d.K[1] + d.K[2];
+  for (int t = threadIdx.x; t < KS; t += blockDim.x) {
+    const int ax = t < d.K[0] ? 0 : (t < d.K[0] + d.K[1] ? 1 : 2);
+    const int j = t - (ax == 0 ? 0 : (ax == 1 ? d.K[0] : d.K[0] + d.K[1]));
+    const int K = d.K[ax];
+    const int jj = j <= K / 2 ? j : j - K;
+    const double m = jj / (ell * d.npt_L0[ax]);
+    d.pme_m[t] = m;
+    d.pme_ex[t] = exp(-PI_D * PI_D * m * m / (d.beta * d.beta));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_npt_c(SysDev* __restrict__ S, int nsys) {
+  SysDev& d = S[blockIdx.x];
+  if (!d.npt) return;
+  NptState& st = *d.nst;
+  const double h = 0.5 * d.dt;
+  const double Pi = st.Pi + d.dt * 2.0 * from_fixed_e(st.ke1) / (3.0 * st.V1);
+  double V2 = st.V1 + h * Pi / d.npt_mass;
+  double ell2 = cbrt(V2 / d.npt_V0);
+  // below the smallest scale the search ranges cover: flagged (the call fails
+  // and is restored), and the step goes on at the old box
+  const bool shrink = !(ell2 >= NPT_MIN_SCALE);
+  if (shrink) { V2 = st.V; ell2 = st.ell; }
+  const double ell1 = st.ell1, ell_b = st.ell_b;
+  npt_pme_tables(d, ell2);
+  __syncthreads();  // every thread has read the state before it changes
+  if (threadIdx.x == 0) {
+    st.Pi = Pi;
+    st.V = V2;
+    st.a2 = ell1 / ell2;
+    st.ell = ell2;
+    npt_box_fields(d, ell2, ell_b);
+    if (shrink) d.flags[F_OVERFLOW] |= OV_BOX;
+  }
+}
+
+// descriptor fields and PME tables at the state's box, or at the box of the
+// last list build (build_scale: a restore rebuilds that list there)
+__global__ __launch_bounds__(256) void k_npt_apply(SysDev* __restrict__ S, int nsys, int build_scale) {
+  SysDev& d = S[blockIdx.x];
+  if (!d.npt) return;
+  const double ell_b = d.nst->ell_b, ell = build_scale ? ell_b : d.nst->ell;
+  npt_pme_tables(d, ell);
+  if (threadIdx.x == 0) npt_box_fields(d, ell, ell_b);
+}
+
+// a list build since the state last looked moves the build scale to the
+// current box (stand-alone evaluations)
+__global__ void k_npt_mark(const SysDev* __restrict__ S, int nsys) {
+  const SysDev& d = S[blockIdx.x];
+  if (threadIdx.x != 0 || !d.npt) return;
+  NptState& st = *d.nst;
+  if (d.flags[F_NBUILD] != st.nbuild) {
+    st.nbuild = d.flags[F_NBUILD];
+    st.ell_b = st.ell;
+  }
+}
+
+// box change l1 -> l2, SHAKE at the new box against the cluster geometry
+// before the drift (positions o at box l0), slot copies, force clearing and the
+// list-rebuild test of one constraint cluster (cf. step_first_ccl)
+template <bool MIXED>
+__device__ __forceinline__ bool npt_d_ccl(const SysDev& d, const CCl& k, double f2, double ell0) {
+  unsigned long long* U[3] = {d.ux, d.uy, d.uz};
+  const unsigned long long* O[3] = {d.ox, d.oy, d.oz};
+  const unsigned long long* R[3] = {d.rx, d.ry, d.rz};
+  double* Vp[3] = {d.vx, d.vy, d.vz};
+  int sl[4];
+  double mi[4], m[4], v[4][3];
+  unsigned long long u[4][3], o[4][3];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int a = k.at[j] >= 0 ? k.at[j] : k.at[0];
+    sl[j] = d.atom_slot[a];
+    mi[j] = k.at[j] >= 0 ? d.mdyn[a] : 0.0;
+    m[j] = 1.0 / d.minv[a];
+#pragma unroll
+    for (int c = 0; c < 3; c++) { v[j][c] = Vp[c][a]; u[j][c] = U[c][a]; o[j][c] = O[c][a]; }
+  }
+  (void)npt_cluster_box(k.na, m, u, v, f2);
+  double sc0[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) sc0[c] = ell0 * d.npt_L0[c] / 18446744073709551616.0;
+  V3 rn[4], ro[4], dr[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    rn[j] = V3{fx_d(u[0][0], u[j][0], d.sc[0]), fx_d(u[0][1], u[j][1], d.sc[1]), fx_d(u[0][2], u[j][2], d.sc[2])};
+    ro[j] = V3{fx_d(o[0][0], o[j][0], sc0[0]), fx_d(o[0][1], o[j][1], sc0[1]), fx_d(o[0][2], o[j][2], sc0[2])};
+  }
+  if (!shake_core(d, k, rn, ro, mi, dr)) d.flags[F_SHAKE_FAIL] = 1;
+  const double idt = 1.0 / d.dt;
+  bool move = false;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int a = k.at[j];
+    if (a < 0) continue;
+    const double e[3] = {dr[j].x, dr[j].y, dr[j].z};
+    double r2 = 0.0;
+    unsigned long long un3[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const unsigned long long un = u[j][c] + (unsigned long long)__double2ll_rn(e[c] * d.isc[c]);
+      un3[c] = un;
+      U[c][a] = un;
+      Vp[c][a] = v[j][c] + e[c] * idt;
+      const double x = fx_d(R[c][a], un, d.sc[c]);
+      r2 = c == 0 ? x * x : r2 + x * x;
+    }
+    move = move || r2 > d.skin2;
+    store_slot_pos(d, sl[j], un3[0], un3[1], un3[2], d.q[a]);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if (MIXED) ((unsigned long long*)d.Fs)[(size_t)c * d.n_slots_cap + sl[j]] = 0ull;
+      else ((double*)d.Fs)[(size_t)c * d.n_slots_cap + sl[j]] = 0.0;
+    }
+  }
+  return move;
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(128) void k_npt_d(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int t = (blockIdx.x - block_base(pre, nsys, s_)) * 128 + threadIdx.x;
+  const double f2 = d.nst->a2, ell0 = d.nst->ell0;
+  bool move = false;
+  if (t < d.n_ccl) {
+    move = npt_d_ccl<MIXED>(d, load_ccl(d, t), f2, ell0);
+  } else if (t < d.n_ccl + d.n_free) {
+    const int i = d.free_atoms[t - d.n_ccl];
+    const int sl = d.atom_slot[i];
+    d.vx[i] *= f2; d.vy[i] *= f2; d.vz[i] *= f2;
+    store_slot_pos(d, sl, d.ux[i], d.uy[i], d.uz[i], d.q[i]);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if (MIXED) ((unsigned long long*)d.Fs)[(size_t)c * d.n_slots_cap + sl] = 0ull;
+      else ((double*)d.Fs)[(size_t)c * d.n_slots_cap + sl] = 0.0;
+    }
+    move = moved_atom(d, i);
+  }
+  if (__any(move) && (threadIdx.x & 63) == 0) d.flags[F_REBUILD] = 1;
+}
+
+// the clusters' internal virial sum_{i in c} (r_i - R_c) . F_i at the current
+// positions and forces, into the state's wintra (zeroed at the step's start)
+template <bool MIXED>
+__global__ __launch_bounds__(128) void k_npt_intra(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int t = (blockIdx.x - block_base(pre, nsys, s_)) * 128 + threadIdx.x;
+  double w[1] = {0.0};
+  if (t < d.n_ccl) {
+    const CCl k = load_ccl(d, t);
+    double m[4], f[4][3], M = 0.0, Rc[3] = {0.0, 0.0, 0.0};
+    V3 r[4];
+    load_rel(d, k, d.ux, d.uy, d.uz, r);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int a = k.at[j] >= 0 ? k.at[j] : k.at[0];
+      const int sl = d.atom_slot[a];
+      m[j] = k.at[j] >= 0 ? 1.0 / d.minv[a] : 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) f[j][c] = take_force<MIXED>(d, sl, c, false);
+      M += m[j];
+      Rc[0] += m[j] * r[j].x; Rc[1] += m[j] * r[j].y; Rc[2] += m[j] * r[j].z;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) Rc[c] /= M;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (k.at[j] >= 0) w[0] += (r[j].x - Rc[0]) * f[j][0] + (r[j].y - Rc[1]) * f[j][1] + (r[j].z - Rc[2]) * f[j][2];
+  }
+  const int slot[1] = {0};
+  block_acc_fixed<1>(w, &d.nst->wintra, slot);
+}
+
+// piston half kick with the molecular virial trace of the step's forces; a list
+// build in this step moves the build scale to the current box
+__global__ void k_npt_g(const SysDev* __restrict__ S, int nsys) {
+  const SysDev& d = S[blockIdx.x];
+  if (threadIdx.x != 0 || !d.npt) return;
+  NptState& st = *d.nst;
+  const double W = from_fixed_e(d.acc[A_VIR]) + from_fixed_e(d.acc[A_VIR + 4]) + from_fixed_e(d.acc[A_VIR + 8]) -
+                   from_fixed_e(st.wintra);
+  st.Wtr = W;
+  st.Pi = st.Pi + 0.5 * d.dt * (W / (3.0 * st.V) - d.npt_pref);
+  if (d.flags[F_NBUILD] != st.nbuild) {
+    st.nbuild = d.flags[F_NBUILD];
+    st.ell_b = st.ell;
+  }
+}
+
+// the molecular virial of a stand-alone evaluation (forces only) as the piston's W
+__global__ void k_npt_zero_intra(const SysDev* __restrict__ S, int nsys) {
+  const SysDev& d = S[blockIdx.x];
+  if (threadIdx.x == 0 && d.npt) d.nst->wintra = 0ull;
+}
+__global__ void k_npt_wtr(const SysDev* __restrict__ S, int nsys) {
+  const SysDev& d = S[blockIdx.x];
+  if (threadIdx.x != 0 || !d.npt) return;
+  d.nst->Wtr = from_fixed_e(d.acc[A_VIR]) + from_fixed_e(d.acc[A_VIR + 4]) + from_fixed_e(d.acc[A_VIR + 8]) -
+               from_fixed_e(d.nst->wintra);
+}
+
+void launch_npt_a(const Launch& L) { hipLaunchKernelGGL(k_npt_a, dim3(L.nsys), dim3(64), 0, L.stream, L.d_sys, L.nsys); }
+void launch_npt_b(const Launch& L) {
+  if (L.mixed) hipLaunchKernelGGL(k_npt_b<true>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+  else hipLaunchKernelGGL(k_npt_b<false>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+}
+void launch_npt_intra(const Launch& L) {
+  if (L.mixed) hipLaunchKernelGGL(k_npt_intra<true>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+  else hipLaunchKernelGGL(k_npt_intra<false>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+}
+void launch_npt_c(const Launch& L) {
+  hipLaunchKernelGGL(k_npt_c, dim3(L.nsys), dim3(256), 0, L.stream, const_cast<SysDev*>(L.d_sys), L.nsys);
+}
+void launch_npt_apply(const Launch& L, bool build_scale) {
+  hipLaunchKernelGGL(k_npt_apply, dim3(L.nsys), dim3(256), 0, L.stream, const_cast<SysDev*>(L.d_sys), L.nsys,
+                     build_scale ? 1 : 0);
+}
+void launch_npt_mark(const Launch& L) { hipLaunchKernelGGL(k_npt_mark, dim3(L.nsys), dim3(64), 0, L.stream, L.d_sys, L.nsys); }
+void launch_npt_wtr(const Launch& L) {
+  hipLaunchKernelGGL(k_npt_zero_intra, dim3(L.nsys), dim3(64), 0, L.stream, L.d_sys, L.nsys);
+  launch_npt_intra(L);
+  hipLaunchKernelGGL(k_npt_wtr, dim3(L.nsys), dim3(64), 0, L.stream, L.d_sys, L.nsys);
+}
+void launch_npt_d(const Launch& L) {
+  if (L.mixed) hipLaunchKernelGGL(k_npt_d<true>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+  else hipLaunchKernelGGL(k_npt_d<false>, dim3(L.w_step.total), dim3(128), 0, L.stream, L.d_sys, L.nsys, L.w_step.pre);
+}
+void launch_npt_g(const Launch& L) { hipLaunchKernelGGL(k_npt_g, dim3(L.nsys), dim3(64), 0, L.stream, L.d_sys, L.nsys); }
+
 // ---- kinetic energy, velocities ---------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_kinetic(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
@@ -707,6 +1081,7 @@ __global__ __launch_bounds__(256) void k_snapshot(const SysDev* __restrict__ S, 
     }
     ((int*)(sn + (size_t)9 * n))[threadIdx.x] = f;
   }
+  if (lb == 0 && d.npt && threadIdx.x == 0) d.nst[1] = d.nst[0];
   const int i = lb * 256 + threadIdx.x;
   if (i >= n) return;
   const unsigned long long* src[9] = {d.ux, d.uy, d.uz, (const unsigned long long*)d.vx, (const unsigned long long*)d.vy,

@@ -40,6 +40,7 @@ struct Launch {
   int com_any = 0;           // some system has a COM restraint
   int spread_bricks = 1;     // PME spread by bricks (1) or by global atomics (0)
   int search_cap = 1;  // cap on the pair-search waves per I cluster (1, 2 or 4)
+  int npt = 0;         // every system has the Andersen barostat on
 };
 
 // check: run the displacement test first (the fused step kernel does it otherwise)
@@ -65,5 +66,17 @@ void launch_clear_shake_flag(const Launch& L);
 void launch_snapshot(const Launch& L, bool clear_shake);
 void launch_zero_forces(const Launch& L);
 void launch_restraint(const Launch& L);  // COM restraint forces (systems with com_on)
+// Andersen barostat step phases (every system of the launch has the barostat on)
+void launch_npt_a(const Launch& L);
+void launch_npt_b(const Launch& L);
+void launch_npt_c(const Launch& L);
+void launch_npt_d(const Launch& L);
+void launch_npt_intra(const Launch& L);  // the clusters' internal virial (after the forces)
+void launch_npt_g(const Launch& L);
+// box fields and PME tables from the state (after a restore); build_scale: at the
+// box of the last list build
+void launch_npt_apply(const Launch& L, bool build_scale);
+void launch_npt_mark(const Launch& L);  // a list build since: build scale = current box
+void launch_npt_wtr(const Launch& L);   // piston virial from a stand-alone evaluation
 
 }  // namespace fz

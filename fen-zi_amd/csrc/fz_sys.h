@@ -3,6 +3,7 @@
 // a batch is an array of them plus per-launch block-prefix tables, so every
 // kernel runs unchanged on one system or on many.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace fz {
@@ -24,7 +25,26 @@ enum AccSlot {
 enum FlagSlot { F_REBUILD = 0, F_OVERFLOW, F_SHAKE_FAIL, F_NBUILD, F_NSLOT };
 
 enum OverflowBits {
-  OV_SLOTS = 1, OV_COLUMN = 2, OV_SEARCH = 4, OV_PAIRS = 8
+  OV_SLOTS = 1, OV_COLUMN = 2, OV_SEARCH = 4, OV_PAIRS = 8,
+  OV_BOX = 16  // barostat: the box shrank below NPT_MIN_SCALE of its size at the barostat's start
+};
+
+// Andersen barostat: the smallest box scale the column search ranges are sized for
+constexpr double NPT_MIN_SCALE = 0.85;
+
+// Andersen barostat state of one system (device memory).  The box is
+// ell * L0, V = ell^3 V0; Pi = M dV/dt; Wtr the molecular virial at the current
+// positions (tr sum r F minus each constraint cluster's internal part); ell_b
+// the box scale at the last list build (F_NBUILD then nbuild).  ell0, ell1, a1,
+// a2, V1, ke1 and wintra carry one step's intermediate values between its
+// kernels.
+struct NptState {
+  double V, Pi, ell, ell_b, Wtr;
+  double V1, ell0, ell1, a1, a2;
+  unsigned long long ke1;     // molecular kinetic energy after the first kick, fixed point 2^28
+  unsigned long long wintra;  // sum over clusters of (r_i - R_c) . F_i, fixed point 2^28
+  int nbuild;
+  int pad;
 };
 
 struct SysDev {
@@ -160,6 +180,36 @@ struct SysDev {
   // snapshot of a call's starting state: blocks of n_atoms words ux, uy, uz,
   // vx, vy, vz (bits), rx, ry, rz, then the F_NSLOT status flags
   unsigned long long *snap;
+  // ---- Andersen barostat (npt != 0) ------------------------------------
+  int npt;
+  double npt_pref;       // reference pressure, kcal/mol/A^3
+  double npt_mass;       // piston mass, amu/A^4
+  double npt_V0, npt_L0[3];
+  double rl, rc;         // pair-list distance and cutoff (A)
+  NptState *nst;         // [2]: live state, snapshot
 };
+
+// Box-dependent descriptor fields of a barostat system at box scale ell:
+// box, fixed-point scales, the pair search's z window, the pair-list half-skin
+// and the PME prefactor.  A pair beyond rl at the list's build scale ell_b is
+// beyond rl ell / ell_b now, so the list holds every pair within rc while no
+// atom has moved more than half of rl ell / ell_b - rc (the rebuild test).
+// Host and device run the same arithmetic (no contraction), so both copies of
+// the descriptor agree bit for bit; at ell = ell_b = 1 the fields are those of
+// the system set-up.
+__host__ __device__ inline void npt_box_fields(SysDev& d, double ell, double ell_b) {
+#pragma clang fp contract(off)
+  const double two64 = 18446744073709551616.0;
+  for (int k = 0; k < 3; k++) {
+    d.L[k] = ell * d.npt_L0[k];
+    d.sc[k] = d.L[k] / two64;
+    d.isc[k] = two64 / d.L[k];
+    d.sc32[k] = (float)(d.L[k] / 4294967296.0);
+  }
+  d.zwin = (unsigned long long)(d.rl / d.sc[2] * (1.0 + 1e-9)) + 64ull;
+  const double hs = 0.5 * (d.rl * (ell / ell_b) - d.rc);
+  d.skin2 = hs > 0.0 ? hs * hs : 0.0;
+  d.pme_pre = 332.0716 / (3.14159265358979323846 * (d.L[0] * d.L[1] * d.L[2]));
+}
 
 }  // namespace fz

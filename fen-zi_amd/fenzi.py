@@ -46,7 +46,8 @@ class Params(C.Structure):
 class Energies(C.Structure):
     _fields_ = [("step", C.c_int64), ("temperature", C.c_double)] + \
                [(f, C.c_double) for f in E_FIELDS] + \
-               [("virial", C.c_double * 9), ("pairs_within_cutoff", C.c_int64), ("list_builds", C.c_int64)]
+               [("virial", C.c_double * 9), ("pairs_within_cutoff", C.c_int64), ("list_builds", C.c_int64),
+                ("volume", C.c_double), ("pressure", C.c_double)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f in E_FIELDS}
@@ -55,6 +56,8 @@ class Energies(C.Structure):
         d["virial"] = np.array(self.virial[:]).reshape(3, 3)
         d["pairs_within_cutoff"] = self.pairs_within_cutoff
         d["list_builds"] = self.list_builds
+        d["volume"] = self.volume
+        d["pressure"] = self.pressure
         return d
 
 
@@ -65,7 +68,7 @@ class RunSettings(C.Structure):
                 ("output_prefix", C.c_char * 256), ("cgfac", C.c_double), ("sdfac", C.c_double),
                 ("cgrate", C.c_double), ("sdrate", C.c_double), ("checkpoint_every", C.c_int64),
                 ("traj_every", C.c_int64), ("print_level", C.c_int32), ("checkpoint_binary", C.c_int32),
-                ("file_prefix", C.c_char * 256)]
+                ("file_prefix", C.c_char * 256), ("pmass", C.c_double), ("pref", C.c_double)]
 
 
 # every symbol include/fenzi.h declares
@@ -81,7 +84,7 @@ EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create
            "fz_minimize_ex", "fz_batch_minimize_ex", "fz_restart_write", "fz_restart_read", "fz_dcd_open",
            "fz_dcd_write_frame", "fz_dcd_close", "fz_set_fixed_atoms", "fz_set_com_restraint",
            "fz_deck_com_restraint", "fz_deck_fixed_atoms", "fz_set_neighbor_capacity",
-           "fz_pme_convolve_noalloc", "fz_pme_convolve_noalloc_bytes"]
+           "fz_pme_convolve_noalloc", "fz_pme_convolve_noalloc_bytes", "fz_set_barostat", "fz_get_barostat"]
 
 MIN_ADAPTIVE_SD, MIN_REF_CG, MIN_REF_SD = 0, 1, 2
 
@@ -142,6 +145,8 @@ def lib():
         L.fz_reassign_velocities.argtypes = [vp, C.c_double, C.c_uint64, C.c_uint64]
         L.fz_compute_forces.argtypes = [vp, _dp, C.POINTER(Energies)]
         L.fz_step.argtypes = [vp, C.c_int]
+        L.fz_set_barostat.argtypes = [vp, C.c_double, C.c_double]
+        L.fz_get_barostat.argtypes = [vp, _dp, _dp]
         L.fz_get_energies.argtypes = [vp, C.POINTER(Energies)]
         L.fz_minimize.argtypes = [vp, C.c_int, C.c_double]
         L.fz_system_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
@@ -395,6 +400,17 @@ class System:
     def set_neighbor_capacity(self, pairs_cap=0, clusters_cap=0):
         """Neighbour capacities (0 keeps one); the list held stays the same."""
         _check(lib().fz_set_neighbor_capacity(self.h, int(pairs_cap), int(clusters_cap)), "fz_set_neighbor_capacity")
+
+    def set_barostat(self, pref_atm, piston_mass):
+        """Andersen barostat (fz_set_barostat): pref in atm, piston mass in
+        amu/A^4; piston_mass <= 0 turns it off."""
+        _check(lib().fz_set_barostat(self.h, float(pref_atm), float(piston_mass)), "fz_set_barostat")
+
+    def barostat(self):
+        """Current box and piston state: dict(box, volume, Pi, Wtr)."""
+        box, st = np.zeros(3), np.zeros(3)
+        _check(lib().fz_get_barostat(self.h, _p(box, C.c_double), _p(st, C.c_double)), "fz_get_barostat")
+        return dict(box=box, volume=st[0], Pi=st[1], Wtr=st[2])
 
     def info(self):
         n, b, p = C.c_int64(), C.c_int64(), C.c_int32()

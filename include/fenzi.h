@@ -100,6 +100,8 @@ typedef struct fz_energies {
                              (src/shake.cu:300-306, there halved and diagonal only) */
   int64_t pairs_within_cutoff; /* non-excluded pairs inside the cutoff, this evaluation */
   int64_t list_builds;    /* neighbour-list builds so far */
+  double volume;          /* box volume (A^3) */
+  double pressure;        /* (2 KE + tr virial) / 3V in atm (reduce_virial, src/cell_based_kernels.cu:586-589) */
 } fz_energies;
 
 /* ---- errors / context ------------------------------------------------- */
@@ -152,6 +154,19 @@ int fz_compute_forces(fz_system *sys, double *forces, fz_energies *e);
  * grow) the system is back in the state it had before the call.  Replaces
  * SingleStep (src/dynamics.cu:623). */
 int fz_step(fz_system *sys, int nsteps);
+/* Andersen barostat, isotropic (pmass / pref keywords, src/prmread.cu:664-670;
+ * the piston of reduce_virial, src/cell_based_kernels.cu:403-600, whose NPT
+ * step the reference leaves unfinished, src/npt_kernels.cu:26-33): the volume
+ * V follows V'' = (P - pref) / M with P = (2 KE + tr W) / 3V, W the virial with
+ * its SHAKE/RATTLE part; positions scale with the box.  pref in atm, piston
+ * mass M in amu/A^4; mass <= 0 turns the barostat off and keeps the current
+ * box.  The box at the call is the reference box (scale 1); fz_step fails
+ * (FZ_ERR_STATE, state restored) if it shrinks below 0.85 of it.  Thermostat:
+ * the caller's velocity reassignment (fz_reassign_velocities). */
+int fz_set_barostat(fz_system *sys, double pref_atm, double piston_mass);
+/* Current box [3] and piston state (nullable) [3]: volume, piston momentum
+ * M dV/dt, virial trace at the current positions. */
+int fz_get_barostat(fz_system *sys, double *box, double *state);
 /* Energies of the last evaluated step (temperature from current velocities). */
 int fz_get_energies(fz_system *sys, fz_energies *e);
 /* Adaptive steepest descent, nsteps iterations: the atom with the largest force
@@ -334,6 +349,8 @@ typedef struct fz_run_settings {
   int32_t print_level;                 /* printlevel: 0 none, 1 energies + checkpoints, 2 + trajectory */
   int32_t checkpoint_binary;           /* checkpointformat: 1 binary (default), 0 ascii */
   char file_prefix[256];               /* outputtrajectory without its extension: <prefix>.rst, <prefix>_dev<d>.dcd */
+  double pmass;                        /* barostat piston mass (amu/A^4; 0 = fixed box) */
+  double pref;                         /* barostat reference pressure (atm, default 1) */
 } fz_run_settings;
 int fz_deck_settings(const fz_deck *deck, fz_run_settings *out);
 /* Section counts of the PSF as read: bonds, angles, dihedrals, impropers. */

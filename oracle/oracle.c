@@ -56,6 +56,7 @@ struct or_sys {
   int32_t *xlist;
   int n_clusters;
   or_cluster *clusters;
+  uint8_t *in_cluster;  /* [N]: 1 for an atom of a constraint cluster */
   int n_cons_total;
   /* fixed atoms: mobile[i] = 0 holds atom i; NULL = all mobile */
   uint8_t *mobile;
@@ -177,7 +178,9 @@ static void build_clusters(or_sys *s) {
   const or_topo *t = &s->t;
   int n = t->n_atoms, nc = t->n_constraints;
   free(s->clusters);
+  free(s->in_cluster);
   s->n_clusters = 0; s->clusters = NULL; s->n_cons_total = 0;
+  s->in_cluster = (uint8_t *)calloc((size_t)(n > 0 ? n : 1), 1);
   if (nc == 0) return;
   int *par = (int *)malloc(sizeof(int) * (size_t)n);
   for (int i = 0; i < n; i++) par[i] = i;
@@ -201,6 +204,7 @@ static void build_clusters(or_sys *s) {
     for (int k = 0; k < cl->natom; k++) { if (cl->atom[k] == a) la = k; if (cl->atom[k] == b) lb = k; }
     if (la < 0) { la = cl->natom; cl->atom[cl->natom++] = a; }
     if (lb < 0) { lb = cl->natom; cl->atom[cl->natom++] = b; }
+    s->in_cluster[a] = s->in_cluster[b] = 1;
     if (cl->natom > 8 || cl->ncons >= 12) { fprintf(stderr, "oracle: constraint cluster too large\n"); abort(); }
     cl->c1[cl->ncons] = la; cl->c2[cl->ncons] = lb;
     cl->d2[cl->ncons] = t->constraint_length[c] * t->constraint_length[c];
@@ -219,7 +223,7 @@ or_sys *or_create(const or_topo *t, const or_params *p) {
 
 void or_destroy(or_sys *s) {
   if (!s) return;
-  free(s->excl); free(s->ew); free(s->p14); free(s->xoff); free(s->xlist); free(s->clusters);
+  free(s->excl); free(s->ew); free(s->p14); free(s->xoff); free(s->xlist); free(s->clusters); free(s->in_cluster);
   free(s->mobile); free(s->com_grp);
   free(s);
 }
@@ -1053,14 +1057,16 @@ static int solve_lin(int n, double A[12][12], double *b) {
 /* SHAKE of one cluster (Newton iteration on the multipliers, src/shake.cu:20-326),
  * converged when max |r^2 - d^2| / (2 d^2) < tol.  Returns the position
  * corrections dr[natom][3]. */
-static int shake_cluster(const or_sys *s, const or_cluster *cl, const uint64_t *uold, const uint64_t *unew,
-                         const double sc[3], double dr[8][3]) {
+/* SHAKE of one cluster: new positions unew (box scale sc) against the
+ * geometry of uold (box scale sco; the same box except under the barostat) */
+static int shake_cluster_ref(const or_sys *s, const or_cluster *cl, const uint64_t *uold, const double sco[3],
+                             const uint64_t *unew, const double sc[3], double dr[8][3]) {
   int na = cl->natom, nc = cl->ncons;
   double rn[8][3], ro[8][3], minv[8];
   int a0 = cl->atom[0];
   for (int k = 0; k < na; k++) {
     fx_disp(unew, a0, cl->atom[k], sc, rn[k]);
-    fx_disp(uold, a0, cl->atom[k], sc, ro[k]);
+    fx_disp(uold, a0, cl->atom[k], sco, ro[k]);
     minv[k] = dyn_minv(s, cl->atom[k]);
     dr[k][0] = dr[k][1] = dr[k][2] = 0.0;
   }
@@ -1094,6 +1100,11 @@ static int shake_cluster(const or_sys *s, const or_cluster *cl, const uint64_t *
     }
   }
   return -1;
+}
+
+static int shake_cluster(const or_sys *s, const or_cluster *cl, const uint64_t *uold, const uint64_t *unew,
+                         const double sc[3], double dr[8][3]) {
+  return shake_cluster_ref(s, cl, uold, sc, unew, sc, dr);
 }
 
 /* RATTLE velocity projection of one cluster (src/shake.cu:330-615): exact solve
@@ -1212,6 +1223,192 @@ int or_step_ex(or_sys *s, uint64_t *u, double *v, int nsteps, double *e_out, dou
   if (vir_out) memcpy(vir_out, vir, sizeof(vir));
   free(f); free(uold);
   return rc;
+}
+
+/* Andersen barostat, isotropic (NPT with the thermostat of the caller's
+ * velocity reassignment).  The piston coordinate is the volume V, with
+ * V'' = (P - p_ref) / M: the same piston the reference's reduce_virial forms
+ * (src/cell_based_kernels.cu:403-600: (2 KE + W) / (3 V) - pRef PU over the
+ * piston mass), whose NPT step is left unfinished there
+ * (src/npt_kernels.cu:26-33).  The box is L = l L0, l = cbrt(V / V0).
+ *
+ * Scaling is molecular for the constraint clusters (Ryckaert & Ciccotti 1983,
+ * Andersen's piston with constrained molecules): the box moves each cluster's
+ * centre of mass and each atom of no cluster, while a cluster's internal
+ * geometry stays as it is, so a box change never strains a constraint.  The
+ * piston is driven by the molecular pressure (2 KE_mol + W_mol) / 3V, where
+ * KE_mol counts the centre-of-mass motion of clusters and the motion of free
+ * atoms, and W_mol = tr sum_i r_i F_i - sum_clusters sum_{i in c}
+ * (r_i - R_c) . F_i over the non-constraint forces (constraint forces are
+ * internal to their cluster and drop out).  Velocities are peculiar velocities;
+ * a box change from l to l' scales a cluster's centre-of-mass velocity (and a
+ * free atom's velocity) by l / l'.  The extended energy
+ * KE + U + p_ref V + Pi^2 / (2M) is conserved.  One step:
+ *   A  Pi += dt/2 (W_mol / 3V - p_ref); V1 = V + dt/2 Pi / M
+ *   B  v += dt/2 F / m; box l -> l1 (centre-of-mass velocities and internal
+ *      offsets as above); u += dt v at box l1; Pi += dt 2 KE_mol / 3V1
+ *   C  V2 = V1 + dt/2 Pi / M; box l1 -> l2
+ *   D  SHAKE at box l2 against the cluster geometry before the drift
+ *   E  forces and virial at (u, l2)
+ *   F  Pi += dt/2 (W_mol / 3V2 - p_ref); v += dt/2 F / m; RATTLE
+ * Without clusters every part is an exact flow of the extended Hamiltonian, so
+ * the step is symplectic and time reversible. */
+static void set_box_scale(or_sys *s, const or_piston *pc, double l) {
+  for (int d = 0; d < 3; d++) s->p.box[d] = l * pc->L0[d];
+}
+
+/* fixed-point offset of atom a from the cluster's first atom, per axis */
+static inline double cl_off(const uint64_t *u, int a0, int a, int d) {
+  return (double)(int64_t)(u[3 * a + d] - u[3 * a0 + d]);
+}
+
+/* A box change by f = l_old / l_new for every cluster: internal fixed-point
+ * offsets from the centre of mass scale by f (their length in A stays), the
+ * centre-of-mass velocity scales by f; free atoms' velocities scale by f
+ * (their scaled positions stay).  With fkin, accumulates the molecular kinetic
+ * energy after the change. */
+static void npt_box_change(const or_sys *s, uint64_t *u, double *v, double f, double *ke_mol) {
+  const int n = s->t.n_atoms;
+  double ke = 0.0;
+  for (int c = 0; c < s->n_clusters; c++) {
+    const or_cluster *cl = &s->clusters[c];
+    const int a0 = cl->atom[0];
+    double M = 0.0, off[3] = {0, 0, 0}, P[3] = {0, 0, 0};
+    for (int k = 0; k < cl->natom; k++) {
+      const int a = cl->atom[k];
+      const double m = s->t.mass[a];
+      M += m;
+      for (int d = 0; d < 3; d++) { off[d] += m * cl_off(u, a0, a, d); P[d] += m * v[3 * a + d]; }
+    }
+    double Vc[3], dl[8][3];
+    for (int d = 0; d < 3; d++) { off[d] /= M; Vc[d] = P[d] / M; }
+    for (int k = 0; k < cl->natom; k++)
+      for (int d = 0; d < 3; d++) dl[k][d] = cl_off(u, a0, cl->atom[k], d) - off[d];
+    for (int k = 0; k < cl->natom; k++) {
+      const int a = cl->atom[k];
+      for (int d = 0; d < 3; d++) {
+        u[3 * a + d] += (uint64_t)(int64_t)llrint((f - 1.0) * dl[k][d]);
+        v[3 * a + d] += (f - 1.0) * Vc[d];
+      }
+    }
+    for (int d = 0; d < 3; d++) ke += 0.5 * M * (f * Vc[d]) * (f * Vc[d]);
+  }
+  for (int i = 0; i < n; i++) {
+    if (s->in_cluster[i]) continue;
+    for (int d = 0; d < 3; d++) {
+      v[3 * i + d] *= f;
+      ke += 0.5 * s->t.mass[i] * v[3 * i + d] * v[3 * i + d];
+    }
+  }
+  if (ke_mol) *ke_mol = ke;
+}
+
+/* sum over clusters of sum_{i in c} (r_i - R_c) . F_i (forces f at positions u, box as set) */
+static double npt_intra_virial(const or_sys *s, const uint64_t *u, const double *f) {
+  const double sc[3] = {s->p.box[0] * TWO_POW_M64, s->p.box[1] * TWO_POW_M64, s->p.box[2] * TWO_POW_M64};
+  double w = 0.0;
+  for (int c = 0; c < s->n_clusters; c++) {
+    const or_cluster *cl = &s->clusters[c];
+    const int a0 = cl->atom[0];
+    double M = 0.0, R[3] = {0, 0, 0};
+    double r[8][3];
+    for (int k = 0; k < cl->natom; k++) {
+      const int a = cl->atom[k];
+      const double m = s->t.mass[a];
+      fx_disp(u, a0, a, sc, r[k]);
+      M += m;
+      for (int d = 0; d < 3; d++) R[d] += m * r[k][d];
+    }
+    for (int d = 0; d < 3; d++) R[d] /= M;
+    for (int k = 0; k < cl->natom; k++)
+      for (int d = 0; d < 3; d++) w += (r[k][d] - R[d]) * f[3 * cl->atom[k] + d];
+  }
+  return w;
+}
+
+int or_step_npt(or_sys *s, or_piston *pc, uint64_t *u, double *v, int nsteps, double *e_out, double *vir_out) {
+  int n = s->t.n_atoms;
+  double dt = OR_TU * s->p.dt_ps;
+  const double V0 = pc->L0[0] * pc->L0[1] * pc->L0[2], M = pc->mass;
+  double *f = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  uint64_t *uold = (uint64_t *)malloc(sizeof(uint64_t) * 3 * (size_t)n);
+  double e[OR_NE], vir[9];
+  double V = pc->V, l = cbrt(V / V0);
+  set_box_scale(s, pc, l);
+  or_forces(s, u, f, e, vir, OR_ALL);
+  int rc = 0;
+  for (int st = 0; st < nsteps; st++) {
+    /* A */
+    pc->Pi += 0.5 * dt * (pc->Wtr / (3.0 * V) - pc->p_ref);
+    const double V1 = V + 0.5 * dt * pc->Pi / M, l1 = cbrt(V1 / V0);
+    /* B: kick, box l -> l1, drift at l1, piston kick with the molecular KE */
+    for (int i = 0; i < n; i++) {
+      double hm = 0.5 * dt * dyn_minv(s, i);
+      for (int d = 0; d < 3; d++) v[3 * i + d] += hm * f[3 * i + d];
+    }
+    const double sc0[3] = {s->p.box[0] * TWO_POW_M64, s->p.box[1] * TWO_POW_M64, s->p.box[2] * TWO_POW_M64};
+    memcpy(uold, u, sizeof(uint64_t) * 3 * (size_t)n);
+    double ke1 = 0.0;
+    npt_box_change(s, u, v, l / l1, &ke1);
+    set_box_scale(s, pc, l1);
+    double sc1[3] = {s->p.box[0] * TWO_POW_M64, s->p.box[1] * TWO_POW_M64, s->p.box[2] * TWO_POW_M64};
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++) u[3 * i + d] += (uint64_t)(int64_t)llrint(dt * v[3 * i + d] / sc1[d]);
+    pc->Pi += dt * 2.0 * ke1 / (3.0 * V1);
+    /* C */
+    const double V2 = V1 + 0.5 * dt * pc->Pi / M, l2 = cbrt(V2 / V0);
+    npt_box_change(s, u, v, l1 / l2, NULL);
+    set_box_scale(s, pc, l2);
+    double sc2[3] = {s->p.box[0] * TWO_POW_M64, s->p.box[1] * TWO_POW_M64, s->p.box[2] * TWO_POW_M64};
+    /* D: the cluster geometry before the drift is the old positions at the old box */
+    for (int c = 0; c < s->n_clusters; c++) {
+      const or_cluster *cl = &s->clusters[c];
+      double dr[8][3];
+      if (shake_cluster_ref(s, cl, uold, sc0, u, sc2, dr) < 0) rc = -1;
+      for (int k = 0; k < cl->natom; k++)
+        for (int d = 0; d < 3; d++) {
+          int a = cl->atom[k];
+          u[3 * a + d] += (uint64_t)(int64_t)llrint(dr[k][d] / sc2[d]);
+          v[3 * a + d] += dr[k][d] / dt;
+        }
+    }
+    /* E */
+    or_forces(s, u, f, e, vir, OR_ALL);
+    /* F */
+    pc->Wtr = vir[0] + vir[4] + vir[8] - npt_intra_virial(s, u, f);
+    pc->Pi += 0.5 * dt * (pc->Wtr / (3.0 * V2) - pc->p_ref);
+    for (int i = 0; i < n; i++) {
+      double hm = 0.5 * dt * dyn_minv(s, i);
+      for (int d = 0; d < 3; d++) v[3 * i + d] += hm * f[3 * i + d];
+    }
+    for (int c = 0; c < s->n_clusters; c++) rattle_cluster(s, &s->clusters[c], u, sc2, v, dt, vir);
+    V = V2;
+    l = l2;
+  }
+  pc->V = V;
+  if (e_out) memcpy(e_out, e, sizeof(e));
+  if (vir_out) memcpy(vir_out, vir, sizeof(vir));
+  free(f); free(uold);
+  return rc;
+}
+
+/* The piston's molecular virial W_mol at positions u (box as set). */
+double or_virial_trace(or_sys *s, const uint64_t *u) {
+  int n = s->t.n_atoms;
+  double *f = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  double e[OR_NE], vir[9];
+  or_forces(s, u, f, e, vir, OR_ALL);
+  const double w = vir[0] + vir[4] + vir[8] - npt_intra_virial(s, u, f);
+  free(f);
+  return w;
+}
+
+void or_set_box(or_sys *s, const double box[3]) {
+  for (int d = 0; d < 3; d++) s->p.box[d] = box[d];
+}
+
+void or_get_box(const or_sys *s, double box[3]) {
+  for (int d = 0; d < 3; d++) box[d] = s->p.box[d];
 }
 
 /* SHAKE of every constraint cluster of positions u, with uold as the

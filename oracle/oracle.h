@@ -106,6 +106,22 @@ int or_step(or_sys *s, uint64_t *u, double *v, int nsteps);
  * step (nullable): the force virial at the final positions plus the
  * constraint virial of the final RATTLE projection, (2/dt) sum_c b_c r_c (x) r_c. */
 int or_step_ex(or_sys *s, uint64_t *u, double *v, int nsteps, double *e_out, double *vir_out);
+/* Andersen barostat (isotropic): piston state and parameters.  L0: the box at
+ * V0 = L0[0] L0[1] L0[2]; the box is cbrt(V / V0) L0.  p_ref in kcal/mol/A^3
+ * (pref [atm] x 1.4586e-5, src/defs.in.h:105), mass in amu/A^4 (pmass,
+ * src/prmread.cu:664-670); V in A^3; Pi = M dV/dt; Wtr the virial trace at the
+ * current positions (constraint part of the last RATTLE included). */
+typedef struct or_piston {
+  double L0[3];
+  double p_ref, mass;
+  double V, Pi, Wtr;
+} or_piston;
+/* nsteps NPT steps (see oracle.c); the oracle's box follows V and is left at
+ * the final box. */
+int or_step_npt(or_sys *s, or_piston *pc, uint64_t *u, double *v, int nsteps, double *e_out, double *vir_out);
+double or_virial_trace(or_sys *s, const uint64_t *u);
+void or_set_box(or_sys *s, const double box[3]);
+void or_get_box(const or_sys *s, double box[3]);
 /* SHAKE of positions u against reference positions uold (minimiser moves). */
 int or_shake_positions(or_sys *s, const uint64_t *uold, uint64_t *u);
 /* Maxwell velocities (Philox4x32-10 + Box-Muller), momentum removal, RATTLE projection. */

@@ -77,6 +77,12 @@ def lib():
         L.or_step.restype = C.c_int
         L.or_step_ex.argtypes = [C.c_void_p, _up, _dp, C.c_int, _dp, _dp]
         L.or_step_ex.restype = C.c_int
+        L.or_step_npt.argtypes = [C.c_void_p, C.POINTER(Piston), _up, _dp, C.c_int, _dp, _dp]
+        L.or_step_npt.restype = C.c_int
+        L.or_virial_trace.argtypes = [C.c_void_p, _up]
+        L.or_virial_trace.restype = C.c_double
+        L.or_set_box.argtypes = [C.c_void_p, _dp]
+        L.or_get_box.argtypes = [C.c_void_p, _dp]
         L.or_shake_positions.argtypes = [C.c_void_p, _up, _up]
         L.or_shake_positions.restype = C.c_int
         L.or_maxwell.argtypes = [C.c_void_p, _up, C.c_double, C.c_uint64, C.c_uint64, _dp]
@@ -98,6 +104,17 @@ def lib():
 
 def _p(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
+
+
+# atm -> kcal/mol/A^3 (PU, src/defs.in.h:105)
+PU = 1.4586e-5
+
+
+class Piston(C.Structure):
+    """Andersen piston (or_piston): reference box L0, p_ref [kcal/mol/A^3],
+    mass [amu/A^4], volume V, momentum Pi = M dV/dt, virial trace Wtr."""
+    _fields_ = [("L0", C.c_double * 3), ("p_ref", C.c_double), ("mass", C.c_double), ("V", C.c_double),
+                ("Pi", C.c_double), ("Wtr", C.c_double)]
 
 
 class Oracle:
@@ -224,6 +241,34 @@ class Oracle:
         w = np.zeros(9)
         rc = lib().or_step_ex(self.h, _p(u, C.c_uint64), _p(v, C.c_double), int(nsteps), _p(e, C.c_double),
                               _p(w, C.c_double))
+        return u, v, dict(zip(E_NAMES, e)), w.reshape(3, 3), rc
+
+    def set_box(self, box):
+        self.box = np.array(box, dtype=np.float64)
+        lib().or_set_box(self.h, _p(self.box, C.c_double))
+
+    def piston(self, u, pref_atm, mass):
+        """A piston at rest at the current box, Wtr the virial trace at u."""
+        pc = Piston()
+        pc.L0[:] = [float(x) for x in self.box]
+        pc.p_ref, pc.mass = float(pref_atm) * PU, float(mass)
+        pc.V = float(np.prod(self.box))
+        pc.Pi = 0.0
+        u = np.ascontiguousarray(u, dtype=np.uint64)
+        pc.Wtr = lib().or_virial_trace(self.h, _p(u, C.c_uint64))
+        return pc
+
+    def step_npt(self, u, v, nsteps, pc):
+        """nsteps Andersen-barostat steps; pc (Piston) is advanced in place and
+        the oracle's box follows it.  Returns u, v, energies, virial, rc."""
+        u = np.ascontiguousarray(u, dtype=np.uint64).copy()
+        v = np.ascontiguousarray(v, dtype=np.float64).copy()
+        e = np.zeros(10)
+        w = np.zeros(9)
+        rc = lib().or_step_npt(self.h, C.byref(pc), _p(u, C.c_uint64), _p(v, C.c_double), int(nsteps),
+                               _p(e, C.c_double), _p(w, C.c_double))
+        self.box = np.zeros(3)
+        lib().or_get_box(self.h, _p(self.box, C.c_double))
         return u, v, dict(zip(E_NAMES, e)), w.reshape(3, 3), rc
 
     def shake_positions(self, uold, u):
