@@ -845,7 +845,9 @@ void Engine::drop_graphs() {
   if (g_plain_) (void)hipGraphExecDestroy(g_plain_);
   if (g_ev_) (void)hipGraphExecDestroy(g_ev_);
   if (g_npt_) (void)hipGraphExecDestroy(g_npt_);
-  g_plain_ = g_ev_ = g_npt_ = nullptr;
+  if (g_plain_k_) (void)hipGraphExecDestroy(g_plain_k_);
+  if (g_npt_k_) (void)hipGraphExecDestroy(g_npt_k_);
+  g_plain_ = g_ev_ = g_npt_ = g_plain_k_ = g_npt_k_ = nullptr;
 }
 
 int Engine::make_work(Work& w, const std::vector<int>& blocks) {
@@ -1050,10 +1052,10 @@ void Engine::record_npt_step() {
   launch_kinetic(L_);
 }
 
-int Engine::capture_npt(hipGraphExec_t* out) {
+int Engine::capture_npt(hipGraphExec_t* out, int nsteps) {
   hipGraph_t g = nullptr;
   FZ_HIP(hipStreamBeginCapture(L_.stream, hipStreamCaptureModeRelaxed));
-  record_npt_step();
+  for (int k = 0; k < nsteps; k++) record_npt_step();
   FZ_HIP(hipStreamEndCapture(L_.stream, &g));
   hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -1068,8 +1070,11 @@ int Engine::steps_npt(int n) {
   for (int attempt = 0; attempt < 4; attempt++) {
     FZ_TRY(snapshot(true));
     if (!g_npt_) FZ_TRY(capture_npt(&g_npt_));
+    if (!g_npt_k_ && (sys_.size() == 1 || n >= GRAPH_STEPS)) FZ_TRY(capture_npt(&g_npt_k_, GRAPH_STEPS));
     launch_gather_sorted(L_);
-    for (int k = 0; k < n; k++) FZ_HIP(hipGraphLaunch(g_npt_, L_.stream));
+    int k = 0;
+    for (; k + GRAPH_STEPS <= n; k += GRAPH_STEPS) FZ_HIP(hipGraphLaunch(g_npt_k_, L_.stream));
+    for (; k < n; k++) FZ_HIP(hipGraphLaunch(g_npt_, L_.stream));
     FZ_TRY(read_status());
     std::vector<int> grow(sys_.size(), 0);
     int overflow = 0, fatal = -1, shake = -1, box = -1;
@@ -1101,10 +1106,10 @@ int Engine::steps_npt(int n) {
   return set_error(FZ_ERR_CAPACITY, "neighbour capacity could not be grown");
 }
 
-int Engine::capture(bool ev, hipGraphExec_t* out) {
+int Engine::capture(bool ev, hipGraphExec_t* out, int nsteps) {
   hipGraph_t g = nullptr;
   FZ_HIP(hipStreamBeginCapture(L_.stream, hipStreamCaptureModeRelaxed));
-  record_step(ev);
+  for (int k = 0; k < nsteps; k++) record_step(ev);
   FZ_HIP(hipStreamEndCapture(L_.stream, &g));
   hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -1292,13 +1297,19 @@ int Engine::steps(int n) {
     FZ_TRY(snapshot(true));  // the SHAKE flag reports this attempt's steps only
     if (!g_plain_) FZ_TRY(capture(false, &g_plain_));
     if (!g_ev_) FZ_TRY(capture(true, &g_ev_));
+    // a single system captures it with the others (so that no later call pays
+    // for its capture); a batch on its first call long enough to use it
+    if (!g_plain_k_ && (sys_.size() == 1 || n - 1 >= GRAPH_STEPS)) FZ_TRY(capture(false, &g_plain_k_, GRAPH_STEPS));
     // The step graph has no slot gather: it relies on the slot copies and
     // work items of the last list build or force evaluation, which may belong
     // to another engine over the same systems, so lay them out for this one
     // first.
     launch_gather_sorted(L_);
     launch_step_first(L_);
-    for (int k = 0; k < n; k++) FZ_HIP(hipGraphLaunch(k == n - 1 ? g_ev_ : g_plain_, L_.stream));
+    int k = 0;
+    for (; k + GRAPH_STEPS <= n - 1; k += GRAPH_STEPS) FZ_HIP(hipGraphLaunch(g_plain_k_, L_.stream));
+    for (; k < n - 1; k++) FZ_HIP(hipGraphLaunch(g_plain_, L_.stream));
+    FZ_HIP(hipGraphLaunch(g_ev_, L_.stream));
     FZ_TRY(read_status());
     // classify before touching any state
     std::vector<int> grow(sys_.size(), 0);

@@ -152,6 +152,11 @@ class Engine {
   SysDev* d_sys_ = nullptr;
   std::vector<int*> d_pre_;
   hipGraphExec_t g_plain_ = nullptr, g_ev_ = nullptr, g_npt_ = nullptr;
+  // The same steps captured GRAPH_STEPS at a time: the host cost of launching a
+  // graph whose streams fork and join is a fixed ~60 us plus ~3 us per node, about
+  // as much as a small system's whole step, so steps are launched in blocks.
+  static constexpr int GRAPH_STEPS = 8;
+  hipGraphExec_t g_plain_k_ = nullptr, g_npt_k_ = nullptr;
   // The force evaluation forks after the list build: the pair kernel on the
   // main stream, bonded terms and the PME chain on two side streams (default
   // priority), joined before the second half kick.  The PME branch forks
@@ -167,11 +172,11 @@ class Engine {
   int ensure();  // (re)prepare when a member's buffers moved since the last prepare
   int time_steps(int nsteps, double* ms);
   int make_work(Work& w, const std::vector<int>& blocks);
-  int capture(bool ev, hipGraphExec_t* out);
+  int capture(bool ev, hipGraphExec_t* out, int nsteps = 1);
   void record_step(bool ev);
   void record_npt_step();
   int steps_npt(int n);
-  int capture_npt(hipGraphExec_t* out);
+  int capture_npt(hipGraphExec_t* out, int nsteps = 1);
   void record_forces(bool ev, bool lite = false, bool check = true);
   int grow_and_reset(int s, int bits);
   int realloc_neighbor(int s, long long pairs, int clus);

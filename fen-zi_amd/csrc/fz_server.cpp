@@ -16,6 +16,8 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "fz_engine.h"
@@ -175,34 +177,48 @@ struct fz_server {
     FZ_HIP(hipSetDevice(device));
     admit();
     const int chunk = std::max(1, cfg.chunk_steps);
-    // minimisation round
-    if (Engine* e = group(FZ_JOB_MINIMIZING, min_eng, min_members)) {
-      int n = chunk;
-      for (int64_t id : min_members) n = std::min(n, jobs[id]->min_steps - jobs[id]->min_done);
-      std::vector<int64_t> ids = min_members;
-      if (e->minimize(n, cfg.min_max_step) != FZ_OK) {
+    // The minimisation round (a host loop that reads each iteration's energies)
+    // runs on a second host thread while this one runs the dynamics round: the
+    // two batches have their own engines and streams, so they overlap on the
+    // device.  Job bookkeeping happens afterwards, here.  Jobs that finish
+    // minimising join the dynamics batch in the next round.
+    Engine* me = group(FZ_JOB_MINIMIZING, min_eng, min_members);
+    Engine* de = group(FZ_JOB_DYNAMICS, md_eng, md_members);
+    int n_min = chunk, n_md = chunk;
+    for (int64_t id : min_members) n_min = std::min(n_min, jobs[id]->min_steps - jobs[id]->min_done);
+    // the dynamics round ends at the first job's completion or velocity
+    // reassignment, so every job sees its own schedule exactly
+    for (int64_t id : md_members) {
+      Job& j = *jobs[id];
+      n_md = std::min(n_md, j.md_steps - j.md_done);
+      if (j.reassign_every > 0) n_md = std::min(n_md, j.reassign_every - j.md_done % j.reassign_every);
+    }
+    const std::vector<int64_t> min_ids = min_members, md_ids = md_members;
+    int rc_min = FZ_OK, rc_md = FZ_OK;
+    std::thread th;
+    if (me)
+      th = std::thread([&] {
+        if (hipSetDevice(device) != hipSuccess) { rc_min = FZ_ERR_HIP; return; }
+        rc_min = me->minimize(n_min, cfg.min_max_step);
+      });
+    if (de) rc_md = de->steps(n_md);
+    if (th.joinable()) th.join();
+    if (me) {
+      if (rc_min != FZ_OK) {
         min_eng.reset(); min_members.clear();
-        run_isolated(ids, [&](Job& j) { return j.solo->minimize(n, cfg.min_max_step); });
+        run_isolated(min_ids, [&](Job& j) { return j.solo->minimize(n_min, cfg.min_max_step); });
       }
-      for (int64_t id : ids) {
+      for (int64_t id : min_ids) {
         Job& j = *jobs[id];
         if (j.state != FZ_JOB_MINIMIZING) continue;
-        j.min_done += n;
+        j.min_done += n_min;
         if (j.min_done >= j.min_steps && start_dynamics(j) != FZ_OK) finish(j, FZ_JOB_FAILED);
       }
     }
-    // dynamics round: all dynamics jobs advance together; the round ends at the
-    // first job's completion or velocity reassignment, so every job sees its
-    // own schedule exactly
-    if (Engine* e = group(FZ_JOB_DYNAMICS, md_eng, md_members)) {
-      int n = chunk;
-      for (int64_t id : md_members) {
-        Job& j = *jobs[id];
-        n = std::min(n, j.md_steps - j.md_done);
-        if (j.reassign_every > 0) n = std::min(n, j.reassign_every - j.md_done % j.reassign_every);
-      }
-      std::vector<int64_t> ids = md_members;
-      if (e->steps(n) != FZ_OK) {
+    if (de) {
+      const int n = n_md;
+      std::vector<int64_t> ids = md_ids;
+      if (rc_md != FZ_OK) {
         md_eng.reset(); md_members.clear();
         run_isolated(ids, [&](Job& j) { return j.solo->steps(n); });
       }
