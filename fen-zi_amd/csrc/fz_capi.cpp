@@ -221,6 +221,22 @@ int fz_autotune(fz_system* s, int nsteps, double* out) {
   FZ_GUARD_END
 }
 
+int fz_set_launch(fz_system* s, int nl, int grid, int pme_path, int32_t* out) {
+  FZ_GUARD_BEGIN
+  if (!s) return set_error(FZ_ERR_ARG, "invalid argument");
+  FZ_HIP(hipSetDevice(s->ctx->device));
+  return s->eng->set_launch(nl, grid, pme_path, out);
+  FZ_GUARD_END
+}
+
+int fz_batch_set_launch(fz_batch* b, int nl, int grid, int pme_path, int32_t* out) {
+  FZ_GUARD_BEGIN
+  if (!b) return set_error(FZ_ERR_ARG, "invalid argument");
+  FZ_HIP(hipSetDevice(b->ctx->device));
+  return b->eng->set_launch(nl, grid, pme_path, out);
+  FZ_GUARD_END
+}
+
 int fz_batch_autotune(fz_batch* b, int nsteps, double* out) {
   FZ_GUARD_BEGIN
   if (!b || nsteps <= 0 || !out) return set_error(FZ_ERR_ARG, "invalid argument");

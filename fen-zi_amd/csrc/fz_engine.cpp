@@ -460,6 +460,7 @@ int System::alloc_device() {
     else { double2* t; FZ_TRY(dalloc(&t, KS)); FZ_TRY(up(t, tw64)); d.tw = t; }
   }
   FZ_TRY(dalloc(&d.acc, A_NSLOT));
+  FZ_TRY(dalloc(&d.acc_ev, (size_t)EV_SPREAD * EV_W));
   FZ_TRY(dalloc(&d.flags, F_NSLOT));
   FZ_TRY(dalloc(&d.mstep, 2));
   FZ_TRY(dalloc(&d.mH, (size_t)3 * n));
@@ -593,7 +594,7 @@ int64_t footprint_bytes(const fz_topology* t, const fz_params* p, int precision)
   b += rnd256(8 * K3) + rnd256(KH * (mixed ? 8 : 16)) + rnd256(K3 * (mixed ? 4 : 8));
   b += rnd256(4 * nbr) + rnd256(4 * (nbr + 1)) + rnd256(n * 16 * (mixed ? 4 : 8)) + rnd256(nbr * 1331 * 8);
   b += 3 * rnd256(8 * KS) + rnd256(KS * (mixed ? 8 : 16));
-  b += rnd256(8 * A_NSLOT) + rnd256(4 * F_NSLOT) + rnd256(16) + 2 * rnd256(24 * n);
+  b += rnd256(8 * A_NSLOT) + rnd256(8 * EV_SPREAD * EV_W) + rnd256(4 * F_NSLOT) + rnd256(16) + 2 * rnd256(24 * n);
   b += rnd256(72 * n + 16);                                  // snapshot
   // the neighbour block after two growth steps of the overflow path (pair-list
   // entries x1.6 + 4096, clusters x1.3 + 64 each): clustered chains can need
@@ -907,7 +908,7 @@ int Engine::prepare() {
     L_.lj_sep &= s->dv.lj_sep;
     L_.com_any |= s->dv.com_on;
   }
-  std::vector<int> atoms, slots, cols, clus, srch, terms, ccl, stp, fx_, fy_, fz_, bricks;
+  std::vector<int> atoms, slots, cols, clus, srch, terms, ccl, stp, fx_, fy_, fz_, bricks, planes;
   // Waves per I cluster in the pair search: up to each system's search_parts
   // when the launch holds few clusters (latency bound: too few waves in
   // flight), one when it holds many.
@@ -937,6 +938,7 @@ int Engine::prepare() {
     fy_.push_back(d.K[2] * ng);
     fz_.push_back(d.K[1] * ng);
     bricks.push_back(d.n_bricks);
+    planes.push_back(d.K[2]);
   }
   FZ_TRY(make_work(L_.w_atoms, atoms));
   FZ_TRY(make_work(L_.w_slots, slots));
@@ -950,6 +952,7 @@ int Engine::prepare() {
   FZ_TRY(make_work(L_.w_ffty, fy_));
   FZ_TRY(make_work(L_.w_fftz, fz_));
   FZ_TRY(make_work(L_.w_bricks, bricks));
+  FZ_TRY(make_work(L_.w_planes, planes));
   // Default persistent pair grid: about ten list chunks per block (a list holds
   // ~19 cluster-pair entries per atom at water density, 32 entries per chunk);
   // measured on the 17K slab, 1,024 blocks beat 640 and 2,048 (the bonded and
@@ -961,6 +964,16 @@ int Engine::prepare() {
   // Spread by bricks from 64K atoms per launch; below, the single atomic kernel
   // is cheaper than the brick pipeline's five launches.  Same grid either way.
   L_.spread_bricks = atoms_total >= 65536 ? 1 : 0;
+  // Systems of up to PLANE_MAX_ATOMS atoms: spread and x, y transforms by the
+  // plane kernels (two launches and no grid atomics instead of four launches).
+  L_.plane_lds = 0;
+  L_.pme_planes = 1;
+  for (auto* s : sys_) {
+    L_.plane_lds = std::max(L_.plane_lds, plane_lds_bytes(s->dv.K[0], s->dv.K[1], csz));
+    if (s->dv.n_atoms > PLANE_MAX_ATOMS) L_.pme_planes = 0;
+  }
+  if (tuned_planes_ == 1) L_.pme_planes = 1;  // asked for: any atom count
+  if (L_.plane_lds > PLANE_MAX_LDS || tuned_planes_ == 0) L_.pme_planes = 0;
   prepared_ = true;
   gens_.clear();
   for (auto* s : sys_) gens_.push_back(s->gen);
@@ -976,6 +989,7 @@ int Engine::ensure() {
   // member whose grid was last spread by bricks (in another engine: a batch
   // over 64K atoms) is cleared before this engine spreads atomically.
   for (auto* s : sys_) {
+    if (L_.pme_planes) continue;  // the plane kernels neither read nor clear the grid accumulator
     if (!L_.spread_bricks && s->grid_mode != 0 && s->dv.Qacc) {
       const size_t K3 = (size_t)s->dv.K[0] * s->dv.K[1] * s->dv.K[2];
       FZ_HIP(hipMemsetAsync(s->dv.Qacc, 0, K3 * 8, L_.stream));
@@ -1565,6 +1579,25 @@ int Engine::time_steps(int nsteps, double* ms) {
 // capacity or SHAKE error) counts as infinitely slow.  Members with an explicit
 // column grid keep it.  On every exit the members' positions, velocities,
 // step counters and minimiser steps are those they had on entry.
+int Engine::set_launch(int nl, int grid, int pme_path, int32_t* out) {
+  if (nl > 64 || grid > 65536 || pme_path > FZ_PME_PLANES) return set_error(FZ_ERR_ARG, "invalid launch shape");
+  if (nl >= 0) tuned_nl_ = nl;
+  if (grid >= 0) tuned_grid_ = grid;
+  if (pme_path >= 0) tuned_planes_ = pme_path == FZ_PME_AUTO ? -1 : pme_path == FZ_PME_PLANES ? 1 : 0;
+  FZ_TRY(prepare());
+  if (tuned_planes_ == 1 && !L_.pme_planes) {
+    tuned_planes_ = -1;
+    FZ_TRY(prepare());
+    return set_error(FZ_ERR_ARG, "a PME grid plane of this set does not fit in %d bytes of LDS", PLANE_MAX_LDS);
+  }
+  if (out) {
+    out[0] = L_.fft_nl;
+    out[1] = L_.pair_grid;
+    out[2] = L_.pme_planes ? FZ_PME_PLANES : FZ_PME_GRID;
+  }
+  return FZ_OK;
+}
+
 int Engine::autotune(int nsteps, double* out) {
   FZ_TRY(ensure());
   const int ns = (int)sys_.size();

@@ -136,6 +136,9 @@ class Engine {
   // fastest, and restores the systems' state.  out: {width factor, fft lines,
   // pair grid, best ms per step}.
   int autotune(int nsteps, double* out);
+  // FFT lines, pair grid (0 default, < 0 keep), PME path (FZ_PME_*, < 0 keep);
+  // out (nullable): the three in effect
+  int set_launch(int nl, int grid, int pme_path, int32_t* out);
   int read_status();                   // flags + accumulators (+ barostat state) of every system -> host
   // barostat members: the piston's virial trace from the last stand-alone
   // evaluation (after a change of positions outside dynamics)
@@ -169,6 +172,7 @@ class Engine {
   bool prepared_ = false;
   std::vector<int> gens_;
   int tuned_nl_ = 0, tuned_grid_ = 0;  // autotuned launch shapes (0 = defaults)
+  int tuned_planes_ = -1;              // PME plane kernels: -1 by size, 0 off, 1 on
   int ensure();  // (re)prepare when a member's buffers moved since the last prepare
   int time_steps(int nsteps, double* ms);
   int make_work(Work& w, const std::vector<int>& blocks);

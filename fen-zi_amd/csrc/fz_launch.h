@@ -15,6 +15,15 @@ struct Work {
 // pair-search waves per I cluster of a system: its buffer parts, at most the launch's cap
 __host__ __device__ inline int search_split(int parts, int cap) { return parts < cap ? parts : cap; }
 
+// LDS of the PME plane kernels: two buffers of K0 K1 complex values of csz
+// bytes and the x and y twiddles
+inline int plane_lds_bytes(int K0, int K1, int csz) { return (2 * K0 * K1 + K0 + K1) * csz; }
+// the plane kernels are used up to this many atoms per system (each of a
+// system's K2 workgroups reads every atom's z position) and this much LDS
+constexpr int PLANE_MAX_ATOMS = 65536, PLANE_MAX_LDS = 65536;
+// launches of fewer planes than this use workgroups of 1,024 threads, others 256
+constexpr int PLANE_WIDE_BELOW = 512;
+
 struct Launch {
   hipStream_t stream = nullptr;
   const SysDev* d_sys = nullptr;  // device array of nsys descriptors
@@ -32,6 +41,7 @@ struct Launch {
   Work w_ffty;    // y-line groups
   Work w_fftz;    // z-line groups
   Work w_bricks;  // one block per PME brick
+  Work w_planes;  // one block per PME grid z plane
   int* chunk_pre = nullptr;  // device: pair-kernel chunks per system (prefix)
   int pair_grid = 0;         // persistent pair-kernel blocks
   int fft_lds = 0;           // dynamic LDS bytes for the FFT kernels
@@ -39,6 +49,8 @@ struct Launch {
   int lj_sep = 0;            // every system follows the LJ combination rule
   int com_any = 0;           // some system has a COM restraint
   int spread_bricks = 1;     // PME spread by bricks (1) or by global atomics (0)
+  int pme_planes = 0;        // PME spread and x, y transforms by the plane kernels
+  int plane_lds = 0;         // their dynamic LDS bytes
   int search_cap = 1;  // cap on the pair-search waves per I cluster (1, 2 or 4)
   int npt = 0;         // every system has the Andersen barostat on
 };
@@ -49,7 +61,6 @@ void launch_gather_sorted(const Launch& L);
 void launch_chunk_prefix(const Launch& L);  // batches: pair-kernel work items, rebuild flags cleared
 void launch_pair(const Launch& L, int ev);  // 0 forces, 1 + FP32 energies, 2 + FP64 energies
 void launch_bonded(const Launch& L, bool ev);
-void launch_pme(const Launch& L, bool ev, int conv_mode);
 void launch_pme_fft_only(const Launch& L, int conv_mode);
 void launch_pme_part(const Launch& L, int part, bool ev);  // 0 spread, 1 FFT + convolution, 2 gather
 // kick, drift, SHAKE, rebuild test; finish: the previous step's kick + RATTLE first

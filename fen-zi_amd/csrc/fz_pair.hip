@@ -253,23 +253,43 @@ struct EvAcc {
 #pragma unroll
     for (int k = 0; k < 8; k++) e[k] += __double2ll_rn(c[k] * E_SCALE);
   }
+  // into this wave's copy of the system's spread sums (folded by k_ev_fold)
   __device__ void flush(const SysDev& d, int lane) {
     long long v[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = wave_sum(e[k]);
     const int np = wave_sum(npr);
     if (lane == 0) {
-      // symmetric virial: xx yy zz xy xz yz
-      const int slot[8] = {A_VDW, A_ELEC, A_VIR + 0, A_VIR + 4, A_VIR + 8, A_VIR + 1, A_VIR + 2, A_VIR + 5};
+      const unsigned wv = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (EV_SPREAD - 1);
+      unsigned long long* c = d.acc_ev + (size_t)wv * EV_W;
 #pragma unroll
-      for (int k = 0; k < 8; k++) atomicAdd(d.acc + slot[k], (unsigned long long)v[k]);
-      atomicAdd(d.acc + A_VIR + 3, (unsigned long long)v[5]);
-      atomicAdd(d.acc + A_VIR + 6, (unsigned long long)v[6]);
-      atomicAdd(d.acc + A_VIR + 7, (unsigned long long)v[7]);
-      atomicAdd(d.acc + A_NPAIR, (unsigned long long)np);
+      for (int k = 0; k < 8; k++) atomicAdd(c + k, (unsigned long long)v[k]);
+      atomicAdd(c + 8, (unsigned long long)np);
     }
   }
 };
+
+// Adds each system's EV_SPREAD copies into its accumulators (the symmetric
+// virial's mirrored entries too) and clears the copies.  One block of
+// EV_SPREAD threads per system; integer sums, so the order does not matter.
+__global__ __launch_bounds__(EV_SPREAD) void k_ev_fold(const SysDev* __restrict__ S) {
+  const SysDev& d = S[blockIdx.x];
+  unsigned long long* c = d.acc_ev + (size_t)threadIdx.x * EV_W;
+  long long v[EV_NV];
+#pragma unroll
+  for (int k = 0; k < EV_NV; k++) {
+    v[k] = wave_sum((long long)c[k]);
+    c[k] = 0ull;
+  }
+  if (threadIdx.x == 0) {
+    const int slot[EV_NV] = {A_VDW, A_ELEC, A_VIR + 0, A_VIR + 4, A_VIR + 8, A_VIR + 1, A_VIR + 2, A_VIR + 5, A_NPAIR};
+#pragma unroll
+    for (int k = 0; k < EV_NV; k++) atomicAdd(d.acc + slot[k], (unsigned long long)v[k]);
+    atomicAdd(d.acc + A_VIR + 3, (unsigned long long)v[5]);
+    atomicAdd(d.acc + A_VIR + 6, (unsigned long long)v[6]);
+    atomicAdd(d.acc + A_VIR + 7, (unsigned long long)v[7]);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // EV: 0 forces only; 1 also energies and virial summed from the FP32 pair terms
@@ -592,6 +612,7 @@ void launch_pair(const Launch& L, int ev) {
     if (ev) hipLaunchKernelGGL(k_pair_f64<true>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
     else hipLaunchKernelGGL(k_pair_f64<false>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
   }
+  if (ev) hipLaunchKernelGGL(k_ev_fold, dim3(L.nsys), dim3(EV_SPREAD), 0, L.stream, L.d_sys);
 }
 
 }  // namespace fz

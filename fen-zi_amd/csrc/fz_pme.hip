@@ -604,6 +604,176 @@ __global__ __launch_bounds__(256) void k_fft_x_c2r(const SysDev* __restrict__ S,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Plane kernels, for systems of up to a few ten thousand atoms whose grid
+// plane fits in LDS: one workgroup per z plane.
+//
+// Forward: the plane's charge sum, then its x (real to half spectrum) and y
+// transforms, all in LDS.  Each workgroup reads every atom's z position and
+// spreads the atoms whose stencil covers its plane (16 points each) with LDS
+// atomics, as int64 fixed point in the mixed form: the same terms and integer
+// sums as the global-atomic and brick spreads, so the plane holds the same
+// values, and the line transforms are those of the x and y passes, so the half
+// spectrum is the same to the bit.  No grid accumulator is read or cleared.
+// Inverse: y, then x (half spectrum to real) transforms of the plane into the
+// potential grid, again the passes' arithmetic.
+// LDS: plane_lds_bytes (the charge sum lives in the second buffer until it is read).
+// Workgroups of NT threads: 1,024 where a launch holds few planes (the atom
+// scan of one workgroup per plane needs many waves to hide its load latency
+// behind), 256 where it holds many (a batch of small systems: more workgroups
+// per CU).
+constexpr int PLANE_BATCH = 8;
+
+template <bool MIXED, int NT>
+__global__ __launch_bounds__(NT) void k_pme_plane_fwd(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using C = typename C2<R>::t;
+  using A = typename std::conditional<MIXED, unsigned long long, double>::type;
+  extern __shared__ char smem[];
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int z = blockIdx.x - block_base(pre, nsys, s_);
+  const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2], Kxh = d.Kxh;
+  const int KP = K0 * K1;
+  C* b0 = (C*)smem;
+  C* b1 = b0 + KP;
+  C* twx = b1 + KP;
+  C* twy = twx + K0;
+  A* acc = (A*)b1;
+  for (int e = threadIdx.x; e < KP; e += NT) acc[e] = A(0);
+  for (int k = threadIdx.x; k < K0 + K1; k += NT) twx[k] = ((const C*)d.tw)[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int h = lane >> 4, ca = lane & 3, cb = (lane >> 2) & 3;
+  const int na = d.n_atoms;
+  // PLANE_BATCH atoms per thread at a time: their z positions, then the rest
+  // of the atoms on this plane, are loaded together (two memory round trips
+  // per batch rather than two per atom)
+  for (int base = 0; base < na; base += NT * PLANE_BATCH) {
+    unsigned long long uz[PLANE_BATCH];
+#pragma unroll
+    for (int k = 0; k < PLANE_BATCH; k++) {
+      const int a = base + k * NT + threadIdx.x;
+      uz[k] = a < na ? d.uz[a] : 0ull;
+    }
+    int cz[PLANE_BATCH];
+    R fz[PLANE_BATCH];
+#pragma unroll
+    for (int k = 0; k < PLANE_BATCH; k++) {
+      const int a = base + k * NT + threadIdx.x;
+      const int gz0 = spline_pos<R>(uz[k], K2, fz[k]);
+      int cc = z - gz0;
+      cc += cc < 0 ? K2 : 0;
+      cz[k] = a < na && cc < 4 ? cc : -1;
+    }
+    double qd[PLANE_BATCH];
+    unsigned long long ux[PLANE_BATCH], uy[PLANE_BATCH];
+#pragma unroll
+    for (int k = 0; k < PLANE_BATCH; k++) {
+      const int a = base + k * NT + threadIdx.x;
+      qd[k] = 0.0;
+      ux[k] = uy[k] = 0ull;
+      if (cz[k] >= 0) {
+        qd[k] = d.q[a];
+        ux[k] = d.ux[a];
+        uy[k] = d.uy[a];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PLANE_BATCH; k++) {
+      const R q = (R)qd[k];
+      const bool hit = cz[k] >= 0 && q != R(0);
+      R fx = R(0), fy = R(0), wz = R(0);
+      int gx0 = 0, gy0 = 0;
+      if (hit) {
+        gx0 = spline_pos<R>(ux[k], K0, fx);
+        gy0 = spline_pos<R>(uy[k], K1, fy);
+        wz = bspline4_k<R>(fz[k], cz[k]);
+      }
+      unsigned long long m = __ballot(hit);
+      while (m) {
+        // four atoms at a time: lanes 16 h + p take the h-th, grid point (p & 3, p >> 2) of its plane rows
+        int src = -1;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int b = m ? __ffsll((long long)m) - 1 : -1;
+          if (b >= 0) m &= m - 1ull;
+          src = h == j ? b : src;
+        }
+        const int sl = src < 0 ? 0 : src;
+        const R qs = __shfl(q, sl, 64), fxs = __shfl(fx, sl, 64), fys = __shfl(fy, sl, 64), wzs = __shfl(wz, sl, 64);
+        int gx = __shfl(gx0, sl, 64) + ca, gy = __shfl(gy0, sl, 64) + cb;
+        if (src >= 0) {
+          gx -= gx >= K0 ? K0 : 0;
+          gy -= gy >= K1 ? K1 : 0;
+          const R v = qs * bspline4_k<R>(fxs, ca) * bspline4_k<R>(fys, cb) * wzs;
+          if (MIXED) atomicAdd((unsigned long long*)&acc[gy * K0 + gx], fixed_f32((float)v));
+          else atomicAdd((double*)&acc[gy * K0 + gx], (double)v);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < KP; e += NT) {
+    C v;
+    if (MIXED) v.x = (R)from_fixed_acc(((const unsigned long long*)acc)[e]);
+    else v.x = (R)((const double*)acc)[e];
+    v.y = R(0);
+    b0[e] = v;
+  }
+  __syncthreads();
+  // x lines (one per y), then y lines (one per kept kx)
+  C* r = fft_lines<C, R>(b0, b1, K0, K1, twx, R(-1));
+  C* o = r == b0 ? b1 : b0;
+  for (int e = threadIdx.x; e < Kxh * K1; e += NT) {
+    const int y = e / Kxh, kx = e - y * Kxh;
+    o[kx * K1 + y] = r[y * K0 + kx];
+  }
+  __syncthreads();
+  C* r2 = fft_lines<C, R>(o, r, K1, Kxh, twy, R(-1));
+  C* Qc = (C*)d.Qc + (size_t)z * K1 * Kxh;
+  for (int e = threadIdx.x; e < Kxh * K1; e += NT) {
+    const int y = e / Kxh, kx = e - y * Kxh;
+    Qc[e] = r2[kx * K1 + y];
+  }
+}
+
+template <bool MIXED, int NT>
+__global__ __launch_bounds__(NT) void k_pme_plane_inv(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using C = typename C2<R>::t;
+  extern __shared__ char smem[];
+  const int s_ = block_sys(pre, nsys, blockIdx.x);
+  const SysDev& d = S[s_];
+  const int z = blockIdx.x - block_base(pre, nsys, s_);
+  const int K0 = d.K[0], K1 = d.K[1], Kxh = d.Kxh;
+  const int KP = K0 * K1;
+  C* b0 = (C*)smem;
+  C* b1 = b0 + KP;
+  C* twx = b1 + KP;
+  C* twy = twx + K0;
+  const C* Qc = (const C*)d.Qc + (size_t)z * K1 * Kxh;
+  for (int e = threadIdx.x; e < Kxh * K1; e += NT) {
+    const int y = e / Kxh, kx = e - y * Kxh;
+    b0[kx * K1 + y] = Qc[e];
+  }
+  for (int k = threadIdx.x; k < K0 + K1; k += NT) twx[k] = ((const C*)d.tw)[k];
+  __syncthreads();
+  C* r = fft_lines<C, R>(b0, b1, K1, Kxh, twy, R(1));
+  C* o = r == b0 ? b1 : b0;
+  for (int e = threadIdx.x; e < KP; e += NT) {
+    const int y = e / K0, k = e - y * K0;
+    C v;
+    if (k < Kxh) v = r[k * K1 + y];
+    else { v = r[(K0 - k) * K1 + y]; v.y = -v.y; }
+    o[e] = v;
+  }
+  __syncthreads();
+  C* r2 = fft_lines<C, R>(o, r, K0, K1, twx, R(1));
+  R* phi = (R*)d.phi + (size_t)z * KP;
+  for (int e = threadIdx.x; e < KP; e += NT) phi[e] = r2[e].x;
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_pme_gather(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using R = typename std::conditional<MIXED, float, double>::type;
@@ -678,22 +848,28 @@ static void launch_fft_passes(const Launch& L, bool ev, int mode) {
   hipLaunchKernelGGL(k_fft_x_c2r<MIXED>, dim3(L.w_fftx.total), dim3(256), lds, L.stream, L.d_sys, L.nsys, L.w_fftx.pre, NL);
 }
 
-void launch_pme(const Launch& L, bool ev, int conv_mode) {
-  if (L.w_fftx.total == 0) return;
-  if (L.mixed) {
-    launch_spread(L);
-    launch_fft_passes<true>(L, ev, conv_mode);
-    hipLaunchKernelGGL(k_pme_gather<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
-  } else {
-    launch_spread(L);
-    launch_fft_passes<false>(L, ev, conv_mode);
-    hipLaunchKernelGGL(k_pme_gather<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+template <bool MIXED, int NT>
+static void launch_planes(const Launch& L, int part, bool ev) {
+  if (part == 0) {
+    hipLaunchKernelGGL((k_pme_plane_fwd<MIXED, NT>), dim3(L.w_planes.total), dim3(NT), L.plane_lds, L.stream, L.d_sys, L.nsys,
+                       L.w_planes.pre);
+    return;
   }
+  const size_t lds = L.fft_lds;
+  const int NL = L.fft_nl;
+  if (ev) hipLaunchKernelGGL((k_fft_z_conv<MIXED, true>), dim3(L.w_fftz.total), dim3(256), lds, L.stream, L.d_sys, L.nsys, L.w_fftz.pre, NL, 0);
+  else hipLaunchKernelGGL((k_fft_z_conv<MIXED, false>), dim3(L.w_fftz.total), dim3(256), lds, L.stream, L.d_sys, L.nsys, L.w_fftz.pre, NL, 0);
+  hipLaunchKernelGGL((k_pme_plane_inv<MIXED, NT>), dim3(L.w_planes.total), dim3(NT), L.plane_lds, L.stream, L.d_sys, L.nsys,
+                     L.w_planes.pre);
 }
 
 void launch_pme_part(const Launch& L, int part, bool ev) {
   if (L.w_fftx.total == 0) return;
-  if (part == 0) {
+  if (part < 2 && L.pme_planes) {
+    const bool wide = L.w_planes.total < PLANE_WIDE_BELOW;
+    if (L.mixed) wide ? launch_planes<true, 1024>(L, part, ev) : launch_planes<true, 256>(L, part, ev);
+    else wide ? launch_planes<false, 1024>(L, part, ev) : launch_planes<false, 256>(L, part, ev);
+  } else if (part == 0) {
     launch_spread(L);
   } else if (part == 1) {
     if (L.mixed) launch_fft_passes<true>(L, ev, 0);

@@ -21,6 +21,13 @@ enum AccSlot {
   A_COM /* 6 slots: sum m x of group 0, then group 1 */, A_NSLOT = A_COM + 6
 };
 
+// The pair kernels' energy / virial sums of one evaluation go first to
+// EV_SPREAD copies of EV_W words (one 128-byte line each; a wave adds into copy
+// wave & (EV_SPREAD - 1)), and a fold kernel adds the copies into acc: several
+// thousand waves adding into the few lines of acc serialise there.  Words: vdw,
+// elec, virial xx yy zz xy xz yz, pair count.
+constexpr int EV_SPREAD = 64, EV_W = 16, EV_NV = 9;
+
 // Device-side status flags.
 enum FlagSlot { F_REBUILD = 0, F_OVERFLOW, F_SHAKE_FAIL, F_NBUILD, F_NSLOT };
 
@@ -173,6 +180,7 @@ struct SysDev {
   double4 *ccl_d2;    // squared lengths (up to 3)
   // ---- accumulators / flags ---------------------------------------------
   unsigned long long *acc;  // [A_NSLOT] int64 fixed point (A_FMAX: bits of max |F|^2)
+  unsigned long long *acc_ev;  // [EV_SPREAD * EV_W] pair energy / virial copies, zero between evaluations
   int *flags;               // [F_NSLOT]
   double *mstep;            // minimiser: [0] largest atom displacement of the next move (A) or the
                             // conjugate-gradient / steepest-descent factor, [1] schedule method

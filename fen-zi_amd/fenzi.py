@@ -84,9 +84,11 @@ EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create
            "fz_minimize_ex", "fz_batch_minimize_ex", "fz_restart_write", "fz_restart_read", "fz_dcd_open",
            "fz_dcd_write_frame", "fz_dcd_close", "fz_set_fixed_atoms", "fz_set_com_restraint",
            "fz_deck_com_restraint", "fz_deck_fixed_atoms", "fz_set_neighbor_capacity",
-           "fz_pme_convolve_noalloc", "fz_pme_convolve_noalloc_bytes", "fz_set_barostat", "fz_get_barostat"]
+           "fz_pme_convolve_noalloc", "fz_pme_convolve_noalloc_bytes", "fz_set_barostat", "fz_get_barostat",
+           "fz_set_launch", "fz_batch_set_launch"]
 
 MIN_ADAPTIVE_SD, MIN_REF_CG, MIN_REF_SD = 0, 1, 2
+PME_AUTO, PME_GRID, PME_PLANES = 0, 1, 2
 
 
 class MinParams(C.Structure):
@@ -154,6 +156,8 @@ def lib():
         L.fz_phase_times.argtypes = [vp, C.c_int, _dp]
         L.fz_autotune.argtypes = [vp, C.c_int, _dp]
         L.fz_batch_autotune.argtypes = [vp, C.c_int, _dp]
+        L.fz_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)]
+        L.fz_batch_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)]
         L.fz_get_cluster_layout.argtypes = [vp, _ip, _ip, _ip, C.c_int64, C.POINTER(C.c_int64)]
         L.fz_get_cluster_pairs.argtypes = [vp, _ip, _up, C.c_int64]
         L.fz_get_cluster_pairs.restype = C.c_int64
@@ -397,6 +401,13 @@ class System:
         _check(lib().fz_autotune(self.h, int(nsteps), _p(out, C.c_double)), "fz_autotune")
         return dict(column_width=out[0], fft_lines=int(out[1]), pair_grid=int(out[2]), ms_per_step=out[3])
 
+    def set_launch(self, fft_lines=-1, pair_grid=-1, pme_path=-1):
+        """fz_set_launch; returns the shapes in effect (fft_lines, pair_grid, pme_path)."""
+        out = np.zeros(3, np.int32)
+        _check(lib().fz_set_launch(self.h, int(fft_lines), int(pair_grid), int(pme_path), _p(out, C.c_int32)),
+               "fz_set_launch")
+        return dict(fft_lines=int(out[0]), pair_grid=int(out[1]), pme_path=int(out[2]))
+
     def set_neighbor_capacity(self, pairs_cap=0, clusters_cap=0):
         """Neighbour capacities (0 keeps one); the list held stays the same."""
         _check(lib().fz_set_neighbor_capacity(self.h, int(pairs_cap), int(clusters_cap)), "fz_set_neighbor_capacity")
@@ -458,6 +469,13 @@ class Batch:
         out = np.zeros(4)
         _check(lib().fz_batch_autotune(self.h, int(nsteps), _p(out, C.c_double)), "fz_batch_autotune")
         return dict(column_width=out[0], fft_lines=int(out[1]), pair_grid=int(out[2]), ms_per_step=out[3])
+
+    def set_launch(self, fft_lines=-1, pair_grid=-1, pme_path=-1):
+        """fz_batch_set_launch; returns the shapes in effect."""
+        out = np.zeros(3, np.int32)
+        _check(lib().fz_batch_set_launch(self.h, int(fft_lines), int(pair_grid), int(pme_path), _p(out, C.c_int32)),
+               "fz_batch_set_launch")
+        return dict(fft_lines=int(out[0]), pair_grid=int(out[1]), pme_path=int(out[2]))
 
     def close(self):
         if self.h:

@@ -207,7 +207,8 @@ int fz_set_neighbor_capacity(fz_system *sys, int64_t pairs_cap, int32_t clusters
  * neighbour rebuild (early exits when none is due), 2 sorted-coordinate gather
  * (run here every step; replayed dynamics steps keep the copies in phase 0 and
  * the rebuild), 3 pair kernel, 4 bonded + exclusion
- * terms, 5 PME spread, 6 PME FFT + convolution, 7 PME force gather,
+ * terms, 5 PME spread (on the plane path with the x and y transforms), 6 PME
+ * FFT + convolution (plane path: z pass and inverse planes), 7 PME force gather,
  * 8 second kick + RATTLE.  Replaces the reference's PROFILING clock() buckets
  * (src/globals.h:796-823). */
 #define FZ_NPHASE 9
@@ -219,6 +220,20 @@ int fz_phase_times(fz_system *sys, int nsteps, double *ms);
  * Replaces the compile-time CELL_BLOCKSIZE / CELL_ATOMS presets of the
  * reference's configure step (src/configure:89-137). */
 int fz_autotune(fz_system *sys, int nsteps, double *out);
+/* Launch shapes set directly (fz_autotune chooses the first two, and the
+ * column width, which this call keeps): FFT lines per workgroup and persistent
+ * pair-kernel blocks (0 = default, < 0 = keep), PME path (< 0 = keep):
+ * FZ_PME_AUTO, the plane kernels for systems of up to 65,536 atoms whose grid
+ * plane fits in 64 KiB of LDS and grid atomics or bricks otherwise;
+ * FZ_PME_GRID; FZ_PME_PLANES (FZ_ERR_ARG where a plane does not fit).  All
+ * paths give the same grid sums (bitwise in the mixed form).  in_effect
+ * (nullable, 3 entries): FFT lines, pair blocks, PME path (FZ_PME_GRID or
+ * FZ_PME_PLANES) in use.  Replaces the same compile-time presets
+ * (src/configure:89-137). */
+#define FZ_PME_AUTO 0
+#define FZ_PME_GRID 1
+#define FZ_PME_PLANES 2
+int fz_set_launch(fz_system *sys, int fft_lines, int pair_grid, int pme_path, int32_t *in_effect);
 
 /* ---- inspection (parity tests) ------------------------------------------------ */
 /* Column grid and cluster layout of the last neighbour build: column of each
@@ -263,6 +278,8 @@ int fz_batch_minimize(fz_batch *b, int nsteps, double max_step);
 int fz_batch_minimize_ex(fz_batch *b, int nsteps, const fz_min_params *p);
 /* fz_autotune over the whole batch (one setting for all members). */
 int fz_batch_autotune(fz_batch *b, int nsteps, double *out);
+/* fz_set_launch for the whole batch. */
+int fz_batch_set_launch(fz_batch *b, int fft_lines, int pair_grid, int pme_path, int32_t *in_effect);
 void fz_batch_destroy(fz_batch *b);
 
 /* ---- serving: continuous batching of MD jobs ------------------------------------------ */

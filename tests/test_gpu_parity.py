@@ -193,31 +193,80 @@ def test_second_engine_over_a_built_system_steps_identically(ctx):
     b.close()
 
 
-def test_brick_and_atomic_spread_give_the_same_trajectory(ctx):
-    """A launch over 64K atoms or more spreads PME charges by grid bricks, a smaller
-    one by global atomics: three ~23K-atom systems run alone (atomic path) and as
-    one batch (brick path) follow bitwise identical mixed-form trajectories."""
+def test_brick_atomic_and_plane_spread_give_the_same_trajectory(ctx):
+    """Three PME paths: a launch over 64K atoms or more spreads charges by grid
+    bricks, a smaller one by global atomics, and systems up to 64K atoms by
+    default by the plane kernels (spread and x, y transforms per z plane in
+    LDS).  Three ~23K-atom systems run alone (planes, then atomics) and as one
+    batch (bricks) follow bitwise identical mixed-form trajectories."""
     # PME grids that are not multiples of the brick edge: blocks wrap across the boundary
     cfgs = [systems.slab(n_chains=6, n_waters=7700, L=62.0, seed=20 + k, K=(60, 54, 50)) for k in range(3)]
     assert sum(int(c[0]["n_atoms"]) for c in cfgs) >= 65536
     assert max(int(c[0]["n_atoms"]) for c in cfgs) < 65536
-    alone = []
-    for topo, xyz, params in cfgs:
-        s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
-        s.reassign_velocities(300.0, 6, 0)
-        s.step(10)
-        alone.append(s.get_fixed())
-        s.close()
+    runs = {}
+    for path in (fenzi.PME_PLANES, fenzi.PME_GRID):
+        runs[path] = []
+        for topo, xyz, params in cfgs:
+            s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+            eff = s.set_launch(pme_path=fenzi.PME_AUTO)
+            assert eff["pme_path"] == fenzi.PME_PLANES
+            assert s.set_launch(pme_path=path)["pme_path"] == path
+            s.reassign_velocities(300.0, 6, 0)
+            s.step(10)
+            runs[path].append(s.get_fixed())
+            s.close()
     mix = []
     for topo, xyz, params in cfgs:
         s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
         s.reassign_velocities(300.0, 6, 0)
         mix.append(s)
     b = fenzi.Batch(ctx, mix)
+    assert b.set_launch(pme_path=fenzi.PME_GRID)["pme_path"] == fenzi.PME_GRID
     b.step(10)
-    for s, u_ref in zip(mix, alone):
-        assert np.array_equal(s.get_fixed(), u_ref)
+    for k, s in enumerate(mix):
+        assert np.array_equal(runs[fenzi.PME_PLANES][k], runs[fenzi.PME_GRID][k])
+        assert np.array_equal(s.get_fixed(), runs[fenzi.PME_GRID][k])
     b.close()
+
+
+@pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
+@pytest.mark.parametrize("make", ["water_box", "slab"])
+def test_plane_kernels_match_the_grid_passes(ctx, prec, make):
+    """The PME plane kernels against the spread + x/y passes on the same
+    positions: the mixed form bitwise (the same integer grid and line
+    transforms), the FP64 form to rounding (its LDS sums run in another order);
+    then a few steps on each path, bitwise in the mixed form."""
+    topo, xyz, params = getattr(systems, make)()
+    out = {}
+    for path in (fenzi.PME_GRID, fenzi.PME_PLANES):
+        s = fenzi.System(ctx, topo, params, prec, xyz=xyz)
+        assert s.set_launch(pme_path=path)["pme_path"] == path
+        f, e = s.forces()
+        s.reassign_velocities(300.0, 4, 0)
+        s.step(6)
+        out[path] = (f, e["pme"], e["virial"], s.get_fixed())
+        s.close()
+    (fa, ea, va, ua), (fb, eb, vb, ub) = out[fenzi.PME_GRID], out[fenzi.PME_PLANES]
+    if prec == fenzi.MIXED:
+        assert np.array_equal(fa, fb)
+        assert ea == eb and np.array_equal(va, vb)
+        assert np.array_equal(ua, ub)
+    else:
+        assert systems.rel_rms(fb, fa) < 1e-13
+        assert abs(eb - ea) <= 1e-12 * abs(ea)
+        assert np.max(np.abs(vb - va)) <= 1e-11 * np.max(np.abs(va))
+
+
+def test_plane_path_refused_where_a_plane_does_not_fit(ctx):
+    """FZ_PME_PLANES on a grid whose plane needs more than 64 KiB of LDS is an
+    argument error; the automatic choice then keeps the grid passes."""
+    topo, xyz, params = systems.water_box()
+    big = dict(params, K=(96, 96, 24))
+    s = fenzi.System(ctx, topo, big, fenzi.MIXED, xyz=xyz)
+    with pytest.raises(fenzi.FenziError):
+        s.set_launch(pme_path=fenzi.PME_PLANES)
+    assert s.set_launch()["pme_path"] == fenzi.PME_GRID
+    s.close()
 
 
 @pytest.mark.parametrize("prec", [fenzi.FP64, fenzi.MIXED])
