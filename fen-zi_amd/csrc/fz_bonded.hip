@@ -5,8 +5,9 @@
 // (src/bonded_forces.cu:597,1384), which evaluates every term once per member
 // atom (a dihedral 4x), and ewaldCorrection (src/mdcuda.cu:520).  Terms are
 // evaluated once, in FP64 in both precision forms, and their forces are added
-// to the per-slot accumulators (fixed point in the mixed form, so the sums are
-// order independent).  Functional forms: src/bonded_forces.cu:22-392 (bond
+// to the atom-ordered bonded sums (fixed point in the mixed form, so the sums
+// are order independent), which need no slot layout: the kernel runs beside
+// the neighbour build.  Functional forms: src/bonded_forces.cu:22-392 (bond
 // K(r-r0)^2; angle K(theta-theta0)^2 with cos clamped at 0.9999; UB K(S-S0)^2;
 // dihedral K(1+cos(n phi - delta)); improper K psi^2) and src/mdcuda.cu:534-552
 // (exclusion correction -CC qi qj erf(beta r)/r).
@@ -29,7 +30,7 @@ __device__ __forceinline__ void cross3(const double* a, const double* b, double*
 
 // Forces of a block's terms are first summed in LDS over the block's atom
 // window (consecutive terms share atoms: chain neighbours, the members of one
-// water), then added to the slot accumulators once per atom and component.
+// water), then added to the atom-ordered sums once per atom and component.
 // A block whose terms span more than BOND_WIN atoms adds directly.  In the
 // mixed form the LDS sums are fixed-point integers, so the result is the same
 // bit for bit either way.
@@ -54,11 +55,11 @@ __device__ __forceinline__ void addf(const Sink<P>& sk, const SysDev& d, int ato
     lds_add(sk.lacc + 2 * BOND_WIN + k, f[2]);
     return;
   }
-  typename P::acc_t* F = (typename P::acc_t*)d.Fs;
-  const int sl = d.atom_slot[atom], cap = d.n_slots_cap;
-  P::add(F + sl, f[0]);
-  P::add(F + cap + sl, f[1]);
-  P::add(F + 2 * cap + sl, f[2]);
+  typename P::acc_t* F = (typename P::acc_t*)d.Fa;
+  const int n = d.n_atoms;
+  P::add(F + atom, f[0]);
+  P::add(F + n + atom, f[1]);
+  P::add(F + 2 * n + atom, f[2]);
 }
 
 __device__ __forceinline__ void add_acc(unsigned long long* p, unsigned long long v) { atomicAdd(p, v); }
@@ -290,15 +291,13 @@ __global__ __launch_bounds__(256) void k_bonded(const SysDev* __restrict__ S, in
   } while (false);
   __syncthreads();
   if (span) {
-    typename P::acc_t* F = (typename P::acc_t*)d.Fs;
-    const int cap = d.n_slots_cap;
+    typename P::acc_t* F = (typename P::acc_t*)d.Fa + sk.w0;
+    const int n = d.n_atoms;
     for (int k = threadIdx.x; k < span; k += 256) {
       const typename P::acc_t fx = lacc[k], fy = lacc[BOND_WIN + k], fz = lacc[2 * BOND_WIN + k];
-      if (fx == 0 && fy == 0 && fz == 0) continue;
-      const int sl = d.atom_slot[sk.w0 + k];
-      if (fx != 0) add_acc(F + sl, fx);
-      if (fy != 0) add_acc(F + cap + sl, fy);
-      if (fz != 0) add_acc(F + 2 * cap + sl, fz);
+      if (fx != 0) add_acc(F + k, fx);
+      if (fy != 0) add_acc(F + n + k, fy);
+      if (fz != 0) add_acc(F + 2 * n + k, fz);
     }
   }
   if (EV) {

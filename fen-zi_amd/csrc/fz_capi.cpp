@@ -172,15 +172,23 @@ int fz_compute_forces(fz_system* s, double* f, fz_energies* e) {
     const SysDev& d = s->sys.dv;
     const int n = d.n_atoms, ns = d.n_slots_cap;
     std::vector<int> slot(n);
-    std::vector<unsigned long long> F((size_t)3 * ns);
+    std::vector<unsigned long long> F((size_t)3 * ns), B((size_t)3 * n), G((size_t)3 * n);
     FZ_HIP(hipMemcpy(slot.data(), d.atom_slot, n * sizeof(int), hipMemcpyDeviceToHost));
     FZ_HIP(hipMemcpy(F.data(), d.Fs, F.size() * 8, hipMemcpyDeviceToHost));
+    FZ_HIP(hipMemcpy(B.data(), d.Fa, B.size() * 8, hipMemcpyDeviceToHost));
+    FZ_HIP(hipMemcpy(G.data(), d.Fg, G.size() * 8, hipMemcpyDeviceToHost));
+    // pair (slot order) + bonded + gather (atom order), as the integrator adds them
     for (int i = 0; i < n; i++)
       for (int k = 0; k < 3; k++) {
-        unsigned long long v = F[(size_t)k * ns + slot[i]];
+        const unsigned long long p = F[(size_t)k * ns + slot[i]], b = B[(size_t)k * n + i], g = G[(size_t)k * n + i];
         double x;
-        if (s->sys.mixed) x = (double)(long long)v / 4294967296.0;
-        else memcpy(&x, &v, 8);
+        if (s->sys.mixed) {
+          x = (double)(long long)(p + b + g) / 4294967296.0;
+        } else {
+          double xp, xb, xg;
+          memcpy(&xp, &p, 8); memcpy(&xb, &b, 8); memcpy(&xg, &g, 8);
+          x = (xp + xb) + xg;
+        }
         f[3 * i + k] = x;
       }
   }

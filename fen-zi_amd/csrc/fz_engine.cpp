@@ -461,6 +461,8 @@ int System::alloc_device() {
   }
   FZ_TRY(dalloc(&d.acc, A_NSLOT));
   FZ_TRY(dalloc(&d.acc_ev, (size_t)EV_SPREAD * EV_W));
+  FZ_TRY(dalloc((unsigned long long**)&d.Fa, (size_t)3 * n));
+  FZ_TRY(dalloc((unsigned long long**)&d.Fg, (size_t)3 * n));
   FZ_TRY(dalloc(&d.flags, F_NSLOT));
   FZ_TRY(dalloc(&d.mstep, 2));
   FZ_TRY(dalloc(&d.mH, (size_t)3 * n));
@@ -594,7 +596,7 @@ int64_t footprint_bytes(const fz_topology* t, const fz_params* p, int precision)
   b += rnd256(8 * K3) + rnd256(KH * (mixed ? 8 : 16)) + rnd256(K3 * (mixed ? 4 : 8));
   b += rnd256(4 * nbr) + rnd256(4 * (nbr + 1)) + rnd256(n * 16 * (mixed ? 4 : 8)) + rnd256(nbr * 1331 * 8);
   b += 3 * rnd256(8 * KS) + rnd256(KS * (mixed ? 8 : 16));
-  b += rnd256(8 * A_NSLOT) + rnd256(8 * EV_SPREAD * EV_W) + rnd256(4 * F_NSLOT) + rnd256(16) + 2 * rnd256(24 * n);
+  b += rnd256(8 * A_NSLOT) + rnd256(8 * EV_SPREAD * EV_W) + 2 * rnd256(24 * n) + rnd256(4 * F_NSLOT) + rnd256(16) + 2 * rnd256(24 * n);
   b += rnd256(72 * n + 16);                                  // snapshot
   // the neighbour block after two growth steps of the overflow path (pair-list
   // entries x1.6 + 4096, clusters x1.3 + 64 each): clustered chains can need
@@ -837,7 +839,6 @@ Engine::~Engine() {
     if (ev_join_[k]) (void)hipEventDestroy(ev_join_[k]);
     if (side_[k]) (void)hipStreamDestroy(side_[k]);
   }
-  if (ev_fork_) (void)hipEventDestroy(ev_fork_);
   if (ev_early_) (void)hipEventDestroy(ev_early_);
   if (L_.stream) (void)hipStreamDestroy(L_.stream);
 }
@@ -886,7 +887,6 @@ int Engine::prepare() {
     if (!side_[k]) FZ_HIP(hipStreamCreateWithFlags(&side_[k], hipStreamNonBlocking));
     if (!ev_join_[k]) FZ_HIP(hipEventCreateWithFlags(&ev_join_[k], hipEventDisableTiming));
   }
-  if (!ev_fork_) FZ_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
   if (!ev_early_) FZ_HIP(hipEventCreateWithFlags(&ev_early_, hipEventDisableTiming));
   if (!d_sys_) FZ_HIP(hipMalloc(&d_sys_, ns * sizeof(SysDev)));
   if (!L_.chunk_pre) {
@@ -1000,17 +1000,16 @@ int Engine::ensure() {
 }
 
 void Engine::record_forces(bool ev, bool lite, bool check) {
-  // Fork: bonded and PME beside the pair kernel (errors of these calls surface
-  // at the stream's next synchronisation or at capture end).  The charge
-  // spread (either path) visits atoms, not cluster slots, so the PME branch
-  // starts before the list build and only its force gather waits for the new
-  // slots.
+  // Fork: bonded and PME beside the neighbour build and the pair kernel
+  // (errors of these calls surface at the stream's next synchronisation or at
+  // capture end).  Their forces go to atom-ordered sums, and the charge spread
+  // and gather visit atoms, so neither branch waits for the build's slot
+  // layout: both start with the build.
   Launch side[2] = {L_, L_};
   for (int k = 0; k < 2; k++) side[k].stream = side_[k];
   (void)hipEventRecord(ev_early_, L_.stream);
-  (void)hipStreamWaitEvent(side_[1], ev_early_, 0);
-  launch_pme_part(side[1], 0, ev);
-  launch_pme_part(side[1], 1, ev);
+  for (int k = 0; k < 2; k++) (void)hipStreamWaitEvent(side_[k], ev_early_, 0);
+  // the critical path (build, pair kernel) is recorded first
   launch_nblist(L_, check);
   // In dynamics the slot copies are kept current by k_step_first and the list
   // build, and a single system's work items are laid out by the build, so
@@ -1018,12 +1017,10 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   // its work items with one single-thread kernel.
   if (check) launch_gather_sorted(L_);
   else if (L_.nsys > 1) launch_chunk_prefix(L_);
-  (void)hipEventRecord(ev_fork_, L_.stream);
-  for (int k = 0; k < 2; k++) (void)hipStreamWaitEvent(side_[k], ev_fork_, 0);
-  // the pair kernel, on the critical path, is recorded before the side
-  // branches' kernels (measured: the replayed step is shorter that way)
   launch_pair(L_, ev ? (lite ? 1 : 2) : 0);
+  launch_pme_part(side[1], 0, ev);
   launch_bonded(side[0], ev);
+  launch_pme_part(side[1], 1, ev);
   launch_pme_part(side[1], 2, ev);
   for (int k = 0; k < 2; k++) {
     (void)hipEventRecord(ev_join_[k], side_[k]);

@@ -160,15 +160,14 @@ class Engine {
   // as much as a small system's whole step, so steps are launched in blocks.
   static constexpr int GRAPH_STEPS = 8;
   hipGraphExec_t g_plain_k_ = nullptr, g_npt_k_ = nullptr;
-  // The force evaluation forks after the list build: the pair kernel on the
-  // main stream, bonded terms and the PME chain on two side streams (default
-  // priority), joined before the second half kick.  The PME branch forks
-  // before the build (ev_early_) and only its force gather waits for the
-  // build's slots (ev_fork_).  Every force and energy
-  // goes into fixed-point (mixed) or FP64 accumulators by atomic adds, so the
-  // branches need no other ordering.
+  // The force evaluation forks before the list build (ev_early_): the build
+  // and the pair kernel on the main stream, bonded terms and the PME chain on
+  // two side streams (default priority), joined before the second half kick.
+  // The side branches write atom-ordered force sums and the pair kernel
+  // slot-ordered ones; energies and virials go into fixed-point (mixed) or
+  // FP64 accumulators by atomic adds, so the branches need no other ordering.
   hipStream_t side_[2] = {nullptr, nullptr};
-  hipEvent_t ev_fork_ = nullptr, ev_early_ = nullptr, ev_join_[2] = {nullptr, nullptr};
+  hipEvent_t ev_early_ = nullptr, ev_join_[2] = {nullptr, nullptr};
   bool prepared_ = false;
   std::vector<int> gens_;
   int tuned_nl_ = 0, tuned_grid_ = 0;  // autotuned launch shapes (0 = defaults)

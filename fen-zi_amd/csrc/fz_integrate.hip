@@ -13,18 +13,41 @@
 
 namespace fz {
 
+// Force on atom a (slot sl): the slot-ordered sum of the pair kernel (and
+// restraints) plus the atom-ordered bonded sum and PME gather term.  In the
+// mixed form the three int64 fixed-point parts add as integers, so the force
+// is the same whatever part a term went to.  zero: the pair and bonded sums
+// are cleared (the gather term is overwritten by every evaluation).
 template <bool MIXED>
-__device__ __forceinline__ double take_force(const SysDev& d, int sl, int comp, bool zero) {
+__device__ __forceinline__ double take_force(const SysDev& d, int sl, int a, int comp, bool zero) {
+  const size_t ia = (size_t)comp * d.n_atoms + a;
   if (MIXED) {
     unsigned long long* F = (unsigned long long*)d.Fs + (size_t)comp * d.n_slots_cap + sl;
-    double v = from_fixed_acc(*F);
-    if (zero) *F = 0ull;
-    return v;
+    unsigned long long* B = (unsigned long long*)d.Fa + ia;
+    const unsigned long long v = *F + *B + ((const unsigned long long*)d.Fg)[ia];
+    if (zero) { *F = 0ull; *B = 0ull; }
+    return from_fixed_acc(v);
   } else {
     double* F = (double*)d.Fs + (size_t)comp * d.n_slots_cap + sl;
-    double v = *F;
-    if (zero) *F = 0.0;
+    double* B = (double*)d.Fa + ia;
+    const double v = (*F + *B) + ((const double*)d.Fg)[ia];
+    if (zero) { *F = 0.0; *B = 0.0; }
     return v;
+  }
+}
+
+// clears the pair and bonded force sums of atom a (slot sl)
+template <bool MIXED>
+__device__ __forceinline__ void clear_force(const SysDev& d, int sl, int a) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (MIXED) {
+      ((unsigned long long*)d.Fs)[(size_t)c * d.n_slots_cap + sl] = 0ull;
+      ((unsigned long long*)d.Fa)[(size_t)c * d.n_atoms + a] = 0ull;
+    } else {
+      ((double*)d.Fs)[(size_t)c * d.n_slots_cap + sl] = 0.0;
+      ((double*)d.Fa)[(size_t)c * d.n_atoms + a] = 0.0;
+    }
   }
 }
 
@@ -38,8 +61,8 @@ template <bool MIXED, bool FINISH = false>
 __device__ __forceinline__ void kick_drift_atom(const SysDev& d, int i) {
   const int sl = d.atom_slot[i];
   const double h = 0.5 * d.dt * d.mdyn[i];
-  const double fx = take_force<MIXED>(d, sl, 0, true), fy = take_force<MIXED>(d, sl, 1, true),
-               fz = take_force<MIXED>(d, sl, 2, true);
+  const double fx = take_force<MIXED>(d, sl, i, 0, true), fy = take_force<MIXED>(d, sl, i, 1, true),
+               fz = take_force<MIXED>(d, sl, i, 2, true);
   double vx = d.vx[i], vy = d.vy[i], vz = d.vz[i];
   if (FINISH) { vx = fma(h, fx, vx); vy = fma(h, fy, vy); vz = fma(h, fz, vz); }
   vx = fma(h, fx, vx); vy = fma(h, fy, vy); vz = fma(h, fz, vz);
@@ -56,9 +79,9 @@ template <bool MIXED>
 __device__ __forceinline__ void kick_atom(const SysDev& d, int i) {
   const int sl = d.atom_slot[i];
   const double h = 0.5 * d.dt * d.mdyn[i];
-  d.vx[i] = fma(h, take_force<MIXED>(d, sl, 0, false), d.vx[i]);
-  d.vy[i] = fma(h, take_force<MIXED>(d, sl, 1, false), d.vy[i]);
-  d.vz[i] = fma(h, take_force<MIXED>(d, sl, 2, false), d.vz[i]);
+  d.vx[i] = fma(h, take_force<MIXED>(d, sl, i, 0, false), d.vx[i]);
+  d.vy[i] = fma(h, take_force<MIXED>(d, sl, i, 1, false), d.vy[i]);
+  d.vz[i] = fma(h, take_force<MIXED>(d, sl, i, 2, false), d.vz[i]);
 }
 
 // ---- constraint clusters ------------------------------------------------------
@@ -336,12 +359,13 @@ __device__ __forceinline__ bool step_first_ccl(const SysDev& d, const CCl& k) {
   unsigned long long* U[3] = {d.ux, d.uy, d.uz};
   const unsigned long long* R[3] = {d.rx, d.ry, d.rz};
   double* Vp[3] = {d.vx, d.vy, d.vz};
-  int sl[4];
+  int sl[4], at[4];
   double mi[4], v[4][3], f[4][3];
   unsigned long long u[4][3], o[4][3];
 #pragma unroll
   for (int m = 0; m < 4; m++) {
     const int a = k.at[m] >= 0 ? k.at[m] : k.at[0];  // padding repeats the first atom and is not stored
+    at[m] = a;
     sl[m] = d.atom_slot[a];
     mi[m] = k.at[m] >= 0 ? d.mdyn[a] : 0.0;
 #pragma unroll
@@ -350,7 +374,7 @@ __device__ __forceinline__ bool step_first_ccl(const SysDev& d, const CCl& k) {
 #pragma unroll
   for (int m = 0; m < 4; m++)
 #pragma unroll
-    for (int c = 0; c < 3; c++) f[m][c] = take_force<MIXED>(d, sl[m], c, false);
+    for (int c = 0; c < 3; c++) f[m][c] = take_force<MIXED>(d, sl[m], at[m], c, false);
   const double h0 = 0.5 * d.dt;
   if (FINISH) {
     V3 r[4], w[4];
@@ -404,13 +428,7 @@ __device__ __forceinline__ bool step_first_ccl(const SysDev& d, const CCl& k) {
     }
     move = move || r2 > d.skin2;
     store_slot_pos(d, sl[m], un3[0], un3[1], un3[2], d.q[a]);  // the pair kernel's slot copy
-    if (MIXED) {
-#pragma unroll
-      for (int c = 0; c < 3; c++) ((unsigned long long*)d.Fs)[(size_t)c * d.n_slots_cap + sl[m]] = 0ull;
-    } else {
-#pragma unroll
-      for (int c = 0; c < 3; c++) ((double*)d.Fs)[(size_t)c * d.n_slots_cap + sl[m]] = 0.0;
-    }
+    clear_force<MIXED>(d, sl[m], a);
   }
   return move;
 }
@@ -578,7 +596,7 @@ __global__ __launch_bounds__(128) void k_npt_b(const SysDev* __restrict__ S, int
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         u[j][c] = U[c][a];
-        v[j][c] = fma(hm, take_force<MIXED>(d, sl, c, false), Vp[c][a]);
+        v[j][c] = fma(hm, take_force<MIXED>(d, sl, a, c, false), Vp[c][a]);
       }
     }
     ke[0] = npt_cluster_box(k.na, m, u, v, f1);
@@ -600,7 +618,7 @@ __global__ __launch_bounds__(128) void k_npt_b(const SysDev* __restrict__ S, int
     double v2 = 0.0;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const double v = fma(hm, take_force<MIXED>(d, sl, c, false), Vp[c][i]) * f1;
+      const double v = fma(hm, take_force<MIXED>(d, sl, i, c, false), Vp[c][i]) * f1;
       const unsigned long long u = U[c][i];
       O[c][i] = u;
       U[c][i] = u + (unsigned long long)__double2ll_rn(d.dt * v * isc1[c]);
@@ -726,11 +744,7 @@ __device__ __forceinline__ bool npt_d_ccl(const SysDev& d, const CCl& k, double 
     }
     move = move || r2 > d.skin2;
     store_slot_pos(d, sl[j], un3[0], un3[1], un3[2], d.q[a]);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      if (MIXED) ((unsigned long long*)d.Fs)[(size_t)c * d.n_slots_cap + sl[j]] = 0ull;
-      else ((double*)d.Fs)[(size_t)c * d.n_slots_cap + sl[j]] = 0.0;
-    }
+    clear_force<MIXED>(d, sl[j], a);
   }
   return move;
 }
@@ -749,11 +763,7 @@ __global__ __launch_bounds__(128) void k_npt_d(const SysDev* __restrict__ S, int
     const int sl = d.atom_slot[i];
     d.vx[i] *= f2; d.vy[i] *= f2; d.vz[i] *= f2;
     store_slot_pos(d, sl, d.ux[i], d.uy[i], d.uz[i], d.q[i]);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      if (MIXED) ((unsigned long long*)d.Fs)[(size_t)c * d.n_slots_cap + sl] = 0ull;
-      else ((double*)d.Fs)[(size_t)c * d.n_slots_cap + sl] = 0.0;
-    }
+    clear_force<MIXED>(d, sl, i);
     move = moved_atom(d, i);
   }
   if (__any(move) && (threadIdx.x & 63) == 0) d.flags[F_REBUILD] = 1;
@@ -778,7 +788,7 @@ __global__ __launch_bounds__(128) void k_npt_intra(const SysDev* __restrict__ S,
       const int sl = d.atom_slot[a];
       m[j] = k.at[j] >= 0 ? 1.0 / d.minv[a] : 0.0;
 #pragma unroll
-      for (int c = 0; c < 3; c++) f[j][c] = take_force<MIXED>(d, sl, c, false);
+      for (int c = 0; c < 3; c++) f[j][c] = take_force<MIXED>(d, sl, a, c, false);
       M += m[j];
       Rc[0] += m[j] * r[j].x; Rc[1] += m[j] * r[j].y; Rc[2] += m[j] * r[j].z;
     }
@@ -921,8 +931,8 @@ __global__ __launch_bounds__(256) void k_fmax(const SysDev* __restrict__ S, int 
   double f2 = 0.0;
   if (i < d.n_atoms && d.mdyn[i] != 0.0) {
     const int sl = d.atom_slot[i];
-    const double fx = take_force<MIXED>(d, sl, 0, false), fy = take_force<MIXED>(d, sl, 1, false),
-                 fz = take_force<MIXED>(d, sl, 2, false);
+    const double fx = take_force<MIXED>(d, sl, i, 0, false), fy = take_force<MIXED>(d, sl, i, 1, false),
+                 fz = take_force<MIXED>(d, sl, i, 2, false);
     f2 = fx * fx + fy * fy + fz * fz;
   }
 #pragma unroll
@@ -944,9 +954,9 @@ __global__ __launch_bounds__(256) void k_min_move(const SysDev* __restrict__ S, 
   const unsigned long long ux = d.ux[i], uy = d.uy[i], uz = d.uz[i];
   d.ox[i] = ux; d.oy[i] = uy; d.oz[i] = uz;
   if (d.mdyn[i] == 0.0) return;  // fixed atom
-  d.ux[i] = ux + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, 0, false) * d.isc[0]);
-  d.uy[i] = uy + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, 1, false) * d.isc[1]);
-  d.uz[i] = uz + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, 2, false) * d.isc[2]);
+  d.ux[i] = ux + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, i, 0, false) * d.isc[0]);
+  d.uy[i] = uy + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, i, 1, false) * d.isc[1]);
+  d.uz[i] = uz + (unsigned long long)__double2ll_rn(a * take_force<MIXED>(d, sl, i, 2, false) * d.isc[2]);
 }
 
 // The reference's minimiser moves (ConjugatedGradient / SteepestDescent,
@@ -966,8 +976,8 @@ __global__ __launch_bounds__(256) void k_min_ref_move(const SysDev* __restrict__
   if (method == 0 || d.mdyn[i] == 0.0) return;  // no move / fixed atom
   const double fac = d.mstep[0];
   const int sl = d.atom_slot[i];
-  const double F[3] = {take_force<MIXED>(d, sl, 0, false), take_force<MIXED>(d, sl, 1, false),
-                       take_force<MIXED>(d, sl, 2, false)};
+  const double F[3] = {take_force<MIXED>(d, sl, i, 0, false), take_force<MIXED>(d, sl, i, 1, false),
+                       take_force<MIXED>(d, sl, i, 2, false)};
   double h[3];
   if (method == 1) {
     double p[3] = {d.mF[i], d.mF[n + i], d.mF[2 * n + i]};
@@ -1035,7 +1045,8 @@ __global__ void k_zero_acc(const SysDev* __restrict__ S, int nsys) {
   if (threadIdx.x < A_NSLOT) d.acc[threadIdx.x] = 0ull;
 }
 
-// clears the slot-ordered force accumulators (used before a stand-alone evaluation)
+// clears the slot-ordered pair and atom-ordered bonded force sums (before a
+// stand-alone evaluation)
 __global__ __launch_bounds__(256) void k_zero_forces(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
@@ -1045,6 +1056,12 @@ __global__ __launch_bounds__(256) void k_zero_forces(const SysDev* __restrict__ 
   F[sl] = 0ull;
   F[d.n_slots_cap + sl] = 0ull;
   F[2 * d.n_slots_cap + sl] = 0ull;
+  unsigned long long* B = (unsigned long long*)d.Fa;
+  for (int a = sl; a < d.n_atoms; a += d.n_slots_cap) {
+    B[a] = 0ull;
+    B[d.n_atoms + a] = 0ull;
+    B[2 * d.n_atoms + a] = 0ull;
+  }
 }
 
 void launch_shake(const Launch& L) {

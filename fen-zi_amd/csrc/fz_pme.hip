@@ -774,17 +774,28 @@ __global__ __launch_bounds__(NT) void k_pme_plane_inv(const SysDev* __restrict__
   for (int e = threadIdx.x; e < KP; e += NT) phi[e] = r2[e].x;
 }
 
+// Force gather, one thread per atom, into the atom-ordered gather term (a
+// store: each atom has one): it needs no slot layout, so it runs beside the
+// neighbour build.
 template <bool MIXED>
 __global__ __launch_bounds__(256) void k_pme_gather(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
   using R = typename std::conditional<MIXED, float, double>::type;
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int sl = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
-  if (sl >= *d.n_clusters * CLUSTER) return;
-  const int a = d.slot_atom[sl];
-  if (a < 0) return;
+  const int a = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
+  if (a >= d.n_atoms) return;
   const double qd = d.q[a];
-  if (qd == 0.0) return;
+  const int n = d.n_atoms;
+  if (qd == 0.0) {
+    if (MIXED) {
+      unsigned long long* F = (unsigned long long*)d.Fg;
+      F[a] = 0ull; F[n + a] = 0ull; F[2 * n + a] = 0ull;
+    } else {
+      double* F = (double*)d.Fg;
+      F[a] = 0.0; F[n + a] = 0.0; F[2 * n + a] = 0.0;
+    }
+    return;
+  }
   const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
   R fx, fy, fz, tx[4], ty[4], tz[4], dx[4], dy[4], dz[4];
   const int gx = spline_pos<R>(d.ux[a], K0, fx);
@@ -820,17 +831,16 @@ __global__ __launch_bounds__(256) void k_pme_gather(const SysDev* __restrict__ S
   const double fxv = -qd * (double)g0 * K0 / d.L[0];
   const double fyv = -qd * (double)g1 * K1 / d.L[1];
   const double fzv = -qd * (double)g2 * K2 / d.L[2];
-  const int cap = d.n_slots_cap;
   if (MIXED) {
-    unsigned long long* F = (unsigned long long*)d.Fs;
-    atomicAdd(F + sl, to_fixed_acc(fxv));
-    atomicAdd(F + cap + sl, to_fixed_acc(fyv));
-    atomicAdd(F + 2 * cap + sl, to_fixed_acc(fzv));
+    unsigned long long* F = (unsigned long long*)d.Fg;
+    F[a] = to_fixed_acc(fxv);
+    F[n + a] = to_fixed_acc(fyv);
+    F[2 * n + a] = to_fixed_acc(fzv);
   } else {
-    double* F = (double*)d.Fs;
-    unsafeAtomicAdd(F + sl, fxv);
-    unsafeAtomicAdd(F + cap + sl, fyv);
-    unsafeAtomicAdd(F + 2 * cap + sl, fzv);
+    double* F = (double*)d.Fg;
+    F[a] = fxv;
+    F[n + a] = fyv;
+    F[2 * n + a] = fzv;
   }
 }
 
@@ -875,8 +885,8 @@ void launch_pme_part(const Launch& L, int part, bool ev) {
     if (L.mixed) launch_fft_passes<true>(L, ev, 0);
     else launch_fft_passes<false>(L, ev, 0);
   } else {
-    if (L.mixed) hipLaunchKernelGGL(k_pme_gather<true>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
-    else hipLaunchKernelGGL(k_pme_gather<false>, dim3(L.w_slots.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_slots.pre);
+    if (L.mixed) hipLaunchKernelGGL(k_pme_gather<true>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+    else hipLaunchKernelGGL(k_pme_gather<false>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
   }
 }
 

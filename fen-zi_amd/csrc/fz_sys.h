@@ -114,7 +114,9 @@ struct SysDev {
   int *atom_slot;        // [n_atoms]
   void *xs;              // packed positions+charge per slot (int4 / longlong4)
   int *ts;               // type per slot
-  void *Fs;              // force accumulators, 3 x n_slots_cap (x block, y block, z block)
+  void *Fs;              // pair-kernel (and restraint) force sums, 3 x n_slots_cap (x, y, z blocks)
+  void *Fa;              // bonded force sums, 3 x n_atoms, atom order (cleared by the consumer)
+  void *Fg;              // PME gather forces, 3 x n_atoms, atom order (stored by every evaluation)
   // ---- binning / search --------------------------------------------------
   int *atom_col;         // [n_atoms]
   int *col_count, *col_fill, *col_clu0; // [ncol], [ncol], [ncol + 1]
