@@ -835,11 +835,13 @@ Engine::~Engine() {
   for (int* p : d_pre_) (void)hipFree(p);
   if (d_sys_) (void)hipFree(d_sys_);
   if (L_.chunk_pre) (void)hipFree(L_.chunk_pre);
-  for (int k = 0; k < 2; k++) {
+  if (L_.chunk_pre_a) (void)hipFree(L_.chunk_pre_a);
+  for (int k = 0; k < NSIDE; k++) {
     if (ev_join_[k]) (void)hipEventDestroy(ev_join_[k]);
     if (side_[k]) (void)hipStreamDestroy(side_[k]);
   }
   if (ev_early_) (void)hipEventDestroy(ev_early_);
+  if (ev_split_) (void)hipEventDestroy(ev_split_);
   if (L_.stream) (void)hipStreamDestroy(L_.stream);
 }
 
@@ -883,15 +885,17 @@ int Engine::prepare() {
   const int ns = (int)sys_.size();
   if (ns == 0) return set_error(FZ_ERR_ARG, "empty system set");
   if (!L_.stream) FZ_HIP(hipStreamCreateWithFlags(&L_.stream, hipStreamNonBlocking));
-  for (int k = 0; k < 2; k++) {
+  for (int k = 0; k < NSIDE; k++) {
     if (!side_[k]) FZ_HIP(hipStreamCreateWithFlags(&side_[k], hipStreamNonBlocking));
     if (!ev_join_[k]) FZ_HIP(hipEventCreateWithFlags(&ev_join_[k], hipEventDisableTiming));
   }
   if (!ev_early_) FZ_HIP(hipEventCreateWithFlags(&ev_early_, hipEventDisableTiming));
+  if (!ev_split_) FZ_HIP(hipEventCreateWithFlags(&ev_split_, hipEventDisableTiming));
   if (!d_sys_) FZ_HIP(hipMalloc(&d_sys_, ns * sizeof(SysDev)));
-  if (!L_.chunk_pre) {
-    FZ_HIP(hipMalloc(&L_.chunk_pre, (ns + 1) * sizeof(int)));
-    FZ_HIP(hipMemset(L_.chunk_pre, 0, (ns + 1) * sizeof(int)));
+  for (int** cp : {&L_.chunk_pre, &L_.chunk_pre_a}) {
+    if (*cp) continue;
+    FZ_HIP(hipMalloc(cp, (ns + 1) * sizeof(int)));
+    FZ_HIP(hipMemset(*cp, 0, (ns + 1) * sizeof(int)));
   }
   FZ_TRY(upload_desc());
   L_.d_sys = d_sys_;
@@ -1005,27 +1009,48 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   // capture end).  Their forces go to atom-ordered sums, and the charge spread
   // and gather visit atoms, so neither branch waits for the build's slot
   // layout: both start with the build.
-  Launch side[2] = {L_, L_};
-  for (int k = 0; k < 2; k++) side[k].stream = side_[k];
+  Launch side[NSIDE] = {L_, L_, L_};
+  for (int k = 0; k < NSIDE; k++) side[k].stream = side_[k];
+  // Batch dynamics: the pair kernel runs twice.  On the third side stream, at
+  // once, over the members whose lists stay valid this step (no rebuild
+  // pending); on the main stream, after the build, over the members it
+  // rebuilt.  The build of a few members then overlaps the pair work of the
+  // others.  (For one system the extra launches and cross-stream wait cost
+  // more than the build's launches they hide.)
+  const bool split = !check && L_.nsys > 1;
+  const int nside = split ? 3 : 2;
+  const int pev = ev ? (lite ? 1 : 2) : 0;
   (void)hipEventRecord(ev_early_, L_.stream);
-  for (int k = 0; k < 2; k++) (void)hipStreamWaitEvent(side_[k], ev_early_, 0);
-  // the critical path (build, pair kernel) is recorded first
-  launch_nblist(L_, check);
-  // In dynamics the slot copies are kept current by k_step_first and the list
-  // build, and a single system's work items are laid out by the build, so
-  // only stand-alone evaluations run the slot gather; a batch's step lays out
-  // its work items with one single-thread kernel.
-  if (check) launch_gather_sorted(L_);
-  else if (L_.nsys > 1) launch_chunk_prefix(L_);
-  launch_pair(L_, ev ? (lite ? 1 : 2) : 0);
+  for (int k = 0; k < nside; k++) (void)hipStreamWaitEvent(side_[k], ev_early_, 0);
   launch_pme_part(side[1], 0, ev);
   launch_bonded(side[0], ev);
   launch_pme_part(side[1], 1, ev);
   launch_pme_part(side[1], 2, ev);
-  for (int k = 0; k < 2; k++) {
+  if (split) {
+    launch_chunk_split(side[2], false);
+    (void)hipEventRecord(ev_split_, side_[2]);
+    launch_pair(side[2], pev, L_.chunk_pre_a, false);
+  }
+  launch_nblist(L_, check);
+  if (check) {
+    // stand-alone evaluation: slot copies and every system's work items
+    launch_gather_sorted(L_);
+    launch_pair(L_, pev);
+  } else if (split) {
+    // the split above reads the rebuild decisions before this one spends them
+    (void)hipStreamWaitEvent(L_.stream, ev_split_, 0);
+    launch_chunk_split(L_, true);
+    launch_pair(L_, pev, L_.chunk_pre, false);
+  } else {
+    // one system in dynamics: the slot copies are kept current by k_step_first
+    // and the build, which also lays out the work items
+    launch_pair(L_, pev, L_.chunk_pre, false, true);
+  }
+  for (int k = 0; k < nside; k++) {
     (void)hipEventRecord(ev_join_[k], side_[k]);
     (void)hipStreamWaitEvent(L_.stream, ev_join_[k], 0);
   }
+  if (!check && pev) launch_ev_fold(L_);
   if (L_.com_any) launch_restraint(L_);
 }
 

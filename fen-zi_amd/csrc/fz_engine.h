@@ -166,8 +166,11 @@ class Engine {
   // The side branches write atom-ordered force sums and the pair kernel
   // slot-ordered ones; energies and virials go into fixed-point (mixed) or
   // FP64 accumulators by atomic adds, so the branches need no other ordering.
-  hipStream_t side_[2] = {nullptr, nullptr};
-  hipEvent_t ev_early_ = nullptr, ev_join_[2] = {nullptr, nullptr};
+  // In dynamics a third side stream runs the pair kernel over the systems
+  // without a pending rebuild (ev_split_: their work items are laid out).
+  static constexpr int NSIDE = 3;
+  hipStream_t side_[NSIDE] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_early_ = nullptr, ev_split_ = nullptr, ev_join_[NSIDE] = {nullptr, nullptr, nullptr};
   bool prepared_ = false;
   std::vector<int> gens_;
   int tuned_nl_ = 0, tuned_grid_ = 0;  // autotuned launch shapes (0 = defaults)

@@ -43,6 +43,7 @@ struct Launch {
   Work w_bricks;  // one block per PME brick
   Work w_planes;  // one block per PME grid z plane
   int* chunk_pre = nullptr;  // device: pair-kernel chunks per system (prefix)
+  int* chunk_pre_a = nullptr;  // device: the same for the systems without a pending rebuild
   int pair_grid = 0;         // persistent pair-kernel blocks
   int fft_lds = 0;           // dynamic LDS bytes for the FFT kernels
   int fft_nl = 16;           // lines per FFT workgroup
@@ -58,8 +59,15 @@ struct Launch {
 // check: run the displacement test first (the fused step kernel does it otherwise)
 void launch_nblist(const Launch& L, bool check);
 void launch_gather_sorted(const Launch& L);
-void launch_chunk_prefix(const Launch& L);  // batches: pair-kernel work items, rebuild flags cleared
-void launch_pair(const Launch& L, int ev);  // 0 forces, 1 + FP32 energies, 2 + FP64 energies
+void launch_chunk_prefix(const Launch& L);  // pair-kernel work items of every system, rebuild flags cleared
+// pair-kernel work items of the systems without (built = false, into
+// chunk_pre_a) or with (built = true, into chunk_pre; flags cleared) a pending rebuild
+void launch_chunk_split(const Launch& L, bool built);
+// 0 forces, 1 + FP32 energies, 2 + FP64 energies; work items cp (null: chunk_pre);
+// fold: the energy / virial copies are folded after it (else launch_ev_fold
+// later); spend: the kernel clears a single system's rebuild decision
+void launch_pair(const Launch& L, int ev, const int* cp = nullptr, bool fold = true, bool spend = false);
+void launch_ev_fold(const Launch& L);
 void launch_bonded(const Launch& L, bool ev);
 void launch_pme_fft_only(const Launch& L, int conv_mode);
 void launch_pme_part(const Launch& L, int part, bool ev);  // 0 spread, 1 FFT + convolution, 2 gather

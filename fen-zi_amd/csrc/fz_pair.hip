@@ -35,9 +35,13 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const SysDev* __restrict_
 }
 
 // One block: the pair kernel's work items (per system, the PAIR_CHUNK-entry
-// chunks of its list; none after an overflow) as a prefix over the systems,
-// and the rebuild decisions are spent.  Systems are taken 1024 at a time, one
-// per thread, with a block scan and a running carry.
+// chunks of its list; none after an overflow) as a prefix over the systems.
+// Systems are taken 1024 at a time, one per thread, with a block scan and a
+// running carry.  MODE 0: every system, and the rebuild decisions are spent;
+// MODE 1: only the systems without a pending rebuild (their lists are valid
+// before the build runs); MODE 2: only the systems with one (after the build),
+// and the rebuild decisions are spent.
+template <int MODE>
 __global__ __launch_bounds__(1024) void k_chunk_prefix(const SysDev* __restrict__ S, int nsys, int* __restrict__ chunk_pre) {
   __shared__ int part[1024];
   __shared__ int carry;
@@ -47,9 +51,11 @@ __global__ __launch_bounds__(1024) void k_chunk_prefix(const SysDev* __restrict_
     const int s = base + t;
     int c = 0;
     if (s < nsys) {
-      const long long np = S[s].flags[F_OVERFLOW] ? 0 : *S[s].n_pairs;
+      const bool pending = S[s].flags[F_REBUILD] != 0;
+      const bool take = MODE == 0 || (MODE == 1 ? !pending : pending);
+      const long long np = (S[s].flags[F_OVERFLOW] || !take) ? 0 : *S[s].n_pairs;
       c = (int)((np + PAIR_CHUNK - 1) / PAIR_CHUNK);
-      S[s].flags[F_REBUILD] = 0;
+      if (MODE != 1) S[s].flags[F_REBUILD] = 0;
     }
     part[t] = c;
     __syncthreads();
@@ -67,7 +73,11 @@ __global__ __launch_bounds__(1024) void k_chunk_prefix(const SysDev* __restrict_
 }
 
 void launch_chunk_prefix(const Launch& L) {
-  hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+  hipLaunchKernelGGL(k_chunk_prefix<0>, dim3(1), dim3(1024), 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+}
+void launch_chunk_split(const Launch& L, bool built) {
+  if (built) hipLaunchKernelGGL(k_chunk_prefix<2>, dim3(1), dim3(1024), 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+  else hipLaunchKernelGGL(k_chunk_prefix<1>, dim3(1), dim3(1024), 0, L.stream, L.d_sys, L.nsys, L.chunk_pre_a);
 }
 
 void launch_gather_sorted(const Launch& L) {
@@ -297,9 +307,10 @@ __global__ __launch_bounds__(EV_SPREAD) void k_ev_fold(const SysDev* __restrict_
 // FP32 distance (reported energies).  SEP: every system of the launch follows
 // the combination rule (per-slot LJ parameters, no table lookup).
 template <int EV, bool SEP>
-__global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S, int nsys, const int* __restrict__ chunk_pre) {
-  // the rebuild decision of this step is spent (single-system dynamics runs no slot gather)
-  if (nsys == 1 && blockIdx.x == 0 && threadIdx.x == 0) S[0].flags[F_REBUILD] = 0;
+__global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S, int nsys, const int* __restrict__ chunk_pre,
+                                                    int spend) {
+  // spend: a single system's rebuild decision of this step is spent here
+  if (spend && blockIdx.x == 0 && threadIdx.x == 0) S[0].flags[F_REBUILD] = 0;
   const int lane = threadIdx.x & 63, il = lane & 7, jl = lane >> 3;
   const int total = chunk_pre[nsys];
   const int nw = gridDim.x * 4;
@@ -494,9 +505,9 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
 }
 
 template <bool EV>
-__global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, int nsys, const int* __restrict__ chunk_pre) {
-  // the rebuild decision of this step is spent (single-system dynamics runs no slot gather)
-  if (nsys == 1 && blockIdx.x == 0 && threadIdx.x == 0) S[0].flags[F_REBUILD] = 0;
+__global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, int nsys, const int* __restrict__ chunk_pre,
+                                                  int spend) {
+  if (spend && blockIdx.x == 0 && threadIdx.x == 0) S[0].flags[F_REBUILD] = 0;
   const int lane = threadIdx.x & 63, il = lane & 7, jl = lane >> 3;
   const int total = chunk_pre[nsys];
   const int nw = gridDim.x * 4;
@@ -596,23 +607,29 @@ __global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, 
   if (EV && evs >= 0) ev.flush(S[evs], lane);
 }
 
-void launch_pair(const Launch& L, int ev) {
+void launch_pair(const Launch& L, int ev, const int* cp, bool fold, bool spend) {
+  const int sp = spend ? 1 : 0;
   dim3 g(L.pair_grid), b(256);
+  if (!cp) cp = L.chunk_pre;
   if (L.mixed) {
     if (L.lj_sep) {
-      if (ev == 2) hipLaunchKernelGGL((k_pair_mixed<2, true>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
-      else if (ev == 1) hipLaunchKernelGGL((k_pair_mixed<1, true>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
-      else hipLaunchKernelGGL((k_pair_mixed<0, true>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+      if (ev == 2) hipLaunchKernelGGL((k_pair_mixed<2, true>), g, b, 0, L.stream, L.d_sys, L.nsys, cp, sp);
+      else if (ev == 1) hipLaunchKernelGGL((k_pair_mixed<1, true>), g, b, 0, L.stream, L.d_sys, L.nsys, cp, sp);
+      else hipLaunchKernelGGL((k_pair_mixed<0, true>), g, b, 0, L.stream, L.d_sys, L.nsys, cp, sp);
     } else {
-      if (ev == 2) hipLaunchKernelGGL((k_pair_mixed<2, false>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
-      else if (ev == 1) hipLaunchKernelGGL((k_pair_mixed<1, false>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
-      else hipLaunchKernelGGL((k_pair_mixed<0, false>), g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+      if (ev == 2) hipLaunchKernelGGL((k_pair_mixed<2, false>), g, b, 0, L.stream, L.d_sys, L.nsys, cp, sp);
+      else if (ev == 1) hipLaunchKernelGGL((k_pair_mixed<1, false>), g, b, 0, L.stream, L.d_sys, L.nsys, cp, sp);
+      else hipLaunchKernelGGL((k_pair_mixed<0, false>), g, b, 0, L.stream, L.d_sys, L.nsys, cp, sp);
     }
   } else {
-    if (ev) hipLaunchKernelGGL(k_pair_f64<true>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
-    else hipLaunchKernelGGL(k_pair_f64<false>, g, b, 0, L.stream, L.d_sys, L.nsys, L.chunk_pre);
+    if (ev) hipLaunchKernelGGL(k_pair_f64<true>, g, b, 0, L.stream, L.d_sys, L.nsys, cp, sp);
+    else hipLaunchKernelGGL(k_pair_f64<false>, g, b, 0, L.stream, L.d_sys, L.nsys, cp, sp);
   }
-  if (ev) hipLaunchKernelGGL(k_ev_fold, dim3(L.nsys), dim3(EV_SPREAD), 0, L.stream, L.d_sys);
+  if (ev && fold) launch_ev_fold(L);
+}
+
+void launch_ev_fold(const Launch& L) {
+  hipLaunchKernelGGL(k_ev_fold, dim3(L.nsys), dim3(EV_SPREAD), 0, L.stream, L.d_sys);
 }
 
 }  // namespace fz
