@@ -221,6 +221,14 @@ int fz_phase_times(fz_system* s, int nsteps, double* ms) {
   FZ_GUARD_END
 }
 
+int fz_pair_kernel_time(fz_system* s, int reps, double* ms) {
+  FZ_GUARD_BEGIN
+  if (!s || reps <= 0 || !ms) return set_error(FZ_ERR_ARG, "invalid argument");
+  FZ_HIP(hipSetDevice(s->ctx->device));
+  return s->eng->pair_kernel_time(reps, ms);
+  FZ_GUARD_END
+}
+
 int fz_autotune(fz_system* s, int nsteps, double* out) {
   FZ_GUARD_BEGIN
   if (!s || nsteps <= 0 || !out) return set_error(FZ_ERR_ARG, "invalid argument");

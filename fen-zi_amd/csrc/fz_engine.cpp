@@ -1521,6 +1521,24 @@ int Engine::minimize_ref(int nsteps, const fz_min_params& p) {
   return npt_reset_virial();
 }
 
+int Engine::pair_kernel_time(int reps, double* ms) {
+  FZ_TRY(forces(true));  // a current list, slot copies and work items
+  hipEvent_t e0, e1;
+  FZ_HIP(hipEventCreate(&e0));
+  FZ_HIP(hipEventCreate(&e1));
+  (void)hipEventRecord(e0, L_.stream);
+  for (int k = 0; k < reps; k++) launch_pair(L_, 0);
+  (void)hipEventRecord(e1, L_.stream);
+  const hipError_t rc = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (rc == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != hipSuccess) return set_error(FZ_ERR_HIP, "event synchronize failed: %s", hipGetErrorString(rc));
+  *ms = (double)t / reps;
+  return forces(true);  // the repeated launches summed their forces several times over
+}
+
 int Engine::phase_times(int nsteps, double* out) {
   FZ_TRY(ensure());
   for (auto* s : sys_)

@@ -131,6 +131,8 @@ class Engine {
   // nsteps eager (non-graph) steps with HIP events between phases; mean ms per
   // phase per step into out[FZ_NPHASE] (order: see fz_phase_times in fenzi.h)
   int phase_times(int nsteps, double* out);
+  // mean time of the force-only pair kernel over reps back-to-back launches
+  int pair_kernel_time(int reps, double* ms);
   // Times graph-replayed steps over candidate column widths, FFT lines per
   // workgroup and persistent pair-kernel grids (coordinate descent), keeps the
   // fastest, and restores the systems' state.  out: {width factor, fft lines,

@@ -85,7 +85,7 @@ EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create
            "fz_dcd_write_frame", "fz_dcd_close", "fz_set_fixed_atoms", "fz_set_com_restraint",
            "fz_deck_com_restraint", "fz_deck_fixed_atoms", "fz_set_neighbor_capacity",
            "fz_pme_convolve_noalloc", "fz_pme_convolve_noalloc_bytes", "fz_set_barostat", "fz_get_barostat",
-           "fz_set_launch", "fz_batch_set_launch"]
+           "fz_set_launch", "fz_batch_set_launch", "fz_pair_kernel_time"]
 
 MIN_ADAPTIVE_SD, MIN_REF_CG, MIN_REF_SD = 0, 1, 2
 PME_AUTO, PME_GRID, PME_PLANES = 0, 1, 2
@@ -157,6 +157,7 @@ def lib():
         L.fz_autotune.argtypes = [vp, C.c_int, _dp]
         L.fz_batch_autotune.argtypes = [vp, C.c_int, _dp]
         L.fz_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)]
+        L.fz_pair_kernel_time.argtypes = [vp, C.c_int, _dp]
         L.fz_batch_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)]
         L.fz_get_cluster_layout.argtypes = [vp, _ip, _ip, _ip, C.c_int64, C.POINTER(C.c_int64)]
         L.fz_get_cluster_pairs.argtypes = [vp, _ip, _up, C.c_int64]
@@ -395,6 +396,12 @@ class System:
         ms = np.zeros(len(self.PHASES))
         _check(lib().fz_phase_times(self.h, int(nsteps), _p(ms, C.c_double)), "fz_phase_times")
         return dict(zip(self.PHASES, ms.tolist()))
+
+    def pair_kernel_time(self, reps=20):
+        """fz_pair_kernel_time: mean ms per force-only pair-kernel launch."""
+        ms = C.c_double()
+        _check(lib().fz_pair_kernel_time(self.h, int(reps), C.byref(ms)), "fz_pair_kernel_time")
+        return ms.value
 
     def autotune(self, nsteps=20):
         out = np.zeros(4)

@@ -213,6 +213,12 @@ int fz_set_neighbor_capacity(fz_system *sys, int64_t pairs_cap, int32_t clusters
  * (src/globals.h:796-823). */
 #define FZ_NPHASE 9
 int fz_phase_times(fz_system *sys, int nsteps, double *ms);
+/* Mean time (ms) of the force-only pair kernel over reps launches back to back
+ * on the system's stream between two HIP events (the per-phase events above
+ * add each launch's gap to a single kernel); the forces are evaluated again
+ * afterwards.  For the pair kernel's roofline; the reference has only the
+ * PROFILING clock() buckets (src/globals.h:796-823). */
+int fz_pair_kernel_time(fz_system *sys, int reps, double *ms);
 /* Autotuner: times graph-replayed steps for candidate column widths (atoms per
  * binning column and cluster compactness), FFT lines per workgroup and
  * persistent pair-kernel grid sizes, keeps the fastest and restores the state.
