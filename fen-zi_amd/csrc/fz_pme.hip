@@ -27,8 +27,12 @@ template <class R> struct C2;
 template <> struct C2<float> { using t = float2; };
 template <> struct C2<double> { using t = double2; };
 
+// The weights are computed with no contraction into fused multiply-adds, so
+// that every spread path (atomic, bricks, planes) forms bitwise the same values
+// whatever code the compiler inlines them into.
 template <class R>
 __device__ __forceinline__ void bspline4(R f, R* th, R* dth) {
+#pragma clang fp contract(off)
   const R f2 = f * f, f3 = f2 * f, g = R(1) - f;
   th[0] = g * g * g * R(1.0 / 6.0);
   th[1] = (R(3) * f3 - R(6) * f2 + R(4)) * R(1.0 / 6.0);
@@ -56,6 +60,7 @@ __device__ __forceinline__ int spline_pos(unsigned long long u, int K, R& f) {
 // weight k (0..3) of the order-4 cardinal B-spline at fraction f
 template <class R>
 __device__ __forceinline__ R bspline4_k(R f, int k) {
+#pragma clang fp contract(off)
   const R f2 = f * f, f3 = f2 * f, g = R(1) - f;
   const R w0 = g * g * g, w1 = R(3) * f3 - R(6) * f2 + R(4), w2 = R(-3) * f3 + R(3) * f2 + R(3) * f + R(1);
   const R w = k == 0 ? w0 : k == 1 ? w1 : k == 2 ? w2 : f3;

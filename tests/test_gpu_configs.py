@@ -116,10 +116,12 @@ def test_slab230k_fp64_vs_oracle(ctx):
 
 def test_batch_over_the_ensemble_size_range_vs_oracle(ctx):
     """32 members drawn as the ensemble's (log-uniform 1,500-24,000 atoms, chain
-    fraction 0-0.3, cutoff 10): one batch minimises and steps them (over 64K
-    atoms per launch: brick spread); every member's forces at its final
+    fraction 0-0.3, cutoff 10): one batch minimises and steps them (PME plane
+    kernels with 256-thread workgroups, and the pair kernel split between the
+    members with and without a rebuild); every member's forces at its final
     positions match the oracle, and the smallest and largest members run alone
-    follow their batch trajectories bitwise."""
+    (1,024-thread plane workgroups, one pair launch) follow their batch
+    trajectories bitwise."""
     n, frac = fz_synth.batch_sizes(32)
     members = [fz_synth.ensemble_member(int(n[k]), float(frac[k]), 1000 + k) for k in range(32)]
     assert sum(int(m[0]["n_atoms"]) for m in members) >= 65536
