@@ -458,6 +458,10 @@ int System::alloc_device() {
     FZ_TRY(dalloc(&d.pme_m, KS)); FZ_TRY(up(d.pme_m, mv));
     if (mixed) { float2* t; FZ_TRY(dalloc(&t, KS)); FZ_TRY(up(t, tw32)); d.tw = t; }
     else { double2* t; FZ_TRY(dalloc(&t, KS)); FZ_TRY(up(t, tw64)); d.tw = t; }
+    // G(m) for the box at creation (the z pass forms it per point once a barostat moves the box)
+    FZ_TRY(dalloc(&d.pme_g, (size_t)d.K[2] * d.K[1] * d.Kxh, false));
+    launch_pme_gtable(d, nullptr);
+    FZ_HIP(hipDeviceSynchronize());
   }
   FZ_TRY(dalloc(&d.acc, A_NSLOT));
   FZ_TRY(dalloc(&d.acc_ev, (size_t)EV_SPREAD * EV_W));
@@ -593,7 +597,7 @@ int64_t footprint_bytes(const fz_topology* t, const fz_params* p, int precision)
   const size_t KS = (size_t)p->pme_grid[0] + p->pme_grid[1] + p->pme_grid[2];
   size_t nbr = 1;
   for (int k = 0; k < 3; k++) nbr *= (size_t)(p->pme_grid[k] + 7) / 8;
-  b += rnd256(8 * K3) + rnd256(KH * (mixed ? 8 : 16)) + rnd256(K3 * (mixed ? 4 : 8));
+  b += rnd256(8 * K3) + rnd256(KH * (mixed ? 8 : 16)) + rnd256(K3 * (mixed ? 4 : 8)) + rnd256(8 * KH);  // + G(m) table
   b += rnd256(4 * nbr) + rnd256(4 * (nbr + 1)) + rnd256(n * 16 * (mixed ? 4 : 8)) + rnd256(nbr * 1331 * 8);
   b += 3 * rnd256(8 * KS) + rnd256(KS * (mixed ? 8 : 16));
   b += rnd256(8 * A_NSLOT) + rnd256(8 * EV_SPREAD * EV_W) + 2 * rnd256(24 * n) + rnd256(4 * F_NSLOT) + rnd256(16) + 2 * rnd256(24 * n);
@@ -663,6 +667,8 @@ int System::set_fixed(const uint8_t* fixed) {
 // NPT_MIN_SCALE; the cutoff must stay below half of that box.
 int System::set_barostat(double pref_atm, double mass) {
   SysDev& d = dv;
+  // the box may move from here on: the z pass forms G(m) from the box tables
+  if (mass > 0.0 || d.npt) d.pme_g = nullptr;
   if (!(mass > 0.0)) {
     d.npt = 0;
     gen++;  // the descriptor changes (re-uploaded by the next prepare)

@@ -70,6 +70,7 @@ void launch_pair(const Launch& L, int ev, const int* cp = nullptr, bool fold = t
 void launch_ev_fold(const Launch& L);
 void launch_bonded(const Launch& L, bool ev);
 void launch_pme_fft_only(const Launch& L, int conv_mode);
+void launch_pme_gtable(const SysDev& d, hipStream_t st);  // d.pme_g from d's box tables (host descriptor)
 void launch_pme_part(const Launch& L, int part, bool ev);  // 0 spread, 1 FFT + convolution, 2 gather
 // kick, drift, SHAKE, rebuild test; finish: the previous step's kick + RATTLE first
 void launch_step_first(const Launch& L, bool finish = false);

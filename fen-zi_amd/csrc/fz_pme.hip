@@ -502,8 +502,41 @@ __global__ __launch_bounds__(256) void k_fft_y(const SysDev* __restrict__ S, int
   }
 }
 
+// G(m) of grid point (kx, y, z) and |m|^2 from the per-axis tables, with no
+// contraction into fused multiply-adds: the table below and the z pass form
+// bitwise the same value.  (kx, y, z) = 0 is not evaluated here.
+__device__ __forceinline__ double pme_G(const double* __restrict__ mt, const double* __restrict__ ex,
+                                        const double* __restrict__ bsp, double pre, int K0, int K1, int kx, int y, int z,
+                                        double& m2) {
+#pragma clang fp contract(off)
+  const double mx = mt[kx], my = mt[K0 + y], mz = mt[K0 + K1 + z];
+  m2 = mx * mx + my * my + mz * mz;
+  return pre * ex[kx] * ex[K0 + y] * ex[K0 + K1 + z] * bsp[kx] * bsp[K0 + y] * bsp[K0 + K1 + z] / m2;
+}
+
+// G(m) over the half spectrum ([z][y][kx], G(0) = 0) for a fixed box, once
+// per system: the z pass reads it instead of forming it per point
+__global__ __launch_bounds__(256) void k_pme_gtable(double* __restrict__ G, const double* __restrict__ mt,
+                                                    const double* __restrict__ ex, const double* __restrict__ bsp,
+                                                    double pre, int K0, int K1, int K2, int Kxh) {
+  const size_t n = (size_t)K2 * K1 * Kxh;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const int kx = (int)(i % Kxh), y = (int)((i / Kxh) % K1), z = (int)(i / ((size_t)Kxh * K1));
+    double m2;
+    G[i] = (kx == 0 && y == 0 && z == 0) ? 0.0 : pme_G(mt, ex, bsp, pre, K0, K1, kx, y, z, m2);
+  }
+}
+
+void launch_pme_gtable(const SysDev& d, hipStream_t st) {
+  const size_t n = (size_t)d.K[2] * d.K[1] * d.Kxh;
+  const int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL(k_pme_gtable, dim3(blocks), dim3(256), 0, st, d.pme_g, d.pme_m, d.pme_ex, d.pme_bsp, d.pme_pre,
+                     d.K[0], d.K[1], d.K[2], d.Kxh);
+}
+
 // z pass: forward FFT, multiply by G(m) (mode 0) or 1 (mode 1), energy and
-// virial, inverse FFT
+// virial, inverse FFT.  G comes from the system's table when it has one (fixed
+// box) and is formed per point otherwise (barostat: the box changes every step).
 template <bool MIXED, bool EV>
 __global__ __launch_bounds__(256) void k_fft_z_conv(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre, int NL, int mode) {
   using R = typename std::conditional<MIXED, float, double>::type;
@@ -540,12 +573,13 @@ __global__ __launch_bounds__(256) void k_fft_z_conv(const SysDev* __restrict__ S
     if (mode == 0) {
       if (kx == 0 && y == 0 && z == 0) {
         G = 0.0;
+      } else if (!EV && d.pme_g) {
+        G = d.pme_g[((size_t)z * K1 + y) * Kxh + kx];
       } else {
-        const double mx = d.pme_m[kx], my = d.pme_m[K0 + y], mz = d.pme_m[K0 + K1 + z];
-        const double m2 = mx * mx + my * my + mz * mz;
-        G = d.pme_pre * d.pme_ex[kx] * d.pme_ex[K0 + y] * d.pme_ex[K0 + K1 + z] *
-            d.pme_bsp[kx] * d.pme_bsp[K0 + y] * d.pme_bsp[K0 + K1 + z] / m2;
+        double m2;
+        G = pme_G(d.pme_m, d.pme_ex, d.pme_bsp, d.pme_pre, K0, K1, kx, y, z, m2);
         if (EV) {
+          const double mx = d.pme_m[kx], my = d.pme_m[K0 + y], mz = d.pme_m[K0 + K1 + z];
           const double wgt = (kx == 0 || 2 * kx == K0) ? 1.0 : 2.0;
           const double Em = 0.5 * wgt * G * ((double)v.x * v.x + (double)v.y * v.y);
           const double fac = 2.0 * (1.0 + d.pi2_beta2 * m2) / m2;

@@ -163,6 +163,7 @@ struct SysDev {
   double *pme_bsp; // per-axis |b(m)|^2, [K0 + K1 + K2]
   double *pme_ex;  // per-axis exp(-pi^2 m_d^2 / beta^2), [K0 + K1 + K2]
   double *pme_m;   // per-axis signed m_d (1/A), [K0 + K1 + K2]
+  double *pme_g;   // G(m) over the half spectrum [K2][K1][Kxh] for the creation box; null once a barostat runs
   void *tw;        // twiddles e^{-2 pi i k / K_d}, [K0 + K1 + K2] complex
   // Charge spreading by grid bricks of 8^3 points: atoms are counted and
   // ordered by the brick holding the first point of their 4^3 stencil, each
