@@ -416,3 +416,27 @@ def test_step_virial_with_constraint_part_vs_oracle(ctx, prec):
     assert abs(e["potential"] - sum(e_ref.values())) < (1e-9 if prec == fenzi.FP64 else 1e-5) * scale
     for k, name in (("vdw", "vdw"), ("elec", "elec"), ("pme", "pme")):
         assert abs(e[name] - e_ref[k]) < (1e-9 if prec == fenzi.FP64 else 1e-5) * scale, k
+
+
+def test_launch_shapes_and_pair_timing_leave_the_forces_unchanged(ctx):
+    """fz_set_launch reports the shapes in effect and keeps them across calls;
+    the mixed-form forces do not depend on them (fixed-point sums), the
+    reciprocal energy's per-workgroup FP64 partial sums do, at rounding level;
+    fz_pair_kernel_time returns a positive time and leaves the system's forces
+    and energies as they were (it evaluates them again after its launches)."""
+    topo, xyz, params = systems.slab(seed=12)
+    s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+    f0, e0 = s.forces()
+    eff = s.set_launch(fft_lines=4, pair_grid=512)
+    assert eff["fft_lines"] == 4 and eff["pair_grid"] == 512
+    assert s.set_launch() == eff  # -1: keep
+    f1, e1 = s.forces()
+    assert np.array_equal(f0, f1)
+    assert abs(e1["potential"] - e0["potential"]) <= 1e-10 * abs(e0["potential"])
+    ms = s.pair_kernel_time(5)
+    assert ms > 0.0
+    f2, e2 = s.forces()
+    assert np.array_equal(f1, f2) and e2["potential"] == e1["potential"]
+    with pytest.raises(fenzi.FenziError):
+        s.set_launch(fft_lines=1000)
+    s.close()
