@@ -918,7 +918,7 @@ int Engine::prepare() {
     L_.lj_sep &= s->dv.lj_sep;
     L_.com_any |= s->dv.com_on;
   }
-  std::vector<int> atoms, slots, cols, clus, srch, terms, ccl, stp, fx_, fy_, fz_, bricks, planes;
+  std::vector<int> atoms, atoms64, slots, cols, clus, srch, terms, ccl, stp, fx_, fy_, fz_, bricks, planes;
   // Waves per I cluster in the pair search: up to each system's search_parts
   // when the launch holds few clusters (latency bound: too few waves in
   // flight), one when it holds many.
@@ -935,6 +935,7 @@ int Engine::prepare() {
   for (auto* s : sys_) {
     const SysDev& d = s->dv;
     atoms.push_back((d.n_atoms + 255) / 256);
+    atoms64.push_back((d.n_atoms + 63) / 64);
     slots.push_back((d.n_slots_cap + 255) / 256);
     cols.push_back(d.ncol);
     clus.push_back((d.n_clusters_cap + 3) / 4);
@@ -951,6 +952,7 @@ int Engine::prepare() {
     planes.push_back(d.K[2]);
   }
   FZ_TRY(make_work(L_.w_atoms, atoms));
+  FZ_TRY(make_work(L_.w_atoms64, atoms64));
   FZ_TRY(make_work(L_.w_slots, slots));
   FZ_TRY(make_work(L_.w_cols, cols));
   FZ_TRY(make_work(L_.w_clus, clus));

@@ -22,6 +22,8 @@ inline int plane_lds_bytes(int K0, int K1, int csz) { return (2 * K0 * K1 + K0 +
 // system's K2 workgroups reads every atom's z position) and this much LDS
 constexpr int PLANE_MAX_ATOMS = 65536, PLANE_MAX_LDS = 65536;
 // launches of fewer planes than this use workgroups of 1,024 threads, others 256
+// PME gather: four lanes per atom in launches of up to this many 256-atom blocks
+constexpr int GATHER_QUAD_BLOCKS = 512;
 constexpr int PLANE_WIDE_BELOW = 512;
 
 struct Launch {
@@ -30,6 +32,7 @@ struct Launch {
   int nsys = 0;
   int mixed = 0;                  // 0 = FP64 form, 1 = mixed form
   Work w_atoms;   // 256 atoms per block
+  Work w_atoms64; // 64 atoms per block
   Work w_slots;   // 256 slots per block
   Work w_cols;    // one block per column
   Work w_clus;    // 4 clusters (waves) per block

@@ -813,27 +813,37 @@ __global__ __launch_bounds__(NT) void k_pme_plane_inv(const SysDev* __restrict__
   for (int e = threadIdx.x; e < KP; e += NT) phi[e] = r2[e].x;
 }
 
-// Force gather, one thread per atom, into the atom-ordered gather term (a
-// store: each atom has one): it needs no slot layout, so it runs beside the
-// neighbour build.
-template <bool MIXED>
+// Force gather into the atom-ordered gather term (a store: each atom has one):
+// it needs no slot layout, so it runs beside the neighbour build.  The sum over
+// the 4x4x4 stencil is formed per stencil plane c (16 potential values each),
+// and the planes' partial sums are added as (p0 + p1) + (p2 + p3), with no
+// contraction into fused multiply-adds.  LPA lanes per atom: with 4 (launches
+// over few atoms, latency bound) lane c forms plane c's sum and the quad adds
+// them by lane exchange; with 1 (many atoms) one lane forms all four.  Both
+// give the same bits, so a system's trajectory does not depend on the batch
+// around it.
+template <bool MIXED, int LPA>
 __global__ __launch_bounds__(256) void k_pme_gather(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre) {
+#pragma clang fp contract(off)
   using R = typename std::conditional<MIXED, float, double>::type;
   const int s_ = block_sys(pre, nsys, blockIdx.x);
   const SysDev& d = S[s_];
-  const int a = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
-  if (a >= d.n_atoms) return;
+  const int a = (blockIdx.x - block_base(pre, nsys, s_)) * (256 / LPA) + (int)(threadIdx.x / LPA);
+  const int c0 = (int)(threadIdx.x % LPA);
+  if (a >= d.n_atoms) return;  // the atom's lanes together
   const double qd = d.q[a];
   const int n = d.n_atoms;
   if (qd == 0.0) {
-    if (MIXED) {
-      unsigned long long* F = (unsigned long long*)d.Fg;
-      F[a] = 0ull; F[n + a] = 0ull; F[2 * n + a] = 0ull;
-    } else {
-      double* F = (double*)d.Fg;
-      F[a] = 0.0; F[n + a] = 0.0; F[2 * n + a] = 0.0;
+    if (c0 == 0) {
+      if (MIXED) {
+        unsigned long long* F = (unsigned long long*)d.Fg;
+        F[a] = 0ull; F[n + a] = 0ull; F[2 * n + a] = 0ull;
+      } else {
+        double* F = (double*)d.Fg;
+        F[a] = 0.0; F[n + a] = 0.0; F[2 * n + a] = 0.0;
+      }
     }
-    return;
+    return;  // the atom's lanes together
   }
   const int K0 = d.K[0], K1 = d.K[1], K2 = d.K[2];
   R fx, fy, fz, tx[4], ty[4], tz[4], dx[4], dy[4], dz[4];
@@ -847,10 +857,15 @@ __global__ __launch_bounds__(256) void k_pme_gather(const SysDev* __restrict__ S
   int ix[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) ix[k] = gx + k >= K0 ? gx + k - K0 : gx + k;
-  R g0 = 0, g1 = 0, g2 = 0;
+  // per plane partial sums (this lane's planes)
+  R p0[4 / LPA], p1[4 / LPA], p2[4 / LPA];
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
+  for (int j = 0; j < 4 / LPA; j++) {
+    const int c = j * LPA + c0;
+    const R tzc = LPA == 1 ? tz[j] : (c == 0 ? tz[0] : c == 1 ? tz[1] : c == 2 ? tz[2] : tz[3]);
+    const R dzc = LPA == 1 ? dz[j] : (c == 0 ? dz[0] : c == 1 ? dz[1] : c == 2 ? dz[2] : dz[3]);
     const int z = gz + c >= K2 ? gz + c - K2 : gz + c;
+    R g0 = 0, g1 = 0, g2 = 0;
 #pragma unroll
     for (int b = 0; b < 4; b++) {
       const int y = gy + b >= K1 ? gy + b - K1 : gy + b;
@@ -862,10 +877,22 @@ __global__ __launch_bounds__(256) void k_pme_gather(const SysDev* __restrict__ S
         sx += tx[k] * p;
         sdx += dx[k] * p;
       }
-      g0 += sdx * ty[b] * tz[c];
-      g1 += sx * dy[b] * tz[c];
-      g2 += sx * ty[b] * dz[c];
+      g0 += sdx * ty[b] * tzc;
+      g1 += sx * dy[b] * tzc;
+      g2 += sx * ty[b] * dzc;
     }
+    p0[j] = g0; p1[j] = g1; p2[j] = g2;
+  }
+  R g0, g1, g2;
+  if (LPA == 4) {
+    g0 = p0[0]; g1 = p1[0]; g2 = p2[0];
+    g0 += __shfl_xor(g0, 1, 64); g1 += __shfl_xor(g1, 1, 64); g2 += __shfl_xor(g2, 1, 64);
+    g0 += __shfl_xor(g0, 2, 64); g1 += __shfl_xor(g1, 2, 64); g2 += __shfl_xor(g2, 2, 64);
+    if (c0 != 0) return;
+  } else {
+    g0 = (p0[0] + p0[1 % (4 / LPA)]) + (p0[2 % (4 / LPA)] + p0[3 % (4 / LPA)]);
+    g1 = (p1[0] + p1[1 % (4 / LPA)]) + (p1[2 % (4 / LPA)] + p1[3 % (4 / LPA)]);
+    g2 = (p2[0] + p2[1 % (4 / LPA)]) + (p2[2 % (4 / LPA)] + p2[3 % (4 / LPA)]);
   }
   const double fxv = -qd * (double)g0 * K0 / d.L[0];
   const double fyv = -qd * (double)g1 * K1 / d.L[1];
@@ -881,6 +908,15 @@ __global__ __launch_bounds__(256) void k_pme_gather(const SysDev* __restrict__ S
     F[n + a] = fyv;
     F[2 * n + a] = fzv;
   }
+}
+
+template <bool MIXED>
+static void launch_gather(const Launch& L) {
+  // four lanes per atom up to GATHER_QUAD_BLOCKS blocks of 256 atoms per launch
+  if (L.w_atoms.total <= GATHER_QUAD_BLOCKS)
+    hipLaunchKernelGGL((k_pme_gather<MIXED, 4>), dim3(L.w_atoms64.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms64.pre);
+  else
+    hipLaunchKernelGGL((k_pme_gather<MIXED, 1>), dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
 }
 
 template <bool MIXED>
@@ -924,8 +960,8 @@ void launch_pme_part(const Launch& L, int part, bool ev) {
     if (L.mixed) launch_fft_passes<true>(L, ev, 0);
     else launch_fft_passes<false>(L, ev, 0);
   } else {
-    if (L.mixed) hipLaunchKernelGGL(k_pme_gather<true>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
-    else hipLaunchKernelGGL(k_pme_gather<false>, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre);
+    if (L.mixed) launch_gather<true>(L);
+    else launch_gather<false>(L);
   }
 }
 
