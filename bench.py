@@ -114,6 +114,7 @@ def md(ctx, topo, x, p, prec, steps, warmup, seed=1):
     pk = s.pair_kernel_time(20)
     out = dict(ms_per_step=ms, ns_per_day=ns_day(ms, p["dt_ps"]), temperature=e["temperature"],
                pairs_within_cutoff=int(e["pairs_within_cutoff"]), list_builds=int(e["list_builds"]), phases_ms=ph,
+               pair_tiles=int(e["pair_tiles"]),
                pair_kernel_ms=pk,
                device_bytes=s.info()["device_bytes"], autotune={k: float(v) for k, v in tuned.items()})
     return out, s
@@ -196,6 +197,8 @@ def roofline(run, prec, workload):
             "traffic": None if tr is None else tr["bytes_per_launch"],
             "traffic_source": None if tr is None else tr["source"],
             "flop_per_launch": flops, "pairs_per_launch": run["pairs_within_cutoff"],
+            "tiles_per_launch": run["pair_tiles"],
+            "in_cutoff_lane_fraction": run["pairs_within_cutoff"] / (64.0 * max(1, run["pair_tiles"])),
             "launch_ms": run["pair_kernel_ms"],
             "timing": "mean of 20 force-only launches back to back, two HIP events on the launch stream "
                       "(fz_pair_kernel_time)"}

@@ -513,7 +513,7 @@ size_t nb_bytes(int ncol, int nzb, int mixed, long long pairs_cap, int clusters_
        rnd256(4 * ns) + rnd256(4 * cc) + rnd256(48 * cc) + rnd256(24 * ns);
   b += rnd256(4 * (cc * parts + 1)) + rnd256(4 * cc * parts * SEARCH_TMP_CAP) + rnd256(8 * cc * parts * SEARCH_TMP_CAP);
   b += rnd256(4 * ((cc + 63) / 64));
-  b += 2 * rnd256(4 * (size_t)pairs_cap) + rnd256(8 * (size_t)pairs_cap);
+  b += 3 * rnd256(4 * (size_t)pairs_cap) + rnd256(8 * (size_t)pairs_cap) + rnd256(4 * cc);
   return b;
 }
 
@@ -570,6 +570,8 @@ int System::alloc_neighbor_block(long long pairs_cap, int clusters_cap) {
   FZ_TRY(dalloc(&d.pl_i, pairs_cap, false));
   FZ_TRY(dalloc(&d.pl_j, pairs_cap, false));
   FZ_TRY(dalloc(&d.pl_mask, pairs_cap, false));
+  FZ_TRY(dalloc(&d.pl_dmin, pairs_cap, false));
+  FZ_TRY(dalloc(&d.clu_disp, clusters_cap));
   return FZ_OK;
 }
 
@@ -824,6 +826,7 @@ int System::energies(fz_energies* e) {
   e->temperature = dof > 0 ? 2.0 * e->kinetic / (kRC * dof) : 0.0;
   for (int k = 0; k < 9; k++) e->virial[k] = fx(a[A_VIR + k]);
   e->pairs_within_cutoff = (int64_t)a[A_NPAIR];
+  e->pair_tiles = (int64_t)a[A_NTILE];
   e->list_builds = flags_host[F_NBUILD];
   e->volume = dv.L[0] * dv.L[1] * dv.L[2];
   // (2 KE + tr W) / 3V in atm (the reference's pressure, src/cell_based_kernels.cu:586-589)
@@ -1035,11 +1038,13 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   launch_pme_part(side[1], 1, ev);
   launch_pme_part(side[1], 2, ev);
   if (split) {
+    launch_clu_disp(side[2], true);
     launch_chunk_split(side[2], false);
     (void)hipEventRecord(ev_split_, side_[2]);
     launch_pair(side[2], pev, L_.chunk_pre_a, false);
   }
   launch_nblist(L_, check);
+  if (!split) launch_clu_disp(L_, false);
   if (check) {
     // stand-alone evaluation: slot copies and every system's work items
     launch_gather_sorted(L_);
@@ -1562,6 +1567,7 @@ int Engine::phase_times(int nsteps, double* out) {
     launch_step_first(L_);
     (void)hipEventRecord(ev[1], L_.stream);
     launch_nblist(L_, false);
+    launch_clu_disp(L_, false);
     (void)hipEventRecord(ev[2], L_.stream);
     launch_gather_sorted(L_);
     (void)hipEventRecord(ev[3], L_.stream);
@@ -1643,6 +1649,12 @@ int Engine::set_launch(int nl, int grid, int pme_path, int32_t* out) {
     out[1] = L_.pair_grid;
     out[2] = L_.pme_planes ? FZ_PME_PLANES : FZ_PME_GRID;
   }
+  return FZ_OK;
+}
+
+int Engine::set_pair_skip(int on) {
+  L_.pair_skip = on ? 1 : 0;
+  drop_graphs();  // the captured launches hold the setting
   return FZ_OK;
 }
 

@@ -18,15 +18,15 @@ constexpr int MAX_FFT = 256;        // largest PME grid dimension
 enum AccSlot {
   A_BOND = 0, A_ANGLE, A_UB, A_IMPR, A_DIHED, A_VDW, A_ELEC, A_PME, A_EWEXCL,
   A_KIN, A_VIR /* 9 slots */, A_NPAIR = A_VIR + 9, A_MOM /* 3 slots */, A_FMAX = A_MOM + 3,
-  A_COM /* 6 slots: sum m x of group 0, then group 1 */, A_NSLOT = A_COM + 6
+  A_COM /* 6 slots: sum m x of group 0, then group 1 */, A_NTILE = A_COM + 6, A_NSLOT
 };
 
 // The pair kernels' energy / virial sums of one evaluation go first to
 // EV_SPREAD copies of EV_W words (one 128-byte line each; a wave adds into copy
 // wave & (EV_SPREAD - 1)), and a fold kernel adds the copies into acc: several
 // thousand waves adding into the few lines of acc serialise there.  Words: vdw,
-// elec, virial xx yy zz xy xz yz, pair count.
-constexpr int EV_SPREAD = 64, EV_W = 16, EV_NV = 9;
+// elec, virial xx yy zz xy xz yz, pair count, tile count.
+constexpr int EV_SPREAD = 64, EV_W = 16, EV_NV = 10;
 
 // Device-side status flags.
 enum FlagSlot { F_REBUILD = 0, F_OVERFLOW, F_SHAKE_FAIL, F_NBUILD, F_NSLOT };
@@ -134,6 +134,16 @@ struct SysDev {
   int *tile_cnt;         // [ceil(n_clusters_cap / 64)] kept entries per 64 consecutive I clusters
   int *pl_i, *pl_j;      // [n_pairs_cap]
   unsigned long long *pl_mask; // [n_pairs_cap]
+  // Skip test of the mixed pair kernel.  pl_dmin[e]: a lower bound, fixed at the
+  // entry's first evaluation, of its pairs' distance at the list build (the
+  // smallest pair distance then, less both clusters' displacement bounds then);
+  // < 0 until that evaluation (the list build writes -1).  clu_disp[c]: the
+  // largest displacement of cluster c's atoms from their build positions, formed
+  // before every pair launch.  A pair's distance now is at least
+  // pl_dmin[e] - clu_disp[I] - clu_disp[J], so an entry where that exceeds the
+  // cutoff holds no pair inside it and is not evaluated.
+  float *pl_dmin;        // [n_pairs_cap]
+  float *clu_disp;       // [n_clusters_cap]
   long long *n_pairs;    // device scalar
   // ---- exclusions ---------------------------------------------------------
   unsigned long long *excl_bits; // [n_atoms]: bit (32 + j - i) set if j excluded, |j-i| <= 31
