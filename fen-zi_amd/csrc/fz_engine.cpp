@@ -979,6 +979,7 @@ int Engine::prepare() {
   // Spread by bricks from 64K atoms per launch; below, the single atomic kernel
   // is cheaper than the brick pipeline's five launches.  Same grid either way.
   L_.spread_bricks = atoms_total >= 65536 ? 1 : 0;
+  L_.pair_split = (ns > 1 || atoms_total >= PAIR_SPLIT_ATOMS) ? 1 : 0;
   // Systems of up to PLANE_MAX_ATOMS atoms: spread and x, y transforms by the
   // plane kernels (two launches and no grid atomics instead of four launches).
   L_.plane_lds = 0;
@@ -1022,13 +1023,15 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   // layout: both start with the build.
   Launch side[NSIDE] = {L_, L_, L_};
   for (int k = 0; k < NSIDE; k++) side[k].stream = side_[k];
-  // Batch dynamics: the pair kernel runs twice.  On the third side stream, at
-  // once, over the members whose lists stay valid this step (no rebuild
-  // pending); on the main stream, after the build, over the members it
-  // rebuilt.  The build of a few members then overlaps the pair work of the
-  // others.  (For one system the extra launches and cross-stream wait cost
-  // more than the build's launches they hide.)
-  const bool split = !check && L_.nsys > 1;
+  // Batch dynamics (and one system of PAIR_SPLIT_ATOMS or more): the pair
+  // kernel runs twice.  On the third side stream, at once, over the members
+  // whose lists stay valid this step (no rebuild pending); on the main stream,
+  // after the build, over the members it rebuilt.  The build of a few members
+  // then overlaps the pair work of the others, and a step without a rebuild
+  // starts the pair kernel without waiting for the list step's early-exit
+  // launches.  (For a small system the extra launches and cross-stream wait
+  // cost more than the list launches they hide.)
+  const bool split = !check && L_.pair_split;
   const int nside = split ? 3 : 2;
   const int pev = ev ? (lite ? 1 : 2) : 0;
   (void)hipEventRecord(ev_early_, L_.stream);

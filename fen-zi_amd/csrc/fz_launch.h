@@ -58,7 +58,14 @@ struct Launch {
   int search_cap = 1;  // cap on the pair-search waves per I cluster (1, 2 or 4)
   int npt = 0;         // every system has the Andersen barostat on
   int pair_skip = 1;   // the mixed pair kernel leaves out entries that cannot hold a pair inside the cutoff
+  int pair_split = 0;  // dynamics: pair kernel over the lists kept this step beside the build (see fz_engine)
 };
+
+// Launches from this many atoms run the pair kernel over the systems whose
+// lists stay valid beside the list step, also for one system: there the list
+// step's early-exit launches (queued behind the PME and bonded kernels) hold
+// the pair kernel back by more than the extra launches of the split cost.
+constexpr long long PAIR_SPLIT_ATOMS = 65536;
 
 // check: run the displacement test first (the fused step kernel does it otherwise)
 void launch_nblist(const Launch& L, bool check);

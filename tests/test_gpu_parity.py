@@ -526,3 +526,30 @@ def test_pair_skip_in_a_batch_matches_alone_without_it(ctx):
         _same_energies(es, e)
         assert es["pair_tiles"] <= e["pair_tiles"]
     b.close()
+
+
+def test_single_large_system_split_matches_a_batch_member(ctx):
+    """From 65,536 atoms one system's dynamics also runs the pair kernel over
+    its kept list beside the list step, as a batch does (the build's own
+    work-item layout is then replaced by the after-build prefix).  The
+    trajectory through several list builds equals, bitwise, the same system's
+    as a member of a batch."""
+    topo, xyz, params = systems.water_box(n_waters=22000, L=88.0, seed=31)
+    params = dict(params, K=(90, 90, 90), pairlist=params["cutoff"] + 0.8)
+    a = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+    assert a.info()["n_atoms"] >= 65536
+    a.reassign_velocities(300.0, 4, 0)
+    a.step(30)
+    ua, ea = a.get_fixed(), a.energies()
+    a.close()
+    assert ea["list_builds"] >= 3
+    b0 = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
+    t2, x2, p2 = systems.water_box(seed=2)
+    b1 = fenzi.System(ctx, t2, p2, fenzi.MIXED, xyz=x2)
+    for s in (b0, b1):
+        s.reassign_velocities(300.0, 4, 0)
+    b = fenzi.Batch(ctx, [b0, b1])
+    b.step(30)
+    assert np.array_equal(b0.get_fixed(), ua)
+    _same_energies(b0.energies(), ea)
+    b.close()
