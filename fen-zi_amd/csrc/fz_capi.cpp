@@ -245,16 +245,6 @@ int fz_set_launch(fz_system* s, int nl, int grid, int pme_path, int32_t* out) {
   FZ_GUARD_END
 }
 
-int fz_set_pair_skip(fz_system* s, int on) {
-  if (!s) return set_error(FZ_ERR_ARG, "invalid argument");
-  return s->eng->set_pair_skip(on);
-}
-
-int fz_batch_set_pair_skip(fz_batch* b, int on) {
-  if (!b) return set_error(FZ_ERR_ARG, "invalid argument");
-  return b->eng->set_pair_skip(on);
-}
-
 int fz_batch_set_launch(fz_batch* b, int nl, int grid, int pme_path, int32_t* out) {
   FZ_GUARD_BEGIN
   if (!b) return set_error(FZ_ERR_ARG, "invalid argument");

@@ -513,7 +513,7 @@ size_t nb_bytes(int ncol, int nzb, int mixed, long long pairs_cap, int clusters_
        rnd256(4 * ns) + rnd256(4 * cc) + rnd256(48 * cc) + rnd256(24 * ns);
   b += rnd256(4 * (cc * parts + 1)) + rnd256(4 * cc * parts * SEARCH_TMP_CAP) + rnd256(8 * cc * parts * SEARCH_TMP_CAP);
   b += rnd256(4 * ((cc + 63) / 64));
-  b += 3 * rnd256(4 * (size_t)pairs_cap) + rnd256(8 * (size_t)pairs_cap) + rnd256(4 * cc);
+  b += 2 * rnd256(4 * (size_t)pairs_cap) + rnd256(8 * (size_t)pairs_cap);
   return b;
 }
 
@@ -570,8 +570,6 @@ int System::alloc_neighbor_block(long long pairs_cap, int clusters_cap) {
   FZ_TRY(dalloc(&d.pl_i, pairs_cap, false));
   FZ_TRY(dalloc(&d.pl_j, pairs_cap, false));
   FZ_TRY(dalloc(&d.pl_mask, pairs_cap, false));
-  FZ_TRY(dalloc(&d.pl_dmin, pairs_cap, false));
-  FZ_TRY(dalloc(&d.clu_disp, clusters_cap));
   return FZ_OK;
 }
 
@@ -979,7 +977,6 @@ int Engine::prepare() {
   // Spread by bricks from 64K atoms per launch; below, the single atomic kernel
   // is cheaper than the brick pipeline's five launches.  Same grid either way.
   L_.spread_bricks = atoms_total >= 65536 ? 1 : 0;
-  L_.pair_split = (ns > 1 || atoms_total >= PAIR_SPLIT_ATOMS) ? 1 : 0;
   // Systems of up to PLANE_MAX_ATOMS atoms: spread and x, y transforms by the
   // plane kernels (two launches and no grid atomics instead of four launches).
   L_.plane_lds = 0;
@@ -1023,15 +1020,13 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   // layout: both start with the build.
   Launch side[NSIDE] = {L_, L_, L_};
   for (int k = 0; k < NSIDE; k++) side[k].stream = side_[k];
-  // Batch dynamics (and one system of PAIR_SPLIT_ATOMS or more): the pair
-  // kernel runs twice.  On the third side stream, at once, over the members
-  // whose lists stay valid this step (no rebuild pending); on the main stream,
-  // after the build, over the members it rebuilt.  The build of a few members
-  // then overlaps the pair work of the others, and a step without a rebuild
-  // starts the pair kernel without waiting for the list step's early-exit
-  // launches.  (For a small system the extra launches and cross-stream wait
-  // cost more than the list launches they hide.)
-  const bool split = !check && L_.pair_split;
+  // Batch dynamics: the pair kernel runs twice.  On the third side stream, at
+  // once, over the members whose lists stay valid this step (no rebuild
+  // pending); on the main stream, after the build, over the members it
+  // rebuilt.  The build of a few members then overlaps the pair work of the
+  // others.  (For one system the extra launches and cross-stream wait cost
+  // more than the build's launches they hide, at 17K and at 230K atoms.)
+  const bool split = !check && L_.nsys > 1;
   const int nside = split ? 3 : 2;
   const int pev = ev ? (lite ? 1 : 2) : 0;
   (void)hipEventRecord(ev_early_, L_.stream);
@@ -1041,13 +1036,11 @@ void Engine::record_forces(bool ev, bool lite, bool check) {
   launch_pme_part(side[1], 1, ev);
   launch_pme_part(side[1], 2, ev);
   if (split) {
-    launch_clu_disp(side[2], true);
     launch_chunk_split(side[2], false);
     (void)hipEventRecord(ev_split_, side_[2]);
     launch_pair(side[2], pev, L_.chunk_pre_a, false);
   }
   launch_nblist(L_, check);
-  if (!split) launch_clu_disp(L_, false);
   if (check) {
     // stand-alone evaluation: slot copies and every system's work items
     launch_gather_sorted(L_);
@@ -1570,7 +1563,6 @@ int Engine::phase_times(int nsteps, double* out) {
     launch_step_first(L_);
     (void)hipEventRecord(ev[1], L_.stream);
     launch_nblist(L_, false);
-    launch_clu_disp(L_, false);
     (void)hipEventRecord(ev[2], L_.stream);
     launch_gather_sorted(L_);
     (void)hipEventRecord(ev[3], L_.stream);
@@ -1652,12 +1644,6 @@ int Engine::set_launch(int nl, int grid, int pme_path, int32_t* out) {
     out[1] = L_.pair_grid;
     out[2] = L_.pme_planes ? FZ_PME_PLANES : FZ_PME_GRID;
   }
-  return FZ_OK;
-}
-
-int Engine::set_pair_skip(int on) {
-  L_.pair_skip = on ? 1 : 0;
-  drop_graphs();  // the captured launches hold the setting
   return FZ_OK;
 }
 

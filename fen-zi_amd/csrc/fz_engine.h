@@ -141,8 +141,6 @@ class Engine {
   // FFT lines, pair grid (0 default, < 0 keep), PME path (FZ_PME_*, < 0 keep);
   // out (nullable): the three in effect
   int set_launch(int nl, int grid, int pme_path, int32_t* out);
-  // the mixed pair kernel's skip test on (default) or off
-  int set_pair_skip(int on);
   int read_status();                   // flags + accumulators (+ barostat state) of every system -> host
   // barostat members: the piston's virial trace from the last stand-alone
   // evaluation (after a change of positions outside dynamics)

@@ -57,21 +57,10 @@ struct Launch {
   int plane_lds = 0;         // their dynamic LDS bytes
   int search_cap = 1;  // cap on the pair-search waves per I cluster (1, 2 or 4)
   int npt = 0;         // every system has the Andersen barostat on
-  int pair_skip = 1;   // the mixed pair kernel leaves out entries that cannot hold a pair inside the cutoff
-  int pair_split = 0;  // dynamics: pair kernel over the lists kept this step beside the build (see fz_engine)
 };
-
-// Launches from this many atoms run the pair kernel over the systems whose
-// lists stay valid beside the list step, also for one system: there the list
-// step's early-exit launches (queued behind the PME and bonded kernels) hold
-// the pair kernel back by more than the extra launches of the split cost.
-constexpr long long PAIR_SPLIT_ATOMS = 65536;
 
 // check: run the displacement test first (the fused step kernel does it otherwise)
 void launch_nblist(const Launch& L, bool check);
-// per-cluster displacement bounds of the pair kernel's skip test (mixed form);
-// before every pair launch, after the last change of positions and list build
-void launch_clu_disp(const Launch& L, bool kept_only);
 void launch_gather_sorted(const Launch& L);
 void launch_chunk_prefix(const Launch& L);  // pair-kernel work items of every system, rebuild flags cleared
 // pair-kernel work items of the systems without (built = false, into

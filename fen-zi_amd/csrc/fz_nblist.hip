@@ -647,8 +647,6 @@ __global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S,
     d.flags[F_NBUILD] += 1;
     if (nsys == 1) { chunk_pre[0] = 0; chunk_pre[1] = (int)((total + PAIR_CHUNK - 1) / PAIR_CHUNK); }
   }
-  // the new clusters sit at their build positions; their entries are not yet evaluated
-  if (lane == 0) d.clu_disp[I] = 0.f;
   long long off = before;
   for (int p = 0; p < split; p++) {
     const int cnt = d.search_cnt[I * split + p];
@@ -659,51 +657,9 @@ __global__ __launch_bounds__(256) void k_pair_fill(const SysDev* __restrict__ S,
       d.pl_i[off + k] = I;
       d.pl_j[off + k] = src[k];
       d.pl_mask[off + k] = msrc[k];
-      d.pl_dmin[off + k] = -1.f;
     }
     off += cnt;
   }
-}
-
-// Per list cluster (mixed form), the largest distance of its atoms from their
-// positions at the list build, for the pair kernel's skip test: 8 lanes per
-// cluster, FP64 displacements of the current positions from the slot-ordered
-// build positions, rounded up to FP32.  Under a barostat the box scale moves the
-// pair distances as well, so these systems get a bound that disables the test.
-// kept_only: only members whose list stays valid this step (no rebuild
-// pending; a rebuild writes its clusters' zeros itself).  off: the bound that
-// disables the test for every system (and for the entries first evaluated now,
-// until their next list build).
-__global__ __launch_bounds__(256) void k_clu_disp(const SysDev* __restrict__ S, int nsys, const int* __restrict__ pre,
-                                                  int kept_only, int off) {
-  SYS_FROM(pre);
-  if (!d.mixed || d.flags[F_OVERFLOW] != 0) return;
-  if (kept_only && d.flags[F_REBUILD] != 0) return;
-  const int nb = pre[s_ + 1] - pre[s_];
-  const int nsl = *d.n_clusters * CLUSTER, nsc = d.n_slots_cap;
-  // whole clusters per step of the loop (256 and the slot count are multiples of 8)
-  for (int t = lb * 256 + threadIdx.x; t < nsl; t += nb * 256) {
-    const int a = d.slot_atom[t];
-    float D = 0.f;
-    if (off || d.npt) {
-      D = 1.0e30f;
-    } else if (a >= 0) {
-      const double dx = fx_d(d.su[t], d.ux[a], d.sc[0]);
-      const double dy = fx_d(d.su[nsc + t], d.uy[a], d.sc[1]);
-      const double dz = fx_d(d.su[2 * nsc + t], d.uz[a], d.sc[2]);
-      D = __double2float_ru(sqrt(dx * dx + dy * dy + dz * dz));
-    }
-    D = fmaxf(D, __shfl_xor(D, 1));
-    D = fmaxf(D, __shfl_xor(D, 2));
-    D = fmaxf(D, __shfl_xor(D, 4));
-    if ((t & 7) == 0) d.clu_disp[t >> 3] = D;
-  }
-}
-
-void launch_clu_disp(const Launch& L, bool kept_only) {
-  if (!L.mixed) return;
-  hipLaunchKernelGGL(k_clu_disp, dim3(L.w_atoms.total), dim3(256), 0, L.stream, L.d_sys, L.nsys, L.w_atoms.pre,
-                     kept_only ? 1 : 0, L.pair_skip ? 0 : 1);
 }
 
 // ---------------------------------------------------------------------------

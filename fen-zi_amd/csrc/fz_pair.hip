@@ -140,7 +140,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef FZ_GLOBAL v4i GI4;
 typedef FZ_GLOBAL v2f GF2;
-typedef FZ_GLOBAL float GF;
 typedef FZ_GLOBAL unsigned long long GU64;
 
 __device__ __forceinline__ void atomic_add_g(GU64* p, unsigned long long v) {
@@ -334,6 +333,7 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
     const long long np = *d.n_pairs;
     const long long e0 = (long long)(w - chunk_pre[s]) * PAIR_CHUNK;
     const int ne = rfl((int)(np - e0 < PAIR_CHUNK ? np - e0 : PAIR_CHUNK));  // uniform: scalar loop control
+    if (EV) ev.ntl += ne;
     // slot data of cluster C at (uniform base + C * stride) + per-lane offset:
     // global loads with an SGPR base and a 32-bit VGPR offset
     const unsigned cap = (unsigned)d.n_slots_cap;
@@ -357,40 +357,21 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
     const float pb = (float)(FZ_ERFC_P * d.beta);
     const float ccb = (float)(TWO_OVER_SQRT_PI * d.beta);
     const float nb2l2e = -(float)(d.beta * d.beta * 1.4426950408889634);
-    const float rcm = (float)d.rc + 1.0e-3f;
     // the chunk's entries, one per lane, read once and broadcast per entry
     int vI = 0, vJ = 0;
     unsigned vMlo = 0u, vMhi = 0u;
-    float vD = 0.f;  // both clusters' displacement bounds
-    bool live = false, rec = false;
     if (lane < ne) {
       vI = ((const GI*)d.pl_i)[e0 + lane];
       vJ = ((const GI*)d.pl_j)[e0 + lane];
       const unsigned long long m = ((const GU64*)d.pl_mask)[e0 + lane];
       vMlo = (unsigned)m;
       vMhi = (unsigned)(m >> 32);
-      const float dmin = ((const GF*)d.pl_dmin)[e0 + lane];
-      vD = ((const GF*)d.clu_disp)[vI] + ((const GF*)d.clu_disp)[vJ];
-      // no pair of the entry can be inside the cutoff unless this holds (the
-      // margin covers the rounding of the FP32 distances, ~1e-5 A)
-      live = dmin < rcm + vD;
-      rec = dmin < 0.f;
     }
-    // entries to evaluate (ne <= 32: the low word of the ballot), and those
-    // evaluated for the first time since the build, whose bound is recorded
-    unsigned lm = (unsigned)__builtin_amdgcn_ballot_w64(live);
-    const unsigned rm = (unsigned)__builtin_amdgcn_ballot_w64(rec);
-    if (EV) ev.ntl += __builtin_popcount(lm);
     // j cluster data run one entry ahead of the arithmetic
     const unsigned xoff = (unsigned)jl * 16u, poff = (unsigned)jl * 8u;
-    int Jn = 0;
-    v4i xjn = {0, 0, 0, 0};
-    v2f pjn = {0.f, 0.f};
-    if (lm) {
-      Jn = __builtin_amdgcn_readlane(vJ, __builtin_ctz(lm));
-      xjn = buf_ld4(xsr, xoff, (unsigned)Jn * (CLUSTER * 16));
-      pjn = buf_ld2(lpr, poff, (unsigned)Jn * (CLUSTER * 8));
-    }
+    int Jn = __builtin_amdgcn_readlane(vJ, 0);
+    v4i xjn = buf_ld4(xsr, xoff, (unsigned)Jn * (CLUSTER * 16));
+    v2f pjn = buf_ld2(lpr, poff, (unsigned)Jn * (CLUSTER * 8));
     int curI = -1;
     int xix = 0, xiy = 0, xiz = 0, ti = 0;
     float qi = 0.f, qi_raw = 0.f;
@@ -401,17 +382,15 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
     // issue order) then covers only an atomic issued a whole tile earlier.
     unsigned long long pend_m = 0ull, pend_v = 0ull;
     int pend_J = 0;
-    while (lm) {
-      const int k = rfl(__builtin_ctz(lm));
-      lm &= lm - 1u;
+    for (int k = 0; k < ne; k++) {
       const int I = __builtin_amdgcn_readlane(vI, k);
       const int J = Jn;
       const unsigned long long mask = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)vMhi, k) << 32) |
                                       (unsigned)__builtin_amdgcn_readlane((int)vMlo, k);
       const v4i xj = xjn;
       const v2f pj = pjn;
-      if (lm) {
-        Jn = __builtin_amdgcn_readlane(vJ, rfl(__builtin_ctz(lm)));
+      if (k + 1 < ne) {
+        Jn = __builtin_amdgcn_readlane(vJ, k + 1);
         xjn = buf_ld4(xsr, xoff, (unsigned)Jn * (CLUSTER * 16));
         pjn = buf_ld2(lpr, poff, (unsigned)Jn * (CLUSTER * 8));
       }
@@ -439,18 +418,6 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
       const float dy = (float)(int)((unsigned)xj.y - (unsigned)xiy) * sy;
       const float dz = (float)(int)((unsigned)xj.z - (unsigned)xiz) * sz;
       float r2 = dx * dx + dy * dy + dz * dz;
-      if ((rm >> k) & 1u) {
-        // first evaluation since the build: the entry's bound for later steps
-        float m2 = __builtin_amdgcn_inverse_ballot_w64(mask) ? r2 : 3.0e38f;
-        m2 = fminf(m2, __shfl_xor(m2, 1));
-        m2 = fminf(m2, __shfl_xor(m2, 2));
-        m2 = fminf(m2, __shfl_xor(m2, 4));
-        m2 = fminf(m2, __shfl_xor(m2, 8));
-        m2 = fminf(m2, __shfl_xor(m2, 16));
-        m2 = fminf(m2, __shfl_xor(m2, 32));
-        const float dk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vD), k));
-        if (lane == 0) ((GF*)d.pl_dmin)[e0 + k] = fmaxf(sqrtf(m2) - dk, 0.f);
-      }
       // pairs inside the cutoff as a lane mask: lane l's bit of the entry mask is
       // tested in scalar registers, not shifted out per lane
       const unsigned long long actm = mask & __builtin_amdgcn_ballot_w64(r2 < rc2);
@@ -563,7 +530,7 @@ __global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, 
     const long long np = *d.n_pairs;
     const long long e0 = (long long)(w - chunk_pre[s]) * PAIR_CHUNK;
     const long long e1 = e0 + PAIR_CHUNK < np ? e0 + PAIR_CHUNK : np;
-    if (EV) ev.ntl += (int)(e1 - e0);  // every entry (no skip test in this form)
+    if (EV) ev.ntl += (int)(e1 - e0);
     const Slot64* __restrict__ xs = (const Slot64*)d.xs;
     const int* __restrict__ ts = d.ts;
     const double2* __restrict__ lj = d.lj64;

@@ -134,16 +134,6 @@ struct SysDev {
   int *tile_cnt;         // [ceil(n_clusters_cap / 64)] kept entries per 64 consecutive I clusters
   int *pl_i, *pl_j;      // [n_pairs_cap]
   unsigned long long *pl_mask; // [n_pairs_cap]
-  // Skip test of the mixed pair kernel.  pl_dmin[e]: a lower bound, fixed at the
-  // entry's first evaluation, of its pairs' distance at the list build (the
-  // smallest pair distance then, less both clusters' displacement bounds then);
-  // < 0 until that evaluation (the list build writes -1).  clu_disp[c]: the
-  // largest displacement of cluster c's atoms from their build positions, formed
-  // before every pair launch.  A pair's distance now is at least
-  // pl_dmin[e] - clu_disp[I] - clu_disp[J], so an entry where that exceeds the
-  // cutoff holds no pair inside it and is not evaluated.
-  float *pl_dmin;        // [n_pairs_cap]
-  float *clu_disp;       // [n_clusters_cap]
   long long *n_pairs;    // device scalar
   // ---- exclusions ---------------------------------------------------------
   unsigned long long *excl_bits; // [n_atoms]: bit (32 + j - i) set if j excluded, |j-i| <= 31

@@ -86,8 +86,7 @@ EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create
            "fz_dcd_write_frame", "fz_dcd_close", "fz_set_fixed_atoms", "fz_set_com_restraint",
            "fz_deck_com_restraint", "fz_deck_fixed_atoms", "fz_set_neighbor_capacity",
            "fz_pme_convolve_noalloc", "fz_pme_convolve_noalloc_bytes", "fz_set_barostat", "fz_get_barostat",
-           "fz_set_launch", "fz_batch_set_launch", "fz_pair_kernel_time", "fz_set_pair_skip",
-           "fz_batch_set_pair_skip"]
+           "fz_set_launch", "fz_batch_set_launch", "fz_pair_kernel_time"]
 
 MIN_ADAPTIVE_SD, MIN_REF_CG, MIN_REF_SD = 0, 1, 2
 PME_AUTO, PME_GRID, PME_PLANES = 0, 1, 2
@@ -161,8 +160,6 @@ def lib():
         L.fz_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)]
         L.fz_pair_kernel_time.argtypes = [vp, C.c_int, _dp]
         L.fz_batch_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)]
-        L.fz_set_pair_skip.argtypes = [vp, C.c_int]
-        L.fz_batch_set_pair_skip.argtypes = [vp, C.c_int]
         L.fz_get_cluster_layout.argtypes = [vp, _ip, _ip, _ip, C.c_int64, C.POINTER(C.c_int64)]
         L.fz_get_cluster_pairs.argtypes = [vp, _ip, _up, C.c_int64]
         L.fz_get_cluster_pairs.restype = C.c_int64
@@ -419,10 +416,6 @@ class System:
                "fz_set_launch")
         return dict(fft_lines=int(out[0]), pair_grid=int(out[1]), pme_path=int(out[2]))
 
-    def set_pair_skip(self, on):
-        """fz_set_pair_skip: the mixed pair kernel's skip test on or off."""
-        _check(lib().fz_set_pair_skip(self.h, int(bool(on))), "fz_set_pair_skip")
-
     def set_neighbor_capacity(self, pairs_cap=0, clusters_cap=0):
         """Neighbour capacities (0 keeps one); the list held stays the same."""
         _check(lib().fz_set_neighbor_capacity(self.h, int(pairs_cap), int(clusters_cap)), "fz_set_neighbor_capacity")
@@ -491,10 +484,6 @@ class Batch:
         _check(lib().fz_batch_set_launch(self.h, int(fft_lines), int(pair_grid), int(pme_path), _p(out, C.c_int32)),
                "fz_batch_set_launch")
         return dict(fft_lines=int(out[0]), pair_grid=int(out[1]), pme_path=int(out[2]))
-
-    def set_pair_skip(self, on):
-        """fz_batch_set_pair_skip."""
-        _check(lib().fz_batch_set_pair_skip(self.h, int(bool(on))), "fz_batch_set_pair_skip")
 
     def close(self):
         if self.h:

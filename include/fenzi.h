@@ -102,8 +102,8 @@ typedef struct fz_energies {
   int64_t list_builds;    /* neighbour-list builds so far */
   double volume;          /* box volume (A^3) */
   double pressure;        /* (2 KE + tr virial) / 3V in atm (reduce_virial, src/cell_based_kernels.cu:586-589) */
-  int64_t pair_tiles;     /* 8x8 cluster-pair tiles the pair kernel evaluated, this evaluation: list
-                             entries less those the mixed form's skip test left out (fz_set_pair_skip) */
+  int64_t pair_tiles;     /* 8x8 cluster-pair tiles the pair kernel evaluated, this evaluation (one per
+                             list entry); pairs_within_cutoff / (64 pair_tiles) is the in-cutoff lane fraction */
 } fz_energies;
 
 /* ---- errors / context ------------------------------------------------- */
@@ -242,16 +242,6 @@ int fz_autotune(fz_system *sys, int nsteps, double *out);
 #define FZ_PME_GRID 1
 #define FZ_PME_PLANES 2
 int fz_set_launch(fz_system *sys, int fft_lines, int pair_grid, int pme_path, int32_t *in_effect);
-/* Skip test of the mixed-form pair kernel (on by default): a list entry is left
- * out when its smallest pair distance at its first evaluation after the list
- * build, less the largest displacement of either cluster's atoms since the
- * build (then and now), exceeds the cutoff by 1e-3 A.  Such an entry holds no
- * pair inside the cutoff, so results are bitwise the same with the test off
- * (fz_energies.pair_tiles counts the entries evaluated).  Turning it off also
- * keeps the entries first evaluated while it is off until the next list build.
- * No counterpart in the reference (its kernel evaluates every list entry,
- * src/nonbonded_forces.cu:25-270). */
-int fz_set_pair_skip(fz_system *sys, int on);
 
 /* ---- inspection (parity tests) ------------------------------------------------ */
 /* Column grid and cluster layout of the last neighbour build: column of each
@@ -298,8 +288,6 @@ int fz_batch_minimize_ex(fz_batch *b, int nsteps, const fz_min_params *p);
 int fz_batch_autotune(fz_batch *b, int nsteps, double *out);
 /* fz_set_launch for the whole batch. */
 int fz_batch_set_launch(fz_batch *b, int fft_lines, int pair_grid, int pme_path, int32_t *in_effect);
-/* fz_set_pair_skip for the whole batch. */
-int fz_batch_set_pair_skip(fz_batch *b, int on);
 void fz_batch_destroy(fz_batch *b);
 
 /* ---- serving: continuous batching of MD jobs ------------------------------------------ */

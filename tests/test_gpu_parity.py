@@ -423,7 +423,8 @@ def test_launch_shapes_and_pair_timing_leave_the_forces_unchanged(ctx):
     the mixed-form forces do not depend on them (fixed-point sums), the
     reciprocal energy's per-workgroup FP64 partial sums do, at rounding level;
     fz_pair_kernel_time returns a positive time and leaves the system's forces
-    and energies as they were (it evaluates them again after its launches)."""
+    and energies as they were (it evaluates them again after its launches);
+    the energies count one pair-kernel tile per list entry."""
     topo, xyz, params = systems.slab(seed=12)
     s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
     f0, e0 = s.forces()
@@ -437,16 +438,11 @@ def test_launch_shapes_and_pair_timing_leave_the_forces_unchanged(ctx):
     assert ms > 0.0
     f2, e2 = s.forces()
     assert np.array_equal(f1, f2) and e2["potential"] == e1["potential"]
+    # one evaluated tile per list entry
+    assert e2["pair_tiles"] == len(s.cluster_pairs()[0])
     with pytest.raises(fenzi.FenziError):
         s.set_launch(fft_lines=1000)
     s.close()
-
-
-def _skip_system(ctx, cfg, on):
-    topo, xyz, params = cfg
-    s = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
-    s.set_pair_skip(on)
-    return s
 
 
 def _same_energies(a, b):
@@ -457,83 +453,11 @@ def _same_energies(a, b):
             assert v == b[k], k
 
 
-def test_pair_skip_leaves_dynamics_and_minimisation_unchanged(ctx):
-    """The mixed pair kernel leaves out list entries whose pairs cannot have
-    come inside the cutoff since the list build (bound recorded at an entry's
-    first evaluation, per-cluster displacements formed before every launch).
-    Such entries add nothing to the fixed-point sums, so with the test on a
-    trajectory through several list builds (thin skin) and a steepest-descent
-    run (stand-alone evaluations at moved positions, no rebuild per move) are
-    bitwise those with it off, with fewer tiles evaluated.  With it off every
-    list entry is one evaluated tile."""
-    topo, xyz, params = systems.slab(seed=7)
-    cfg = (topo, xyz, dict(params, pairlist=params["cutoff"] + 0.8))
-    on, off = _skip_system(ctx, cfg, True), _skip_system(ctx, cfg, False)
-    tiles_on = tiles_off = 0
-    for s in (on, off):
-        s.reassign_velocities(300.0, 4, 0)
-    for block in range(6):
-        on.step(10)
-        off.step(10)
-        eon, eoff = on.energies(), off.energies()
-        assert np.array_equal(on.get_fixed(), off.get_fixed()), block
-        assert np.array_equal(on.get_state()[1], off.get_state()[1]), block
-        _same_energies(eon, eoff)
-        pairs, _ = off.cluster_pairs()
-        assert eoff["pair_tiles"] == len(pairs)
-        assert eon["pair_tiles"] <= eoff["pair_tiles"]
-        tiles_on += eon["pair_tiles"]
-        tiles_off += eoff["pair_tiles"]
-    assert eon["list_builds"] >= 4
-    assert tiles_on < 0.97 * tiles_off
-    for s in (on, off):
-        s.minimize(30, 0.02)
-    assert np.array_equal(on.get_fixed(), off.get_fixed())
-    f_on, e_on = on.forces()
-    f_off, e_off = off.forces()
-    assert np.array_equal(f_on, f_off)
-    _same_energies(e_on, e_off)
-    on.close()
-    off.close()
-
-
-def test_pair_skip_in_a_batch_matches_alone_without_it(ctx):
-    """A batch runs the skip test's displacement pass on its third stream
-    before the pair launch over the members that keep their lists, and the
-    build writes the rebuilt members' zero bounds; members end bitwise equal to
-    the same systems run alone with the test off."""
-    cfgs = [systems.slab(seed=1), systems.water_box(seed=2),
-            systems.slab(n_chains=3, n_waters=300, L=24.0, seed=3, K=(24, 24, 24))]
-    cfgs = [(t, x, dict(p, pairlist=p["cutoff"] + 0.8)) for t, x, p in cfgs]
-    alone = []
-    for cfg in cfgs:
-        s = _skip_system(ctx, cfg, False)
-        s.reassign_velocities(300.0, 4, 0)
-        s.step(30)
-        alone.append((s.get_fixed(), s.energies()))
-        s.close()
-    mix = []
-    for cfg in cfgs:
-        s = _skip_system(ctx, cfg, False)
-        s.reassign_velocities(300.0, 4, 0)
-        mix.append(s)
-    b = fenzi.Batch(ctx, mix)
-    b.set_pair_skip(True)
-    b.step(30)
-    for s, (u, e) in zip(mix, alone):
-        assert np.array_equal(s.get_fixed(), u)
-        es = s.energies()
-        _same_energies(es, e)
-        assert es["pair_tiles"] <= e["pair_tiles"]
-    b.close()
-
-
-def test_single_large_system_split_matches_a_batch_member(ctx):
-    """From 65,536 atoms one system's dynamics also runs the pair kernel over
-    its kept list beside the list step, as a batch does (the build's own
-    work-item layout is then replaced by the after-build prefix).  The
-    trajectory through several list builds equals, bitwise, the same system's
-    as a member of a batch."""
+def test_large_system_alone_matches_a_batch_member(ctx):
+    """A 66,000-atom system (brick spread, grid FFTs) through several list
+    builds: alone, one pair launch after the list step; in a batch, the pair
+    kernel over its kept list beside the build and over its rebuilt list after
+    it.  The trajectories and energies are bitwise the same."""
     topo, xyz, params = systems.water_box(n_waters=22000, L=88.0, seed=31)
     params = dict(params, K=(90, 90, 90), pairlist=params["cutoff"] + 0.8)
     a = fenzi.System(ctx, topo, params, fenzi.MIXED, xyz=xyz)
