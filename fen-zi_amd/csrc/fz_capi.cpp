@@ -253,6 +253,14 @@ int fz_batch_set_launch(fz_batch* b, int nl, int grid, int pme_path, int32_t* ou
   FZ_GUARD_END
 }
 
+int fz_batch_phase_times(fz_batch* b, int nsteps, double* ms) {
+  FZ_GUARD_BEGIN
+  if (!b || nsteps <= 0 || !ms) return set_error(FZ_ERR_ARG, "invalid argument");
+  FZ_HIP(hipSetDevice(b->ctx->device));
+  return b->eng->phase_times(nsteps, ms);
+  FZ_GUARD_END
+}
+
 int fz_batch_autotune(fz_batch* b, int nsteps, double* out) {
   FZ_GUARD_BEGIN
   if (!b || nsteps <= 0 || !out) return set_error(FZ_ERR_ARG, "invalid argument");

@@ -86,7 +86,7 @@ EXPORTS = ["fz_last_error", "fz_ctx_create", "fz_ctx_destroy", "fz_system_create
            "fz_dcd_write_frame", "fz_dcd_close", "fz_set_fixed_atoms", "fz_set_com_restraint",
            "fz_deck_com_restraint", "fz_deck_fixed_atoms", "fz_set_neighbor_capacity",
            "fz_pme_convolve_noalloc", "fz_pme_convolve_noalloc_bytes", "fz_set_barostat", "fz_get_barostat",
-           "fz_set_launch", "fz_batch_set_launch", "fz_pair_kernel_time"]
+           "fz_set_launch", "fz_batch_set_launch", "fz_pair_kernel_time", "fz_batch_phase_times"]
 
 MIN_ADAPTIVE_SD, MIN_REF_CG, MIN_REF_SD = 0, 1, 2
 PME_AUTO, PME_GRID, PME_PLANES = 0, 1, 2
@@ -157,6 +157,7 @@ def lib():
         L.fz_phase_times.argtypes = [vp, C.c_int, _dp]
         L.fz_autotune.argtypes = [vp, C.c_int, _dp]
         L.fz_batch_autotune.argtypes = [vp, C.c_int, _dp]
+        L.fz_batch_phase_times.argtypes = [vp, C.c_int, _dp]
         L.fz_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)]
         L.fz_pair_kernel_time.argtypes = [vp, C.c_int, _dp]
         L.fz_batch_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int32)]
@@ -477,6 +478,12 @@ class Batch:
         out = np.zeros(4)
         _check(lib().fz_batch_autotune(self.h, int(nsteps), _p(out, C.c_double)), "fz_batch_autotune")
         return dict(column_width=out[0], fft_lines=int(out[1]), pair_grid=int(out[2]), ms_per_step=out[3])
+
+    def phase_times(self, nsteps):
+        """fz_batch_phase_times: mean ms per phase of a batch step (eager launches)."""
+        ms = np.zeros(len(System.PHASES))
+        _check(lib().fz_batch_phase_times(self.h, int(nsteps), _p(ms, C.c_double)), "fz_batch_phase_times")
+        return dict(zip(System.PHASES, ms.tolist()))
 
     def set_launch(self, fft_lines=-1, pair_grid=-1, pme_path=-1):
         """fz_batch_set_launch; returns the shapes in effect."""

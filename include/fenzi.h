@@ -286,6 +286,8 @@ int fz_batch_minimize(fz_batch *b, int nsteps, double max_step);
 int fz_batch_minimize_ex(fz_batch *b, int nsteps, const fz_min_params *p);
 /* fz_autotune over the whole batch (one setting for all members). */
 int fz_batch_autotune(fz_batch *b, int nsteps, double *out);
+/* fz_phase_times over the whole batch: each phase is one launch over every member. */
+int fz_batch_phase_times(fz_batch *b, int nsteps, double *ms);
 /* fz_set_launch for the whole batch. */
 int fz_batch_set_launch(fz_batch *b, int fft_lines, int pair_grid, int pme_path, int32_t *in_effect);
 void fz_batch_destroy(fz_batch *b);

@@ -170,6 +170,10 @@ def test_batch_matches_alone_bitwise(ctx):
     b.step(20)
     for s, ref in zip(mix, alone):
         assert np.array_equal(s.get_fixed(), ref)
+    # per-phase timing of the batch (eager launches) advances every member
+    ph = b.phase_times(2)
+    assert ph["pair"] > 0.0 and all(v >= 0.0 for v in ph.values())
+    assert all(s.energies()["step"] == 22 for s in mix)
     b.close()
 
 
