@@ -180,7 +180,7 @@ int fz_compute_forces(fz_system* s, double* f, fz_energies* e) {
     // pair (slot order) + bonded + gather (atom order), as the integrator adds them
     for (int i = 0; i < n; i++)
       for (int k = 0; k < 3; k++) {
-        const unsigned long long p = F[(size_t)k * ns + slot[i]], b = B[(size_t)k * n + i], g = G[(size_t)k * n + i];
+        const unsigned long long p = F[(size_t)slot[i] * 3 + k], b = B[(size_t)k * n + i], g = G[(size_t)k * n + i];
         double x;
         if (s->sys.mixed) {
           x = (double)(long long)(p + b + g) / 4294967296.0;

@@ -22,13 +22,13 @@ template <bool MIXED>
 __device__ __forceinline__ double take_force(const SysDev& d, int sl, int a, int comp, bool zero) {
   const size_t ia = (size_t)comp * d.n_atoms + a;
   if (MIXED) {
-    unsigned long long* F = (unsigned long long*)d.Fs + (size_t)comp * d.n_slots_cap + sl;
+    unsigned long long* F = (unsigned long long*)d.Fs + (size_t)sl * 3 + comp;
     unsigned long long* B = (unsigned long long*)d.Fa + ia;
     const unsigned long long v = *F + *B + ((const unsigned long long*)d.Fg)[ia];
     if (zero) { *F = 0ull; *B = 0ull; }
     return from_fixed_acc(v);
   } else {
-    double* F = (double*)d.Fs + (size_t)comp * d.n_slots_cap + sl;
+    double* F = (double*)d.Fs + (size_t)sl * 3 + comp;
     double* B = (double*)d.Fa + ia;
     const double v = (*F + *B) + ((const double*)d.Fg)[ia];
     if (zero) { *F = 0.0; *B = 0.0; }
@@ -42,10 +42,10 @@ __device__ __forceinline__ void clear_force(const SysDev& d, int sl, int a) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     if (MIXED) {
-      ((unsigned long long*)d.Fs)[(size_t)c * d.n_slots_cap + sl] = 0ull;
+      ((unsigned long long*)d.Fs)[(size_t)sl * 3 + c] = 0ull;
       ((unsigned long long*)d.Fa)[(size_t)c * d.n_atoms + a] = 0ull;
     } else {
-      ((double*)d.Fs)[(size_t)c * d.n_slots_cap + sl] = 0.0;
+      ((double*)d.Fs)[(size_t)sl * 3 + c] = 0.0;
       ((double*)d.Fa)[(size_t)c * d.n_atoms + a] = 0.0;
     }
   }
@@ -1052,10 +1052,10 @@ __global__ __launch_bounds__(256) void k_zero_forces(const SysDev* __restrict__ 
   const SysDev& d = S[s_];
   const int sl = (blockIdx.x - block_base(pre, nsys, s_)) * 256 + threadIdx.x;
   if (sl >= d.n_slots_cap) return;
-  unsigned long long* F = (unsigned long long*)d.Fs;
-  F[sl] = 0ull;
-  F[d.n_slots_cap + sl] = 0ull;
-  F[2 * d.n_slots_cap + sl] = 0ull;
+  unsigned long long* F = (unsigned long long*)d.Fs + (size_t)sl * 3;
+  F[0] = 0ull;
+  F[1] = 0ull;
+  F[2] = 0ull;
   unsigned long long* B = (unsigned long long*)d.Fa;
   for (int a = sl; a < d.n_atoms; a += d.n_slots_cap) {
     B[a] = 0ull;

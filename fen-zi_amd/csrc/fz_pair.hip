@@ -121,13 +121,13 @@ __device__ __forceinline__ T red8(T v) {
 // j forces of one tile: group g = lane >> 3 holds the sums of j slot g; lanes
 // 0..23 gather (component lane >> 3, slot lane & 7) and issue one atomic each
 template <class Acc, class T>
-__device__ __forceinline__ void store_j(Acc* F, int cap, int J, unsigned long long mask, int lane, T jx, T jy, T jz) {
+__device__ __forceinline__ void store_j(Acc* F, int J, unsigned long long mask, int lane, T jx, T jy, T jz) {
   const int c = lane >> 3, jj = lane & 7;
   const int src = jj << 3;
   const T gx = __shfl(jx, src, 64), gy = __shfl(jy, src, 64), gz = __shfl(jz, src, 64);
   const T v = c == 0 ? gx : (c == 1 ? gy : gz);
   if (lane < 24 && ((mask >> (8 * jj)) & 0xffull)) {
-    Acc* p = F + (size_t)c * cap + J * CLUSTER + jj;
+    Acc* p = F + (size_t)(J * CLUSTER + jj) * 3 + c;
     if constexpr (sizeof(Acc) == 8 && sizeof(T) == 4) atomicAdd((unsigned long long*)p, to_fixed_acc((double)v));
     else unsafeAtomicAdd((double*)p, (double)v);
   }
@@ -180,13 +180,12 @@ __device__ __forceinline__ T red_stride8(T v) {  // sum over lanes sharing lane 
 
 // i forces of cluster I at the end of its run of entries: sums over the lanes
 // sharing lane & 7; lane i + 8c (c < 3) adds component c of slot i
-__device__ __forceinline__ void flush_i(FZ_GLOBAL char* Fb, unsigned cap8, int I, int lane, float fix, float fiy,
-                                        float fiz) {
+__device__ __forceinline__ void flush_i(FZ_GLOBAL char* Fb, int I, int lane, float fix, float fiy, float fiz) {
   const float rx = red_stride8(fix), ry = red_stride8(fiy), rz = red_stride8(fiz);
   const int c = lane >> 3;
   const float v = c == 0 ? rx : (c == 1 ? ry : rz);
   if (lane < 24)
-    atomic_add_g((GU64*)(Fb + (size_t)I * (CLUSTER * 8) + (unsigned)c * cap8 + (unsigned)(lane & 7) * 8u), fixed_f32(v));
+    atomic_add_g((GU64*)(Fb + (size_t)I * (CLUSTER * 24) + (unsigned)(lane & 7) * 24u + (unsigned)c * 8u), fixed_f32(v));
 }
 
 // Sums of three values over the 8 lanes sharing lane >> 3, every lane of the
@@ -231,19 +230,19 @@ __device__ __forceinline__ float erfc_fit(float x, float ex) {
 }
 
 template <class Acc>
-__device__ __forceinline__ void add3(Acc* F, int cap, int slot, double fx, double fy, double fz);
+__device__ __forceinline__ void add3(Acc* F, int slot, double fx, double fy, double fz);
 
 template <>
-__device__ __forceinline__ void add3<unsigned long long>(unsigned long long* F, int cap, int slot, double fx, double fy, double fz) {
-  atomicAdd(F + slot, to_fixed_acc(fx));
-  atomicAdd(F + cap + slot, to_fixed_acc(fy));
-  atomicAdd(F + 2 * cap + slot, to_fixed_acc(fz));
+__device__ __forceinline__ void add3<unsigned long long>(unsigned long long* F, int slot, double fx, double fy, double fz) {
+  atomicAdd(F + (size_t)slot * 3, to_fixed_acc(fx));
+  atomicAdd(F + (size_t)slot * 3 + 1, to_fixed_acc(fy));
+  atomicAdd(F + (size_t)slot * 3 + 2, to_fixed_acc(fz));
 }
 template <>
-__device__ __forceinline__ void add3<double>(double* F, int cap, int slot, double fx, double fy, double fz) {
-  unsafeAtomicAdd(F + slot, fx);
-  unsafeAtomicAdd(F + cap + slot, fy);
-  unsafeAtomicAdd(F + 2 * cap + slot, fz);
+__device__ __forceinline__ void add3<double>(double* F, int slot, double fx, double fy, double fz) {
+  unsafeAtomicAdd(F + (size_t)slot * 3, fx);
+  unsafeAtomicAdd(F + (size_t)slot * 3 + 1, fy);
+  unsafeAtomicAdd(F + (size_t)slot * 3 + 2, fz);
 }
 
 // Energy / virial sums of an evaluation step.  Each lane sums one chunk in FP64
@@ -343,9 +342,8 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
     const int sep = SEP || d.lj_sep;
     const int T = d.n_types;
     FZ_GLOBAL char* Fb = (FZ_GLOBAL char*)d.Fs;
-    const unsigned cap8 = cap * 8u;
     // force atomics: lane 8k + c (c < 3) adds component c of slot k of a cluster
-    const unsigned foff = (unsigned)(lane & 7) * cap8 + (unsigned)(lane >> 3) * 8u;
+    const unsigned foff = (unsigned)(lane >> 3) * 24u + (unsigned)(lane & 7) * 8u;
     const float sx = d.sc32[0], sy = d.sc32[1], sz = d.sc32[2];
     const float rc2 = (float)d.rc2, ron2 = (float)d.ron2;
     // CHARMM switch S(r2) = (rc2 - r2)^2 (rc2 + 2 r2 - 3 ron2) / (rc2 - ron2)^3 in the
@@ -395,13 +393,13 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
         pjn = buf_ld2(lpr, poff, (unsigned)Jn * (CLUSTER * 8));
       }
       if (__builtin_amdgcn_inverse_ballot_w64(pend_m))
-        atomic_add_g((GU64*)(Fb + (size_t)pend_J * (CLUSTER * 8) + foff), pend_v);
+        atomic_add_g((GU64*)(Fb + (size_t)pend_J * (CLUSTER * 24) + foff), pend_v);
       if (I != curI) {
         // new i cluster's data first, then the old one's sums, so that the wait
         // for the loads below does not include the flush's atomic
         const v4i v = buf_ld4(xsr, (unsigned)il * 16u, (unsigned)I * (CLUSTER * 16));
         const v2f pv = buf_ld2(lpr, (unsigned)il * 8u, (unsigned)I * (CLUSTER * 8));
-        if (curI >= 0) flush_i(Fb, cap8, curI, lane, fix, fiy, fiz);
+        if (curI >= 0) flush_i(Fb, curI, lane, fix, fiy, fiz);
         curI = I;
         xix = v.x; xiy = v.y; xiz = v.z;
         qi_raw = __int_as_float(v.w);
@@ -501,8 +499,8 @@ __global__ __launch_bounds__(256) void k_pair_mixed(const SysDev* __restrict__ S
       }
     }
     if (__builtin_amdgcn_inverse_ballot_w64(pend_m))
-      atomic_add_g((GU64*)(Fb + (size_t)pend_J * (CLUSTER * 8) + foff), pend_v);
-    if (curI >= 0) flush_i(Fb, cap8, curI, lane, fix, fiy, fiz);
+      atomic_add_g((GU64*)(Fb + (size_t)pend_J * (CLUSTER * 24) + foff), pend_v);
+    if (curI >= 0) flush_i(Fb, curI, lane, fix, fiy, fiz);
     if (EV) ev.add_chunk(cev);
   }
   if (EV && evs >= 0) ev.flush(S[evs], lane);
@@ -536,7 +534,6 @@ __global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, 
     const double2* __restrict__ lj = d.lj64;
     const int T = d.n_types;
     double* F = (double*)d.Fs;
-    const int cap = d.n_slots_cap;
     const double sx = d.sc[0], sy = d.sc[1], sz = d.sc[2];
     const double rc2 = d.rc2, ron2 = d.ron2, beta = d.beta, swi = d.sw_inv;
     const double ccb = TWO_OVER_SQRT_PI * beta;
@@ -552,7 +549,7 @@ __global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, 
       if (I != curI) {
         if (curI >= 0) {
           double rx = red_stride8(fix), ry = red_stride8(fiy), rz = red_stride8(fiz);
-          if (jl == 0) add3(F, cap, curI * CLUSTER + il, rx, ry, rz);
+          if (jl == 0) add3(F, curI * CLUSTER + il, rx, ry, rz);
         }
         curI = I;
         const Slot64 v = xs[I * CLUSTER + il];
@@ -594,7 +591,7 @@ __global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, 
       const double fx = fs * dx, fy = fs * dy, fz = fs * dz;
       fix -= fx; fiy -= fy; fiz -= fz;
       const double jx = red8(fx), jy = red8(fy), jz = red8(fz);
-      store_j(F, cap, J, mask, lane, jx, jy, jz);
+      store_j(F, J, mask, lane, jx, jy, jz);
       if (EV) {
         if (act) {
           cev[0] += e_lj; cev[1] += e_c; ev.npr++;
@@ -605,7 +602,7 @@ __global__ __launch_bounds__(256) void k_pair_f64(const SysDev* __restrict__ S, 
     }
     if (curI >= 0) {
       double rx = red_stride8(fix), ry = red_stride8(fiy), rz = red_stride8(fiz);
-      if (jl == 0) add3(F, cap, curI * CLUSTER + il, rx, ry, rz);
+      if (jl == 0) add3(F, curI * CLUSTER + il, rx, ry, rz);
     }
     if (EV) ev.add_chunk(cev);
   }

@@ -58,11 +58,11 @@ __global__ __launch_bounds__(256) void k_com_force(const SysDev* __restrict__ S,
   if (MIXED) {
     unsigned long long* F = (unsigned long long*)d.Fs;
 #pragma unroll
-    for (int k = 0; k < 3; k++) atomicAdd(F + k * cap + sl, to_fixed_acc(dr[k]));
+    for (int k = 0; k < 3; k++) atomicAdd(F + (size_t)sl * 3 + k, to_fixed_acc(dr[k]));
   } else {
     double* F = (double*)d.Fs;
 #pragma unroll
-    for (int k = 0; k < 3; k++) unsafeAtomicAdd(F + k * cap + sl, dr[k]);
+    for (int k = 0; k < 3; k++) unsafeAtomicAdd(F + (size_t)sl * 3 + k, dr[k]);
   }
 }
 

@@ -114,7 +114,7 @@ struct SysDev {
   int *atom_slot;        // [n_atoms]
   void *xs;              // packed positions+charge per slot (int4 / longlong4)
   int *ts;               // type per slot
-  void *Fs;              // pair-kernel (and restraint) force sums, 3 x n_slots_cap (x, y, z blocks)
+  void *Fs;              // pair-kernel (and restraint) force sums, [n_slots_cap][3]: a slot's x, y, z adjacent
   void *Fa;              // bonded force sums, 3 x n_atoms, atom order (cleared by the consumer)
   void *Fg;              // PME gather forces, 3 x n_atoms, atom order (stored by every evaluation)
   // ---- binning / search --------------------------------------------------
