@@ -1699,7 +1699,7 @@ int Engine::autotune(int nsteps, double* out) {
     if (rc != FZ_OK || (rc = trial(best_f, nl, best_grid, &ms)) != FZ_OK) break;
     if (ms < best_ms) { best_ms = ms; best_nl = nl; }
   }
-  for (int grid : {512, 1024, 4096, 8192, 16384}) {
+  for (int grid : {512, 1024, 4096, 8192, 16384, 32768}) {
     if (rc != FZ_OK || (rc = trial(best_f, best_nl, grid, &ms)) != FZ_OK) break;
     if (ms < best_ms) { best_ms = ms; best_grid = grid; }
   }
